@@ -1,0 +1,185 @@
+/*
+ * thistle_scatter.h -- C ABI of the MI355X (gfx950) total / elastic j0
+ * X-ray scattering path.
+ *
+ * The library replaces the Fortran module procedure
+ *   main_calculation_mod::total_scattering_calculation
+ * (reference src/main_calculation.f90:13-152, types 1 and 2) and the
+ * routines it calls: maxcoincidence/createtwordm (src/twordms.f90:8-556),
+ * onerdm_creat (src/one_rdm.f90:27-222), read_twordm's consumer
+ * total_scattering_calculation_2 readrdm route (src/main_calculation.f90:
+ * 582-658), variables_total / variables_elastic (src/Variables.f90:8-1281),
+ * tot_integration / elastic_integration and their kernels
+ * (src/Integrals.f90:8-472,1575-1939,4708-5226,5730-5901,6662-6739,
+ * 7037-7184), and unique_total (src/unique_module.f90:90-145).
+ *
+ * Conventions (those of the Fortran dummies):
+ *   - 2-D arrays are column-major: mmod(nmo,ngto) at mmod[i + nmo*j],
+ *     confs(nconf,lconfs), civecs(nconf,nstates), geom(natom,3),
+ *     mat(nmat,4).
+ *   - group[] and every orbital label in mat are 1-based; each group is a
+ *     contiguous run of GTOs.
+ *   - All pointers are host pointers unless a name says _dev.
+ *   - Return value: THISTLE_OK (0) or one of the status codes below; the
+ *     library never exits the process (the reference's `stop`s become
+ *     THISTLE_ELMAX / THISTLE_ELLMAX).
+ *   - Calls are thread-safe for distinct plans; one plan must not be
+ *     executed from two threads at once.
+ */
+#ifndef THISTLE_SCATTER_H
+#define THISTLE_SCATTER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    THISTLE_OK = 0,
+    THISTLE_EINVAL = 1,      /* bad sizes / arguments                      */
+    THISTLE_ELMAX = 2,       /* a GTO with l, m or n > 4 (fill_md_table stop,
+                                src/Variables.f90:681-682,939-940)          */
+    THISTLE_ELLMAX = 3,      /* quartet LLmax + 1 > 20
+                                (src/Integrals.f90:5117-5120)               */
+    THISTLE_EGROUP = 4,      /* a group is not one contiguous run of GTOs    */
+    THISTLE_EDEVICE = 5,     /* HIP or rocBLAS failure                       */
+    THISTLE_ENOMEM = 6,      /* device allocation failed                     */
+    THISTLE_ETYPE = 7        /* calculation type other than 1 (total j0) or
+                                2 (elastic j0)                               */
+};
+
+enum { THISTLE_FP64 = 0, THISTLE_MIXED = 1 };
+
+/* Basis, orbitals, q grid and cutoffs: the non-CI arguments of
+ * total_scattering_calculation (src/main_calculation.f90:13-62). */
+typedef struct ThistleBasis {
+    int32_t ngto, ng, maxl;       /* maxl = maxval(l), sizes the MD tables */
+    const double *ga;             /* [ngto] exponents                      */
+    const int32_t *l, *m, *n;     /* [ngto] cartesian exponents            */
+    const double *xx, *yy, *zz;   /* [ngto] centres, bohr                  */
+    const int32_t *group;         /* [ngto] 1-based primitive-shell label  */
+    int32_t nmo;                  /* rows of mmod                          */
+    const double *mmod;           /* (nmo, ngto) MO coefficients           */
+    int32_t nq;
+    const double *q;              /* [nq] |q|, a.u.                        */
+    double cutoffz, cutoffmd, cutoffcentre;
+} ThistleBasis;
+
+typedef struct ThistleOpts {
+    int32_t precision;            /* THISTLE_FP64 or THISTLE_MIXED        */
+    double rdm_cutoff;            /* |Gamma| selection threshold of
+                                     createtwordm; the reference uses 1e-30 */
+    void *stream;                 /* hipStream_t, or NULL for the library's */
+    int32_t device;               /* HIP device ordinal                    */
+    int32_t waves;                /* persistent waves for the quartet
+                                     kernel; 0 = library default           */
+} ThistleOpts;
+
+/* Drop-in for main_calculation_mod::total_scattering_calculation, types 1
+ * (total j0) and 2 (elastic j0): src/main_calculation.f90:13-15,97-152.
+ * Zn/geom/state3 are accepted for signature parity and unused (as in the
+ * reference for these types).  q_abs_out receives zeros (:97-98). */
+int32_t thistle_total_scattering_calculation(
+    int32_t type, int32_t natom, const double *Zn, const double *geom,
+    int32_t state1, int32_t state2, int32_t state3, const ThistleBasis *basis,
+    int32_t nconf, int32_t lconfs, int32_t nstates, const int32_t *confs,
+    const double *civecs, const ThistleOpts *opts, double *q_abs_out,
+    double *result_out);
+
+/* readrdm route of total_scattering_calculation_2 (src/main_calculation.f90:
+ * 582-658 after read_twordm, src/one_rdm.f90:2112-2146): total j0 from a
+ * 2-RDM given as rows (p,s,q,r) with values. */
+int32_t thistle_total_from_twordm(const ThistleBasis *basis, int64_t nmat,
+                                  const int32_t *mat, const double *total,
+                                  const ThistleOpts *opts, double *result_out);
+
+/* Elastic j0 from a given 1-RDM (norb x norb, column-major): the body of
+ * case 2 after onerdm_creat (src/main_calculation.f90:137-152). */
+int32_t thistle_elastic_from_onerdm(const ThistleBasis *basis, int32_t norb,
+                                    const double *onerdm,
+                                    const ThistleOpts *opts, double *result_out);
+
+/* ---- 2-RDM / 1-RDM construction (host, reference accumulation order) --- */
+
+/* maxcoincidence + createtwordm accumulation (src/twordms.f90:8-512):
+ * dense twordm(p,q,r,s), norb = lconfs/2, column-major norb^4. */
+int32_t thistle_twordm_dense(const int32_t *confs, int32_t nconf,
+                             int32_t lconfs, const double *civecs,
+                             int32_t nstates, int32_t state1, int32_t state2,
+                             double *twordm);
+/* Selection loop of createtwordm (src/twordms.f90:515-550): rows (p,s,q,r)
+ * with value twordm(p,q,r,s), |value| >= cutoff, p,q,r,s loop order.
+ * mat is (cap,4) column-major.  Returns the count (pass cap=0 to count). */
+int64_t thistle_twordm_select(const double *twordm, int32_t norb,
+                              double cutoff, int32_t *mat, double *total,
+                              int64_t cap);
+/* onerdm_creat / createonerdm (src/one_rdm.f90:27-222). */
+int32_t thistle_onerdm(const int32_t *confs, int32_t nconf, int32_t lconfs,
+                       const double *civecs, int32_t nstates, int32_t state1,
+                       int32_t state2, double *onerdm);
+
+/* ---- device stages, exposed for parity checks ------------------------- */
+
+/* Reference quartet enumeration (tot_integration's five nests,
+ * src/Integrals.f90:79-82,178-180,258-260,340-341,416) computed on the GPU.
+ * Writes 6 int32 per quartet (gi,gj,gk,gr 1-based, multiplicity, branch
+ * 1 = h < cutoffcentre) in reference order.  Returns Q_ref or -status. */
+int64_t thistle_quartet_list(const ThistleBasis *basis, int32_t *out,
+                             int64_t cap, const ThistleOpts *opts);
+
+/* Screened quartet list: the entries above whose 2-RDM block holds some
+ * |ztot| >= cutoffz, in reference order (stable prefix-scan compaction).
+ * type 1 takes the 2-RDM rows, type 2 the 1-RDM. */
+int64_t thistle_quartet_list_screened(int32_t type, const ThistleBasis *basis,
+                                      int64_t nmat, const int32_t *mat,
+                                      const double *total, int32_t norb,
+                                      const double *onerdm, int32_t *out,
+                                      int64_t cap, const ThistleOpts *opts);
+
+/* unique_total (src/unique_module.f90:90-145) on the GPU: first-occurrence
+ * unique rows of matrix(nrow,ncol) (column-major, ncol <= 8) with summed
+ * totals in row order; iuni/irec 1-based as src/UniqueModule.jl:156-168.
+ * rmat is (nrow,ncol) column-major capacity; returns the unique count. */
+int64_t thistle_unique_total(const int32_t *matrix, int64_t nrow,
+                             int32_t ncol, const double *total, int32_t *rmat,
+                             double *rtot, int32_t *iuni, int32_t *irec,
+                             const ThistleOpts *opts);
+
+/* Pair tables of variables_total (src/Variables.f90:1113-1150): for each
+ * group pair gi <= gj (pair index p in row-major upper-triangle order)
+ * the Hermite tables E[p][axis][la][lb][t] (strides (maxl+1)^2(2maxl+1)),
+ * centres P[p][3] and e12[p][nq]. */
+int32_t thistle_pair_tables(const ThistleBasis *basis, double *E, double *P,
+                            double *e12, const ThistleOpts *opts);
+
+/* ---- plan API: inputs resident in HBM across repeated executions ------ */
+
+typedef struct ThistlePlan ThistlePlan;
+
+/* type 1: mat/total rows; type 2: onerdm (norb x norb). */
+int32_t thistle_plan_create(int32_t type, const ThistleBasis *basis,
+                            int64_t nmat, const int32_t *mat,
+                            const double *total, int32_t norb,
+                            const double *onerdm, const ThistleOpts *opts,
+                            ThistlePlan **plan);
+/* Runs every device stage (pair tables, enumeration, 2-RDM transform,
+ * contraction + Bessel series, reduction); result_out may be NULL (the
+ * result then stays on the device, retrievable by thistle_plan_result). */
+int32_t thistle_plan_execute(ThistlePlan *plan, double *result_out);
+int32_t thistle_plan_result(ThistlePlan *plan, double *result_out);
+/* Last execution's device time per stage in ms: [pair tables, enumeration,
+ * transform, quartet kernel, reduction], measured with hipEvents on the
+ * plan's stream.  Also the number of quartets, waves and the units
+ * (quartets x q points) of the last execution. */
+int32_t thistle_plan_timings(ThistlePlan *plan, double *ms5, int64_t *nquart,
+                             int64_t *units);
+void thistle_plan_destroy(ThistlePlan *plan);
+
+/* Library version string. */
+const char *thistle_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

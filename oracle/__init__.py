@@ -1,0 +1,234 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker.  See oracle.h for the restated
+reference routines and their array conventions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+_i = C.c_int
+_d = C.c_double
+_p = C.c_void_p
+
+
+def _declare(L):
+    L.orc_set_p0.argtypes = [_p, _i, _p, _i]
+    L.orc_pair_tables.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
+                                  _p, _i, _p, _p, _p, _p, _p, _p, _p]
+    L.orc_twordm_dense.argtypes = [_p, _i, _i, _p, _i, _i, _i, _p]
+    L.orc_twordm_select.argtypes = [_p, _i, _d, _p, _p, _i]
+    L.orc_twordm_select.restype = _i
+    L.orc_pair_phase.argtypes = [_p, _i, _i, _i, _i, _p, _p]
+    L.orc_onerdm.argtypes = [_p, _i, _i, _p, _i, _i, _i, _p]
+    L.orc_elastic_z.argtypes = [_p, _i, _p, _i, _i, _p]
+    L.orc_quartet_list.argtypes = [_i, _p, _p, _p, _d, _p]
+    L.orc_quartet_list.restype = C.c_int64
+    L.orc_total_j0.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                               _i, _p, _p, _i, _p, _i, _d, _d, _d, _i, _i, _p]
+    L.orc_total_j0.restype = _i
+    L.orc_elastic_j0.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
+                                 _p, _p, _i, _d, _d, _d, _i, _p]
+    L.orc_elastic_j0.restype = _i
+    L.orc_quartet_series.argtypes = [_p, _p, _p, _p, _p, _i, _i, _p, _p, _p,
+                                     _i, _i, _i, _i, _d, _d, _d, _d, _p, _d,
+                                     _d, _p, _i, _i, _p, _p]
+    L.orc_dphrec.argtypes = [_p, _p, _i, _i, _i]
+    L.orc_bessels0rr.argtypes = [_p, _i, _p, _i, _d, _p]
+    L.orc_unique_total.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
+    L.orc_unique_total.restype = _i
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_p) if a is not None else None
+
+
+def _I(a, order="C"):
+    return np.require(np.asarray(a), dtype=np.int32, requirements=[order])
+
+
+def _D(a, order="C"):
+    return np.require(np.asarray(a), dtype=np.float64, requirements=[order])
+
+
+def set_p0(lmax4, q):
+    q = _D(q)
+    D = lmax4 + 1
+    P0 = np.zeros((D, D, D, len(q)))
+    lib().orc_set_p0(_ptr(P0), lmax4, _ptr(q), len(q))
+    return P0
+
+
+def pair_tables(l, m, n, group, ga, xx, yy, zz, q, maxl=None):
+    l, m, n, group = _I(l), _I(m), _I(n), _I(group)
+    ga, xx, yy, zz, q = _D(ga), _D(xx), _D(yy), _D(zz), _D(q)
+    ngto = len(l)
+    ng = int(group.max())
+    maxl = int(l.max()) if maxl is None else maxl
+    L1, T1 = maxl + 1, 2 * maxl + 1
+    dd = [np.zeros((ng, ng, L1, L1, T1)) for _ in range(3)]
+    P = [np.zeros((ng, ng)) for _ in range(3)]
+    e12 = np.zeros((ng, ng, len(q)))
+    lib().orc_pair_tables(ngto, ng, maxl, _ptr(l), _ptr(m), _ptr(n), _ptr(group),
+                          _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(q), len(q),
+                          _ptr(dd[0]), _ptr(dd[1]), _ptr(dd[2]), _ptr(P[0]),
+                          _ptr(P[1]), _ptr(P[2]), _ptr(e12))
+    return dd, P, e12
+
+
+def twordm_dense(confs, civs, state1=1, state2=1):
+    confs = _I(confs, "F")
+    civs = _D(civs, "F")
+    nconf, lconfs = confs.shape
+    norb = lconfs // 2
+    tw = np.zeros((norb,) * 4, order="F")
+    lib().orc_twordm_dense(_ptr(confs), nconf, lconfs, _ptr(civs), civs.shape[1],
+                           state1, state2, _ptr(tw))
+    return tw
+
+
+def twordm_select(tw, cutoff=1e-30):
+    tw = _D(tw, "F")
+    norb = tw.shape[0]
+    cnt = lib().orc_twordm_select(_ptr(tw), norb, cutoff, None, None, 0)
+    mat = np.zeros((cnt, 4), dtype=np.int32, order="F")
+    tot = np.zeros(cnt)
+    lib().orc_twordm_select(_ptr(tw), norb, cutoff, _ptr(mat), _ptr(tot), cnt)
+    return np.ascontiguousarray(mat), tot
+
+
+def createtwordm(confs, civs, state1=1, state2=1, cutoff=1e-30):
+    return twordm_select(twordm_dense(confs, civs, state1, state2), cutoff)
+
+
+def pair_phase(confs, c1, c2):
+    confs = _I(confs, "F")
+    ep = np.zeros(1, dtype=np.int32)
+    nd = np.zeros(1, dtype=np.int32)
+    lib().orc_pair_phase(_ptr(confs), confs.shape[0], confs.shape[1], c1, c2,
+                         _ptr(ep), _ptr(nd))
+    return int(ep[0]), int(nd[0])
+
+
+def onerdm(confs, civs, state1=1, state2=1):
+    confs = _I(confs, "F")
+    civs = _D(civs, "F")
+    norb = confs.shape[1] // 2
+    g = np.zeros((norb, norb), order="F")
+    lib().orc_onerdm(_ptr(confs), confs.shape[0], confs.shape[1], _ptr(civs),
+                     civs.shape[1], state1, state2, _ptr(g))
+    return g
+
+
+def elastic_z(g, mmod):
+    g = _D(g, "F")
+    mmod = _D(mmod, "F")
+    nmo, ngto = mmod.shape
+    Z = np.zeros((ngto, ngto), order="F")
+    lib().orc_elastic_z(_ptr(g), g.shape[0], _ptr(mmod), nmo, ngto, _ptr(Z))
+    return Z
+
+
+def quartet_list(px, py, pz, cutoffcentre):
+    px, py, pz = _D(px), _D(py), _D(pz)
+    ng = px.shape[0]
+    Q = lib().orc_quartet_list(ng, _ptr(px), _ptr(py), _ptr(pz), cutoffcentre, None)
+    out = np.zeros((Q, 6), dtype=np.int32)
+    lib().orc_quartet_list(ng, _ptr(px), _ptr(py), _ptr(pz), cutoffcentre, _ptr(out))
+    return out
+
+
+def total_j0(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q,
+             cutoffz=1e-15, cutoffmd=1e-20, cutoffcentre=0.001, zmode=1,
+             nthreads=0, maxl=None):
+    l, m, n, group = _I(l), _I(m), _I(n), _I(group)
+    ga, xx, yy, zz, q = _D(ga), _D(xx), _D(yy), _D(zz), _D(q)
+    mmod = _D(mmod, "F")
+    mat = _I(mat, "F")
+    total = _D(total)
+    ngto = len(l)
+    ng = int(group.max())
+    maxl = int(l.max()) if maxl is None else maxl
+    out = np.zeros(len(q))
+    rc = lib().orc_total_j0(ngto, ng, maxl, _ptr(l), _ptr(m), _ptr(n), _ptr(group),
+                            _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(mmod),
+                            mmod.shape[0], _ptr(mat), _ptr(total), len(total),
+                            _ptr(q), len(q), cutoffz, cutoffmd, cutoffcentre,
+                            zmode, nthreads, _ptr(out))
+    if rc:
+        raise RuntimeError(f"oracle total_j0 status {rc}")
+    return out
+
+
+def elastic_j0(l, m, n, group, ga, xx, yy, zz, Z, q, cutoffz=1e-15,
+               cutoffmd=1e-20, cutoffcentre=0.001, nthreads=0, maxl=None):
+    l, m, n, group = _I(l), _I(m), _I(n), _I(group)
+    ga, xx, yy, zz, q = _D(ga), _D(xx), _D(yy), _D(zz), _D(q)
+    Z = _D(Z, "F")
+    ngto = len(l)
+    ng = int(group.max())
+    maxl = int(l.max()) if maxl is None else maxl
+    out = np.zeros(len(q))
+    rc = lib().orc_elastic_j0(ngto, ng, maxl, _ptr(l), _ptr(m), _ptr(n), _ptr(group),
+                              _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(Z),
+                              _ptr(q), len(q), cutoffz, cutoffmd, cutoffcentre,
+                              nthreads, _ptr(out))
+    if rc:
+        raise RuntimeError(f"oracle elastic_j0 status {rc}")
+    return out
+
+
+def dphrec(x, lmax1, leps1=100):
+    x = _D(x)
+    out = np.zeros((len(x), leps1))
+    lib().orc_dphrec(_ptr(x), _ptr(out), leps1, lmax1, len(x))
+    return out  # out[iq, k] = RBES(k+1, iq)
+
+
+def bessels0rr(order, mu, H, h_saved):
+    mu = _D(mu)
+    hs = np.zeros(20)
+    hs[:len(h_saved)] = h_saved
+    out = np.zeros(len(mu))
+    lib().orc_bessels0rr(_ptr(out), order, _ptr(mu), len(mu), H, _ptr(hs))
+    return out
+
+
+def unique_total(matrix, total):
+    matrix = _I(matrix, "F")
+    total = _D(total)
+    nrow, ncol = matrix.shape
+    rmat = np.zeros((nrow, ncol), dtype=np.int32, order="F")
+    rtot = np.zeros(nrow)
+    iuni = np.zeros(nrow, dtype=np.int32)
+    irec = np.zeros(nrow, dtype=np.int32)
+    cnt = lib().orc_unique_total(_ptr(matrix), nrow, ncol, _ptr(total), _ptr(rmat),
+                                 _ptr(rtot), _ptr(iuni), _ptr(irec))
+    rm = np.zeros((cnt, ncol), dtype=np.int32, order="F")
+    lib().orc_unique_total(_ptr(matrix), nrow, ncol, _ptr(total), _ptr(rm), None,
+                           None, None)
+    return np.ascontiguousarray(rm), rtot[:cnt].copy(), iuni[:cnt].copy(), irec

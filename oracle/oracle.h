@@ -1,0 +1,122 @@
+/*
+ * CPU oracle for the THISTLE total / elastic j0 scattering path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * Fortran algorithm (KirranderGroup/thistle, src/Integrals.f90,
+ * src/Variables.f90, src/twordms.f90, src/one_rdm.f90,
+ * src/unique_module.f90).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker.  The
+ * product path (thistle_amd/) never calls it.
+ *
+ * Array conventions (all match the Fortran dummies of the reference):
+ *   - 2-D inputs are column-major: mmod(nmo,ngto) at mmod[i + nmo*j],
+ *     confs(nconf,lconfs) at confs[c + nconf*j], civs(nconf,nstates).
+ *   - group[] is 1-based and each group is contiguous.
+ *   - MD tables dd[((((gi*ng+gj)*(L1)+la)*L1+lb)*T1 + t] with L1 = maxl+1,
+ *     T1 = 2*maxl+1: Hermite coefficient E_t^{la,lb} of the pair whose first
+ *     GTO lies in group gi (0-based) and second in gj (reference Ddx(t+1,
+ *     la+1, lb+1, gj+1, gi+1)).
+ *   - e12[(gi*ng+gj)*nq + iq], px/py/pz[gi*ng+gj].
+ *   - P0[((L*D+M)*D+N)*nq + iq], D = 4*maxl+1.
+ *
+ * Parity pins for each function are listed in tests/test_oracle.py.
+ */
+#ifndef THISTLE_ORACLE_H
+#define THISTLE_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* set_P0 / P_LMN, src/Integrals.f90:3001-3031,3102-3135 */
+void orc_set_p0(double *P0, int lmax4, const double *q, int nq);
+
+/* fill_md_table + variables_total pair data, src/Variables.f90:8-946,
+ * 1113-1150.  Outputs ddx/ddy/ddz, px/py/pz, e12. */
+void orc_pair_tables(int ngto, int ng, int maxl, const int *l, const int *m,
+                     const int *n, const int *group, const double *ga,
+                     const double *xx, const double *yy, const double *zz,
+                     const double *q, int nq, double *ddx, double *ddy,
+                     double *ddz, double *px, double *py, double *pz,
+                     double *e12);
+
+/* maxcoincidence + createtwordm, src/twordms.f90:8-81,83-172,175-556.
+ * Writes the dense twordm(p,q,r,s) (column-major, norb^4). */
+void orc_twordm_dense(const int *confs, int nconf, int lconfs,
+                      const double *civs, int nstates, int state1, int state2,
+                      double *twordm);
+/* Selection loop of createtwordm (src/twordms.f90:522-550): rows
+ * (p,s,q,r) with value twordm(p,q,r,s), |value| >= cutoff, in p,q,r,s loop
+ * order.  mat is (count,4) column-major (1-based orbital labels).  Returns
+ * the count; pass mat=NULL/total=NULL to count only. */
+int orc_twordm_select(const double *twordm, int norb, double cutoff,
+                      int *mat, double *total, int cap);
+
+/* maxcoincidence ep2 / ndiff for one ordered pair (src/twordms.f90:8-81). */
+void orc_pair_phase(const int *confs, int nconf, int lconfs, int c1, int c2,
+                    int *ep2, int *ndiff);
+
+/* onerdm_creat / createonerdm, src/one_rdm.f90:27-46,109-222. */
+void orc_onerdm(const int *confs, int nconf, int lconfs, const double *civs,
+                int nstates, int state1, int state2, double *onerdm);
+
+/* variables_elastic Z, src/Variables.f90:1199-1216.  Z(ngto,ngto) col-major. */
+void orc_elastic_z(const double *onerdm, int norb, const double *mmod, int nmo,
+                   int ngto, double *Z);
+
+/* Reference quartet enumeration (tot_integration nests,
+ * src/Integrals.f90:79-82,178-180,258-260,340-341,416).  Returns Q_ref; when
+ * out != NULL writes 6 int32 per quartet: gi,gj,gk,gr (1-based), mult, branch
+ * (1 = h < cutoffcentre -> pzero). */
+int64_t orc_quartet_list(int ng, const double *px, const double *py,
+                         const double *pz, double cutoffcentre, int32_t *out);
+
+/* Full drivers.  type 1 = total j0 (tot_integration), type 2 = elastic j0
+ * (elastic_integration).
+ *   zmode 0: per-quartet contraction restating the dgemm calls (needs the
+ *            nmat x ngto^2 z1/z2 arrays; small systems only);
+ *   zmode 1: primitive-pair 2-RDM assembled once from the MO-pair matrix.
+ * For type 1 pass mat(nmat,4) col-major + total; for type 2 pass Z.
+ * tsi receives S(q).  nthreads <= 0 uses the OpenMP default. */
+int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
+                 const int *n, const int *group, const double *ga,
+                 const double *xx, const double *yy, const double *zz,
+                 const double *mmod, int nmo, const int *mat,
+                 const double *total, int nmat, const double *q, int nq,
+                 double cutoffz, double cutoffmd, double cutoffcentre,
+                 int zmode, int nthreads, double *tsi);
+int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
+                   const int *n, const int *group, const double *ga,
+                   const double *xx, const double *yy, const double *zz,
+                   const double *Z, const double *q, int nq, double cutoffz,
+                   double cutoffmd, double cutoffcentre, int nthreads,
+                   double *tsi);
+
+/* Per-quartet pieces for targeted parity checks. */
+/* tot_integral_k_ijkr with an explicit ztot block ztot[((a*sj+b)*sk+c)*sr+d]
+ * (a in gi, b in gj, c in gk, d in gr): writes h_saved[20] and F[nq]. */
+void orc_quartet_series(const int *l, const int *m, const int *n,
+                        const int *gstart, const int *gcount, int maxl, int ng,
+                        const double *ddx, const double *ddy, const double *ddz,
+                        int gi, int gj, int gk, int gr, double hx, double hy,
+                        double hz, double h, const double *ztot,
+                        double cutoffz, double cutoffmd, const double *q,
+                        int nq, int elastic, double *h_saved, double *F);
+
+/* dphrec, src/Integrals.f90:7147-7184: rbes(leps1,nq) col-major. */
+void orc_dphrec(const double *x, double *rbes, int leps1, int lmax1, int nq);
+/* bessels0rr, src/Integrals.f90:6662-6739. */
+void orc_bessels0rr(double *hsum, int order, const double *mu, int nq,
+                    double H, const double *h_saved);
+
+/* unique_total, src/unique_module.f90:90-145 (+ iuni/irec of
+ * src/UniqueModule.jl:156-168).  matrix(nrow,ncol) col-major.  Returns the
+ * unique count; rmat(cnt,ncol) col-major, rtot(cnt), iuni(cnt) and
+ * irec(nrow) 1-based. */
+int orc_unique_total(const int *matrix, int nrow, int ncol, const double *total,
+                     int *rmat, double *rtot, int *iuni, int *irec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

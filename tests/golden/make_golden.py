@@ -1,0 +1,143 @@
+"""Write golden vectors from the reference's own compiled kernels.
+
+Runs in the development container only (it reads the reference sources
+through oracle/_ref).  For each case it generates the seeded synthetic
+inputs, reads them back through thistle_amd.readers, computes the setup
+tables with the oracle (the reference's src/Variables.f90 does not compile),
+then runs oracle/_ref/ref_driver, which calls the reference's set_P0,
+maxcoincidence, createtwordm, tot_integration, onerdm_creat,
+elastic_integration and unique_total.  Inputs and outputs are saved as
+compressed .npz fixtures beside this script.
+
+    python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from thistle_amd import driver, synth  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+CASES = {
+    # H2, 2 s + 1 p contracted shells of 3 primitives per atom, CAS(2,2), 100 q
+    "h2": lambda: synth.make_h2(),
+    # 3 atoms, s+p uncontracted, 1 core + CAS(4,4), 32 q
+    "sp3": lambda: synth.make_small(seed=11, natom=3, spec={"s": 2, "p": 1},
+                                    nact=4, nelec=4, ncore=1, nq=32, qmax=8.0),
+    # 2 atoms with d shells, 1 core + CAS(2,2), 24 q
+    "spd2": lambda: synth.make_small(seed=12, natom=2, spec={"s": 1, "p": 1, "d": 1},
+                                     nact=2, nelec=2, ncore=1, nq=24, qmax=6.0),
+    # 2 atoms with an f shell, CAS(2,2), 16 q
+    "f2": lambda: synth.make_small(seed=13, natom=2, spec={"s": 1, "f": 1},
+                                   nact=2, nelec=2, ncore=0, nq=16, qmax=5.0),
+    # X-Y-X, 4s2p2d per atom (66 functions), 3 core + CAS(4,4), 100 q
+    "xyx_cas44": lambda: synth.make_xyx(nact=4, nelec=4, nq=100),
+}
+
+
+def case_inputs(name):
+    w = CASES[name]()
+    with tempfile.TemporaryDirectory() as d:
+        mld, pun = synth.write_inputs(w, d)
+        inp = driver.prepare_inputs(mld, pun, w.ncore)
+    return w, inp
+
+
+def write_record(path, inp, q, cutoffs, dolist=1):
+    maxl = inp.maxl
+    dd, P, e12 = oracle.pair_tables(inp.l, inp.m, inp.n, inp.group, inp.ga, inp.xx,
+                                    inp.yy, inp.zz, q, maxl)
+    i8 = lambda *v: np.array(v, dtype=np.int32).tobytes()  # noqa: E731
+    with open(path, "wb") as f:
+        f.write(i8(inp.ngto, inp.ng, len(q), maxl, inp.mmod.shape[0],
+                   inp.confs.shape[0], inp.confs.shape[1], inp.civs.shape[1], 1, 1,
+                   dolist))
+        f.write(np.array(cutoffs, dtype=np.float64).tobytes())
+        for a in (inp.l, inp.m, inp.n, inp.group):
+            f.write(np.asarray(a, dtype=np.int32).tobytes())
+        for a in (inp.ga, inp.xx, inp.yy, inp.zz):
+            f.write(np.asarray(a, dtype=np.float64).tobytes())
+        f.write(np.asfortranarray(inp.mmod).tobytes(order="F"))
+        f.write(np.asarray(q, dtype=np.float64).tobytes())
+        f.write(np.asfortranarray(inp.confs.astype(np.int32)).tobytes(order="F"))
+        f.write(np.asfortranarray(inp.civs).tobytes(order="F"))
+        # px(ng,ng) column-major: px(gi,gj) at gi + ng*gj; ours is [gi, gj]
+        for a in P:
+            f.write(np.asfortranarray(a).tobytes(order="F"))
+        # Ddx(t, la, lb, gj, gi) column-major == C order [gi, gj, lb, la, t];
+        # ours is [gi, gj, la, lb, t]
+        for a in dd:
+            f.write(np.ascontiguousarray(np.transpose(a, (0, 1, 3, 2, 4))).tobytes(order="C"))
+        # e12(q, gi, gj): ours is [gi, gj, q]
+        f.write(np.ascontiguousarray(np.transpose(e12, (1, 0, 2))).tobytes(order="C"))
+
+
+def read_result(path, nq, ngto, ncol=4, D=None):
+    with open(path, "rb") as f:
+        nmat, maxnmo, cnt = np.frombuffer(f.read(12), dtype=np.int32)
+        mat = np.frombuffer(f.read(16 * nmat), dtype=np.int32).reshape(4, nmat).T
+        total = np.frombuffer(f.read(8 * nmat), dtype=np.float64)
+        res1 = np.frombuffer(f.read(8 * nq), dtype=np.float64)
+        res2 = np.frombuffer(f.read(8 * nq), dtype=np.float64)
+        orm = np.frombuffer(f.read(8 * maxnmo * maxnmo), dtype=np.float64).reshape(maxnmo, maxnmo).T
+        zel = np.frombuffer(f.read(8 * ngto * ngto), dtype=np.float64).reshape(ngto, ngto).T
+        rmat = np.frombuffer(f.read(4 * ncol * cnt), dtype=np.int32).reshape(ncol, cnt).T
+        rtot = np.frombuffer(f.read(8 * cnt), dtype=np.float64)
+        p0 = np.frombuffer(f.read(), dtype=np.float64)
+    return dict(mat=mat.copy(), total=total.copy(), S_total=res1.copy(),
+                S_elastic=res2.copy(), onerdm=orm.copy(), Z=zel.copy(),
+                rmat=rmat.copy(), rtot=rtot.copy(), P0=p0.copy())
+
+
+def run_case(name, cutoffs=(1e-15, 1e-20, 0.001)):
+    w, inp = case_inputs(name)
+    q = w.q
+    with tempfile.TemporaryDirectory() as d:
+        rin, rout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        write_record(rin, inp, q, cutoffs)
+        env = dict(os.environ, OMP_NUM_THREADS="8", MKL_NUM_THREADS="1")
+        subprocess.run([REF, rin, rout], cwd=d, check=True, env=env,
+                       stdout=subprocess.DEVNULL)
+        ref = read_result(rout, len(q), inp.ngto)
+    return w, inp, q, ref
+
+
+def save(name, inp, q, ref, cutoffs=(1e-15, 1e-20, 0.001)):
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(
+        path, ga=inp.ga, l=inp.l, m=inp.m, n=inp.n, xx=inp.xx, yy=inp.yy, zz=inp.zz,
+        mmod=inp.mmod, group=inp.group, confs=inp.confs.astype(np.int8),
+        civs=inp.civs, q=q, cutoffs=np.array(cutoffs),
+        mat=ref["mat"].astype(np.int16), total=ref["total"], S_total=ref["S_total"],
+        S_elastic=ref["S_elastic"], onerdm=ref["onerdm"], rmat=ref["rmat"].astype(np.int16),
+        rtot=ref["rtot"], P0_sum=np.array([ref["P0"].sum(), np.abs(ref["P0"]).sum()]))
+    return path
+
+
+def main(names):
+    for name in names:
+        w, inp, q, ref = run_case(name)
+        path = save(name, inp, q, ref)
+        mat, tot = oracle.createtwordm(inp.confs, inp.civs)
+        S = oracle.total_j0(inp.l, inp.m, inp.n, inp.group, inp.ga, inp.xx, inp.yy,
+                            inp.zz, inp.mmod, mat, tot, q, zmode=0)
+        rel = np.max(np.abs(S - ref["S_total"]) / np.abs(ref["S_total"]))
+        print(f"{name}: ngto={inp.ngto} ng={inp.ng} nmat={len(tot)} "
+              f"mat-equal={np.array_equal(mat, ref['mat'])} "
+              f"total-maxrel={np.max(np.abs(tot - ref['total']) / np.abs(ref['total'])):.2e} "
+              f"S_total-maxrel={rel:.2e} size={os.path.getsize(path)}B", flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or list(CASES))

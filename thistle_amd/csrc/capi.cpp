@@ -1,0 +1,709 @@
+// C ABI and execution plan of the MI355X scattering path.
+//
+// A plan owns every device buffer of one (basis, 2-RDM, q grid) problem.
+// thistle_plan_execute runs the device stages on the plan's stream:
+//   1. pair tables   (k_pair_info, k_pair_E, k_e12)
+//   2. enumeration   (k_enumerate: the reference's five nests)
+//   3. transform     (unique rows -> MO-pair matrix Gs -> T -> Z = T^T Gs T
+//                     with two rocBLAS GEMMs on the matrix cores), or the
+//                     elastic Z of the 1-RDM
+//   4. quartet kernel (k_series: contraction + Bessel series per quartet)
+//   5. reduction + stable compaction of the screened quartet list
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "thistle_internal.hpp"
+
+#define HCHECK(x)                                                     \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) {                                       \
+            std::fprintf(stderr, "thistle: %s failed: %s (%s:%d)\n", #x, \
+                         hipGetErrorString(e_), __FILE__, __LINE__);  \
+            return THISTLE_EDEVICE;                                   \
+        }                                                             \
+    } while (0)
+
+#define BCHECK(x)                                                       \
+    do {                                                                \
+        rocblas_status s_ = (x);                                        \
+        if (s_ != rocblas_status_success) {                             \
+            std::fprintf(stderr, "thistle: %s failed: %d (%s:%d)\n", #x, \
+                         (int)s_, __FILE__, __LINE__);                  \
+            return THISTLE_EDEVICE;                                     \
+        }                                                               \
+    } while (0)
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+int dalloc(DevBuf& b, size_t bytes) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = bytes;
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        b.p = nullptr;
+        return THISTLE_ENOMEM;
+    }
+    return THISTLE_OK;
+}
+
+const ThistleOpts kDefaultOpts = {THISTLE_FP64, 1e-30, nullptr, 0, 0};
+
+}  // namespace
+
+struct ThistlePlan {
+    int type = 1, precision = 0;
+    int ngto = 0, ng = 0, maxl = 0, nq = 0, nmo = 0, norb = 0;
+    int npair = 0, npmo = 0, waves = 0, nrows = 0;
+    int64_t npp = 0, nquart = 0, nmat = 0, units = 0;
+    double cutoffz = 0, cutoffmd = 0, cutoffcentre = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+    rocblas_handle blas = nullptr;
+    std::vector<double> h_q;
+    // device buffers
+    DevBuf G, P, E, e12, q, lxyz, rows, qlist, flags, scan, qcomp, count;
+    DevBuf mmod, mat, total, rmat, rtot, hkeys, hmin, hcnt, hslot, uscan;
+    DevBuf V, Gs, T, M, Z, mopairs, xa, xb, onerdm;
+    DevBuf part, partc, S, cls;
+    int64_t cls_off[18] = {0};  // per-LL ranges in `cls` (positions, ref order)
+    hipEvent_t ev[6] = {};
+    double ms[5] = {0, 0, 0, 0, 0};
+    ~ThistlePlan() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (blas) rocblas_destroy_handle(blas);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int check_basis(const ThistleBasis* b, std::vector<GroupInfo>& groups, int& maxl_eff) {
+    if (!b || b->ngto <= 0 || b->ng <= 0 || b->nq <= 0 || !b->ga || !b->l || !b->m ||
+        !b->n || !b->xx || !b->yy || !b->zz || !b->group || !b->q)
+        return THISTLE_EINVAL;
+    if (b->ng > 1024) return THISTLE_EINVAL;
+    int maxv = 0;
+    for (int j = 0; j < b->ngto; ++j) {
+        if (b->l[j] < 0 || b->m[j] < 0 || b->n[j] < 0) return THISTLE_EINVAL;
+        maxv = std::max({maxv, (int)b->l[j], (int)b->m[j], (int)b->n[j]});
+        if (b->group[j] < 1 || b->group[j] > b->ng) return THISTLE_EGROUP;
+    }
+    if (maxv > 4) return THISTLE_ELMAX;
+    maxl_eff = std::max(maxv, (int)b->maxl);
+    groups.assign(b->ng, GroupInfo{});
+    std::vector<int> seen(b->ng, 0);
+    for (int g = 0; g < b->ng; ++g) groups[g].start = -1;
+    for (int j = 0; j < b->ngto; ++j) {
+        int g = b->group[j] - 1;
+        if (groups[g].start < 0) {
+            groups[g].start = j;
+            groups[g].x = b->xx[j];
+            groups[g].y = b->yy[j];
+            groups[g].z = b->zz[j];
+            groups[g].ga = b->ga[j];
+            groups[g].L = b->l[j] + b->m[j] + b->n[j];
+        }
+        groups[g].count++;
+    }
+    for (int g = 0; g < b->ng; ++g) {
+        if (groups[g].start < 0) return THISTLE_EGROUP;
+        for (int j = groups[g].start; j < groups[g].start + groups[g].count; ++j)
+            if (b->group[j] != g + 1) return THISTLE_EGROUP;
+    }
+    // LLmax + 1 > 20 for the largest quartet (src/Integrals.f90:5117-5120)
+    int Lmax = 0;
+    for (auto& g : groups) Lmax = std::max(Lmax, g.L);
+    if (4 * Lmax + 1 > 20) return THISTLE_ELLMAX;
+    return THISTLE_OK;
+}
+
+// Rows of the five nests in reference order (src/Integrals.f90:79-416).
+std::vector<EnumRow> enum_rows(int ng, int64_t& total) {
+    std::vector<EnumRow> rows;
+    int64_t off = 0;
+    auto add = [&](int nest, int i, int j, int k, int r0, int cnt) {
+        if (cnt <= 0) return;
+        rows.push_back(EnumRow{nest, i, j, k, r0, cnt, off});
+        off += cnt;
+    };
+    for (int i = 0; i < ng; ++i)
+        for (int j = i + 1; j < ng; ++j)
+            for (int k = i + 1; k < ng; ++k) add(1, i, j, k, k + 1, ng - 1 - k);
+    for (int i = 0; i < ng; ++i)
+        for (int j = i + 1; j < ng; ++j) add(2, i, j, -1, i + 1, ng - 1 - i);
+    for (int i = 0; i < ng; ++i)
+        for (int k = 0; k < ng; ++k) add(3, i, -1, k, k + 1, ng - 1 - k);
+    for (int i = 0; i < ng; ++i) add(4, i, -1, -1, i + 1, ng - 1 - i);
+    add(5, -1, -1, -1, 0, ng);
+    total = off;
+    return rows;
+}
+
+int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
+    std::vector<GroupInfo> groups;
+    int maxl = 0;
+    int rc = check_basis(b, groups, maxl);
+    if (rc) return rc;
+    pl->precision = o->precision;
+    pl->ngto = b->ngto;
+    pl->ng = b->ng;
+    pl->maxl = maxl;
+    pl->nq = b->nq;
+    pl->cutoffz = b->cutoffz;
+    pl->cutoffmd = b->cutoffmd;
+    pl->cutoffcentre = b->cutoffcentre;
+    pl->device = o->device;
+    HCHECK(hipSetDevice(o->device));
+    if (o->stream) {
+        pl->stream = (hipStream_t)o->stream;
+    } else {
+        HCHECK(hipStreamCreate(&pl->stream));
+        pl->own_stream = true;
+    }
+    for (auto& e : pl->ev) HCHECK(hipEventCreate(&e));
+    int ng = b->ng;
+    pl->npair = ng * (ng + 1) / 2;
+    std::vector<PairInfo> pairs(pl->npair);
+    std::vector<int32_t> xa, xb;
+    int64_t off = 0;
+    for (int gi = 0; gi < ng; ++gi)
+        for (int gj = gi; gj < ng; ++gj) {
+            PairInfo& p = pairs[pair_index(gi, gj, ng)];
+            std::memset(&p, 0, sizeof p);
+            p.gi = gi;
+            p.gj = gj;
+            p.off = (int32_t)off;
+            p.nab = groups[gi].count * groups[gj].count;
+            for (int a = 0; a < groups[gi].count; ++a)
+                for (int c = 0; c < groups[gj].count; ++c) {
+                    xa.push_back(groups[gi].start + a);
+                    xb.push_back(groups[gj].start + c);
+                }
+            off += p.nab;
+        }
+    if (off > 0x7fffffff) return THISTLE_EINVAL;
+    pl->npp = off;
+    std::vector<char4> lx(b->ngto);
+    for (int j = 0; j < b->ngto; ++j)
+        lx[j] = make_char4((char)b->l[j], (char)b->m[j], (char)b->n[j],
+                           (char)(b->l[j] + b->m[j] + b->n[j]));
+    int64_t Q = 0;
+    std::vector<EnumRow> rows = enum_rows(ng, Q);
+    pl->nquart = Q;
+    pl->nrows = (int)rows.size();
+    // positions of each LL = L_i+L_j+L_k+L_r class, in reference order
+    std::vector<int32_t> cls(Q);
+    {
+        std::vector<int64_t> cnt(18, 0);
+        auto LLof = [&](const EnumRow& R, int x) {
+            int gi, gj, gk, gr;
+            switch (R.nest) {
+                case 1: gi = R.i; gj = R.j; gk = R.k; gr = x; break;
+                case 2: gi = R.i; gj = R.j; gk = R.i; gr = x; break;
+                case 3: gi = R.i; gj = R.i; gk = R.k; gr = x; break;
+                case 4: gi = R.i; gj = R.i; gk = x; gr = x; break;
+                default: gi = gj = gk = gr = x; break;
+            }
+            return groups[gi].L + groups[gj].L + groups[gk].L + groups[gr].L;
+        };
+        for (const auto& R : rows)
+            for (int u = 0; u < R.count; ++u) cnt[LLof(R, R.r0 + u)]++;
+        pl->cls_off[0] = 0;
+        for (int k = 0; k < 17; ++k) pl->cls_off[k + 1] = pl->cls_off[k] + cnt[k];
+        std::vector<int64_t> fill(pl->cls_off, pl->cls_off + 17);
+        for (const auto& R : rows)
+            for (int u = 0; u < R.count; ++u) cls[fill[LLof(R, R.r0 + u)]++] = (int32_t)(R.out + u);
+    }
+    int L1 = maxl + 1, T1 = 2 * maxl + 1;
+    if ((rc = dalloc(pl->G, sizeof(GroupInfo) * ng))) return rc;
+    if ((rc = dalloc(pl->P, sizeof(PairInfo) * pl->npair))) return rc;
+    if ((rc = dalloc(pl->E, sizeof(double) * (size_t)pl->npair * 3 * L1 * L1 * T1))) return rc;
+    if ((rc = dalloc(pl->e12, sizeof(double) * (size_t)pl->npair * b->nq))) return rc;
+    if ((rc = dalloc(pl->q, sizeof(double) * b->nq))) return rc;
+    if ((rc = dalloc(pl->lxyz, sizeof(char4) * b->ngto))) return rc;
+    if ((rc = dalloc(pl->rows, sizeof(EnumRow) * rows.size()))) return rc;
+    if ((rc = dalloc(pl->qlist, sizeof(uint64_t) * Q))) return rc;
+    if ((rc = dalloc(pl->flags, Q))) return rc;
+    int64_t nb = (Q + 2047) / 2048 + 1;
+    if ((rc = dalloc(pl->scan, sizeof(int64_t) * (Q + nb + 1)))) return rc;
+    if ((rc = dalloc(pl->qcomp, sizeof(uint64_t) * Q))) return rc;
+    if ((rc = dalloc(pl->count, sizeof(int64_t) * 2))) return rc;
+    if ((rc = dalloc(pl->xa, sizeof(int32_t) * pl->npp))) return rc;
+    if ((rc = dalloc(pl->xb, sizeof(int32_t) * pl->npp))) return rc;
+    if ((rc = dalloc(pl->cls, sizeof(int32_t) * Q))) return rc;
+    HCHECK(hipMemcpyAsync(pl->cls.p, cls.data(), sizeof(int32_t) * Q, hipMemcpyHostToDevice,
+                          pl->stream));
+    HCHECK(hipStreamSynchronize(pl->stream));
+    pl->h_q.assign(b->q, b->q + b->nq);
+    HCHECK(hipMemcpyAsync(pl->G.p, groups.data(), sizeof(GroupInfo) * ng, hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(pl->P.p, pairs.data(), sizeof(PairInfo) * pl->npair, hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(pl->q.p, b->q, sizeof(double) * b->nq, hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(pl->lxyz.p, lx.data(), sizeof(char4) * b->ngto, hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(pl->rows.p, rows.data(), sizeof(EnumRow) * rows.size(), hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(pl->xa.p, xa.data(), sizeof(int32_t) * pl->npp, hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(pl->xb.p, xb.data(), sizeof(int32_t) * pl->npp, hipMemcpyHostToDevice, pl->stream));
+    int waves = o->waves;
+    if (waves <= 0) {
+        hipDeviceProp_t prop;
+        HCHECK(hipGetDeviceProperties(&prop, o->device));
+        waves = prop.multiProcessorCount * 8;
+    }
+    pl->waves = (int)std::max<int64_t>(1, std::min<int64_t>(waves, Q));
+    int nqc = std::min(b->nq, 512);
+    if ((rc = dalloc(pl->part, sizeof(double) * (size_t)pl->waves * nqc))) return rc;
+    if ((rc = dalloc(pl->partc, sizeof(double) * (size_t)pl->waves * nqc))) return rc;
+    if ((rc = dalloc(pl->S, sizeof(double) * b->nq))) return rc;
+    if (b->nmo > 0 && b->mmod) {
+        pl->nmo = b->nmo;
+        if ((rc = dalloc(pl->mmod, sizeof(double) * (size_t)b->nmo * b->ngto))) return rc;
+        HCHECK(hipMemcpyAsync(pl->mmod.p, b->mmod, sizeof(double) * (size_t)b->nmo * b->ngto,
+                              hipMemcpyHostToDevice, pl->stream));
+    }
+    return THISTLE_OK;
+}
+
+int run_tables_and_enum(ThistlePlan* pl) {
+    HCHECK(launch_pair_info(pl->G.as<GroupInfo>(), pl->P.as<PairInfo>(), pl->npair, pl->stream));
+    HCHECK(launch_pair_E(pl->G.as<GroupInfo>(), pl->P.as<PairInfo>(), pl->npair, pl->maxl,
+                         pl->E.as<double>(), pl->stream));
+    HCHECK(launch_e12(pl->P.as<PairInfo>(), pl->npair, pl->q.as<double>(), pl->nq,
+                      pl->e12.as<double>(), nullptr, pl->stream));
+    HCHECK(hipEventRecord(pl->ev[1], pl->stream));
+    HCHECK(launch_enumerate(pl->rows.as<EnumRow>(), pl->nrows, pl->P.as<PairInfo>(), pl->ng,
+                            pl->cutoffcentre, pl->qlist.as<uint64_t>(), pl->stream));
+    HCHECK(hipEventRecord(pl->ev[2], pl->stream));
+    return THISTLE_OK;
+}
+
+int run_transform_total(ThistlePlan* pl) {
+    const int norb = pl->norb, npmo = pl->npmo;
+    UniqueWork w{pl->hkeys.as<uint64_t>(), pl->hmin.as<int32_t>(), pl->hcnt.as<int32_t>(),
+                 pl->hslot.as<int32_t>(), pl->uscan.as<int64_t>(),
+                 (int64_t)(pl->hkeys.bytes / sizeof(uint64_t))};
+    HCHECK(launch_unique(pl->mat.as<int32_t>(), pl->nmat, 4, pl->total.as<double>(), w,
+                         pl->rmat.as<int32_t>(), pl->rtot.as<double>(), nullptr, nullptr,
+                         pl->count.as<int64_t>() + 1, pl->stream));
+    HCHECK(hipMemsetAsync(pl->V.p, 0, pl->V.bytes, pl->stream));
+    // only the unique prefix of rmat/rtot is valid; its length stays on the
+    // device and guards the scatter
+    HCHECK(launch_gs(pl->rmat.as<int32_t>(), pl->rtot.as<double>(), pl->nmat,
+                     pl->count.as<int64_t>() + 1, pl->nmat, norb, pl->V.as<double>(),
+                     pl->stream));
+    HCHECK(launch_gs_matrix(pl->V.as<double>(), norb, pl->mopairs.as<int2>(), npmo,
+                            pl->Gs.as<double>(), pl->stream));
+    HCHECK(launch_tmat2(pl->mmod.as<double>(), pl->nmo, pl->mopairs.as<int2>(), npmo,
+                        pl->xa.as<int32_t>(), pl->xb.as<int32_t>(), pl->npp, pl->T.as<double>(),
+                        pl->stream));
+    const double one = 1.0, zero = 0.0;
+    // M = Gs * T   (npmo x npp)
+    BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_none, rocblas_operation_none, npmo,
+                         (rocblas_int)pl->npp, npmo, &one, pl->Gs.as<double>(), npmo,
+                         pl->T.as<double>(), npmo, &zero, pl->M.as<double>(), npmo));
+    // Z = T^T * M  (npp x npp)
+    BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_transpose, rocblas_operation_none,
+                         (rocblas_int)pl->npp, (rocblas_int)pl->npp, npmo, &one,
+                         pl->T.as<double>(), npmo, pl->M.as<double>(), npmo, &zero,
+                         pl->Z.as<double>(), (rocblas_int)pl->npp));
+    return THISTLE_OK;
+}
+
+int run_series(ThistlePlan* pl, double* result_host) {
+    const bool elastic = pl->type == 2;
+    SeriesArgs a{};
+    a.qlist = pl->qlist.as<uint64_t>();
+    a.nquart = pl->nquart;
+    a.G = pl->G.as<GroupInfo>();
+    a.P = pl->P.as<PairInfo>();
+    a.E = pl->E.as<double>();
+    a.maxl = pl->maxl;
+    a.ng = pl->ng;
+    a.ngto = pl->ngto;
+    a.lxyz = pl->lxyz.as<char4>();
+    a.q = pl->q.as<double>();
+    a.nq = pl->nq;
+    a.e12 = pl->e12.as<double>();
+    a.Z = pl->Z.as<double>();
+    a.ldz = elastic ? pl->ngto : pl->npp;
+    a.elastic = elastic ? 1 : 0;
+    a.mixed = pl->precision == THISTLE_MIXED;
+    a.cutoffz = pl->cutoffz;
+    a.part = pl->part.as<double>();
+    a.partc = pl->partc.as<double>();
+    a.flags = pl->flags.as<uint8_t>();
+    for (int q0 = 0; q0 < pl->nq; q0 += 512) {
+        a.q0 = q0;
+        a.nqc = std::min(512, pl->nq - q0);
+        HCHECK(hipMemsetAsync(pl->part.p, 0, pl->part.bytes, pl->stream));
+        HCHECK(hipMemsetAsync(pl->partc.p, 0, pl->partc.bytes, pl->stream));
+        // heaviest classes first
+        for (int LL = 16; LL >= 0; --LL) {
+            int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
+            if (n <= 0) continue;
+            int w = (int)std::min<int64_t>(pl->waves, n);
+            HCHECK(launch_series(a, LL, pl->cls.as<int32_t>() + pl->cls_off[LL], n, w, 0,
+                                 pl->stream));
+        }
+        HCHECK(launch_reduce(a.part, a.partc, pl->waves, a.nqc, pl->S.as<double>() + q0,
+                             pl->stream));
+    }
+    HCHECK(hipEventRecord(pl->ev[4], pl->stream));
+    HCHECK(launch_compact(pl->flags.as<uint8_t>(), pl->qlist.as<uint64_t>(), pl->nquart,
+                          pl->scan.as<int64_t>(), pl->qcomp.as<uint64_t>(), pl->count.as<int64_t>(),
+                          pl->stream));
+    HCHECK(hipEventRecord(pl->ev[5], pl->stream));
+    if (result_host)
+        HCHECK(hipMemcpyAsync(result_host, pl->S.p, sizeof(double) * pl->nq, hipMemcpyDeviceToHost,
+                              pl->stream));
+    HCHECK(hipStreamSynchronize(pl->stream));
+    pl->units = pl->nquart * (int64_t)pl->nq;
+    return THISTLE_OK;
+}
+
+int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const int32_t* mat,
+                     const double* total, int32_t norb, const double* onerdm,
+                     const ThistleOpts* o, ThistlePlan** out) {
+    if (type != 1 && type != 2) return THISTLE_ETYPE;
+    std::unique_ptr<ThistlePlan> pl(new ThistlePlan());
+    pl->type = type;
+    int rc = setup_common(pl.get(), b, o);
+    if (rc) return rc;
+    if (!b->mmod || b->nmo <= 0) return THISTLE_EINVAL;
+    if (type == 1) {
+        if (nmat < 0 || (nmat > 0 && (!mat || !total))) return THISTLE_EINVAL;
+        int nrb = 0;
+        for (int64_t k = 0; k < 4 * nmat; ++k) {
+            if (mat[k] < 1 || mat[k] > b->nmo || mat[k] > 65535) return THISTLE_EINVAL;
+            nrb = std::max(nrb, (int)mat[k]);
+        }
+        pl->norb = std::max(nrb, 1);
+        pl->nmat = nmat;
+        pl->npmo = pl->norb * (pl->norb + 1) / 2;
+        std::vector<int2> mop;
+        for (int p = 0; p < pl->norb; ++p)
+            for (int s = p; s < pl->norb; ++s) mop.push_back(make_int2(p, s));
+        int64_t hs = 16;
+        while (hs < 2 * std::max<int64_t>(nmat, 1)) hs <<= 1;
+        size_t N4 = (size_t)pl->norb * pl->norb * pl->norb * pl->norb;
+        if ((rc = dalloc(pl->mat, sizeof(int32_t) * 4 * std::max<int64_t>(nmat, 1)))) return rc;
+        if ((rc = dalloc(pl->total, sizeof(double) * std::max<int64_t>(nmat, 1)))) return rc;
+        if ((rc = dalloc(pl->rmat, sizeof(int32_t) * 4 * std::max<int64_t>(nmat, 1)))) return rc;
+        if ((rc = dalloc(pl->rtot, sizeof(double) * std::max<int64_t>(nmat, 1)))) return rc;
+        if ((rc = dalloc(pl->hkeys, sizeof(uint64_t) * hs))) return rc;
+        if ((rc = dalloc(pl->hmin, sizeof(int32_t) * hs))) return rc;
+        if ((rc = dalloc(pl->hcnt, sizeof(int32_t) * hs))) return rc;
+        if ((rc = dalloc(pl->hslot, sizeof(int32_t) * std::max<int64_t>(nmat, 1)))) return rc;
+        int64_t nb = (nmat + 2047) / 2048 + 1;
+        if ((rc = dalloc(pl->uscan, sizeof(int64_t) * (nmat + nb + 1)))) return rc;
+        if ((rc = dalloc(pl->V, sizeof(double) * N4))) return rc;
+        if ((rc = dalloc(pl->Gs, sizeof(double) * (size_t)pl->npmo * pl->npmo))) return rc;
+        if ((rc = dalloc(pl->T, sizeof(double) * (size_t)pl->npmo * pl->npp))) return rc;
+        if ((rc = dalloc(pl->M, sizeof(double) * (size_t)pl->npmo * pl->npp))) return rc;
+        if ((rc = dalloc(pl->Z, sizeof(double) * (size_t)pl->npp * pl->npp))) return rc;
+        if ((rc = dalloc(pl->mopairs, sizeof(int2) * mop.size()))) return rc;
+        HCHECK(hipMemcpyAsync(pl->mopairs.p, mop.data(), sizeof(int2) * mop.size(),
+                              hipMemcpyHostToDevice, pl->stream));
+        if (nmat > 0) {
+            HCHECK(hipMemcpyAsync(pl->mat.p, mat, sizeof(int32_t) * 4 * nmat, hipMemcpyHostToDevice,
+                                  pl->stream));
+            HCHECK(hipMemcpyAsync(pl->total.p, total, sizeof(double) * nmat, hipMemcpyHostToDevice,
+                                  pl->stream));
+        }
+        if (rocblas_create_handle(&pl->blas) != rocblas_status_success) return THISTLE_EDEVICE;
+        BCHECK(rocblas_set_stream(pl->blas, pl->stream));
+    } else {
+        if (norb <= 0 || !onerdm || norb > b->nmo) return THISTLE_EINVAL;
+        pl->norb = norb;
+        if ((rc = dalloc(pl->onerdm, sizeof(double) * norb * norb))) return rc;
+        if ((rc = dalloc(pl->Z, sizeof(double) * (size_t)b->ngto * b->ngto))) return rc;
+        HCHECK(hipMemcpyAsync(pl->onerdm.p, onerdm, sizeof(double) * norb * norb,
+                              hipMemcpyHostToDevice, pl->stream));
+    }
+    HCHECK(hipStreamSynchronize(pl->stream));
+    *out = pl.release();
+    return THISTLE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* thistle_version(void) { return "thistle-mi355x 0.1 (gfx950)"; }
+
+int32_t thistle_plan_create(int32_t type, const ThistleBasis* basis, int64_t nmat,
+                            const int32_t* mat, const double* total, int32_t norb,
+                            const double* onerdm, const ThistleOpts* opts, ThistlePlan** plan) {
+    if (!plan) return THISTLE_EINVAL;
+    *plan = nullptr;
+    return plan_create_impl(type, basis, nmat, mat, total, norb, onerdm,
+                            opts ? opts : &kDefaultOpts, plan);
+}
+
+int32_t thistle_plan_execute(ThistlePlan* pl, double* result_out) {
+    if (!pl) return THISTLE_EINVAL;
+    HCHECK(hipSetDevice(pl->device));
+    HCHECK(hipEventRecord(pl->ev[0], pl->stream));
+    int rc = run_tables_and_enum(pl);
+    if (rc) return rc;
+    if (pl->type == 1) {
+        rc = run_transform_total(pl);
+        if (rc) return rc;
+    } else {
+        HCHECK(launch_zel(pl->mmod.as<double>(), pl->nmo, pl->norb, pl->onerdm.as<double>(),
+                          pl->ngto, pl->Z.as<double>(), pl->stream));
+    }
+    HCHECK(hipEventRecord(pl->ev[3], pl->stream));
+    rc = run_series(pl, result_out);
+    if (rc) return rc;
+    float t[5];
+    for (int k = 0; k < 5; ++k) {
+        HCHECK(hipEventElapsedTime(&t[k], pl->ev[k], pl->ev[k + 1]));
+        pl->ms[k] = t[k];
+    }
+    return THISTLE_OK;
+}
+
+int32_t thistle_plan_result(ThistlePlan* pl, double* result_out) {
+    if (!pl || !result_out) return THISTLE_EINVAL;
+    HCHECK(hipMemcpy(result_out, pl->S.p, sizeof(double) * pl->nq, hipMemcpyDeviceToHost));
+    return THISTLE_OK;
+}
+
+int32_t thistle_plan_timings(ThistlePlan* pl, double* ms5, int64_t* nquart, int64_t* units) {
+    if (!pl) return THISTLE_EINVAL;
+    if (ms5)
+        for (int k = 0; k < 5; ++k) ms5[k] = pl->ms[k];
+    if (nquart) *nquart = pl->nquart;
+    if (units) *units = pl->units;
+    return THISTLE_OK;
+}
+
+void thistle_plan_destroy(ThistlePlan* pl) { delete pl; }
+
+int32_t thistle_total_from_twordm(const ThistleBasis* basis, int64_t nmat, const int32_t* mat,
+                                  const double* total, const ThistleOpts* opts, double* result_out) {
+    ThistlePlan* pl = nullptr;
+    int rc = thistle_plan_create(1, basis, nmat, mat, total, 0, nullptr, opts, &pl);
+    if (rc) return rc;
+    rc = thistle_plan_execute(pl, result_out);
+    thistle_plan_destroy(pl);
+    return rc;
+}
+
+int32_t thistle_elastic_from_onerdm(const ThistleBasis* basis, int32_t norb, const double* onerdm,
+                                    const ThistleOpts* opts, double* result_out) {
+    ThistlePlan* pl = nullptr;
+    int rc = thistle_plan_create(2, basis, 0, nullptr, nullptr, norb, onerdm, opts, &pl);
+    if (rc) return rc;
+    rc = thistle_plan_execute(pl, result_out);
+    thistle_plan_destroy(pl);
+    return rc;
+}
+
+int32_t thistle_twordm_dense(const int32_t* confs, int32_t nconf, int32_t lconfs,
+                             const double* civecs, int32_t nstates, int32_t state1, int32_t state2,
+                             double* twordm) {
+    if (!confs || !civecs || !twordm) return THISTLE_EINVAL;
+    return thistle_twordm_dense_host(confs, nconf, lconfs, civecs, nstates, state1, state2, twordm);
+}
+
+int64_t thistle_twordm_select(const double* twordm, int32_t norb, double cutoff, int32_t* mat,
+                              double* total, int64_t cap) {
+    if (!twordm || norb <= 0) return -THISTLE_EINVAL;
+    return thistle_twordm_select_host(twordm, norb, cutoff, cap > 0 ? mat : nullptr,
+                                      cap > 0 ? total : nullptr, cap);
+}
+
+int32_t thistle_onerdm(const int32_t* confs, int32_t nconf, int32_t lconfs, const double* civecs,
+                       int32_t nstates, int32_t state1, int32_t state2, double* onerdm) {
+    if (!confs || !civecs || !onerdm) return THISTLE_EINVAL;
+    return thistle_onerdm_host(confs, nconf, lconfs, civecs, nstates, state1, state2, onerdm);
+}
+
+int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const double* Zn,
+                                             const double* geom, int32_t state1, int32_t state2,
+                                             int32_t state3, const ThistleBasis* basis,
+                                             int32_t nconf, int32_t lconfs, int32_t nstates,
+                                             const int32_t* confs, const double* civecs,
+                                             const ThistleOpts* opts, double* q_abs_out,
+                                             double* result_out) {
+    (void)natom;
+    (void)Zn;
+    (void)geom;
+    (void)state3;
+    if (type != 1 && type != 2) return THISTLE_ETYPE;
+    if (!basis || !result_out) return THISTLE_EINVAL;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    if (q_abs_out)
+        for (int i = 0; i < basis->nq; ++i) q_abs_out[i] = 0.0;
+    const int norb = lconfs / 2;
+    if (type == 1) {
+        std::vector<double> tw((size_t)norb * norb * norb * norb);
+        int rc = thistle_twordm_dense_host(confs, nconf, lconfs, civecs, nstates, state1, state2,
+                                           tw.data());
+        if (rc) return rc;
+        double cut = o->rdm_cutoff > 0 ? o->rdm_cutoff : 1e-30;
+        int64_t cnt = thistle_twordm_select_host(tw.data(), norb, cut, nullptr, nullptr, 0);
+        std::vector<int32_t> mat((size_t)4 * std::max<int64_t>(cnt, 1));
+        std::vector<double> tot(std::max<int64_t>(cnt, 1));
+        thistle_twordm_select_host(tw.data(), norb, cut, mat.data(), tot.data(), cnt);
+        return thistle_total_from_twordm(basis, cnt, mat.data(), tot.data(), o, result_out);
+    }
+    std::vector<double> g((size_t)norb * norb);
+    int rc = thistle_onerdm_host(confs, nconf, lconfs, civecs, nstates, state1, state2, g.data());
+    if (rc) return rc;
+    return thistle_elastic_from_onerdm(basis, norb, g.data(), o, result_out);
+}
+
+int64_t thistle_quartet_list(const ThistleBasis* basis, int32_t* out, int64_t cap,
+                             const ThistleOpts* opts) {
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    std::unique_ptr<ThistlePlan> pl(new ThistlePlan());
+    int rc = setup_common(pl.get(), basis, o);
+    if (rc) return -rc;
+    rc = run_tables_and_enum(pl.get());
+    if (rc) return -rc;
+    if (out && cap >= pl->nquart) {
+        DevBuf tmp;
+        if (dalloc(tmp, sizeof(int32_t) * 6 * pl->nquart)) return -THISTLE_ENOMEM;
+        if (launch_unpack_quartets(pl->qlist.as<uint64_t>(), pl->nquart, tmp.as<int32_t>(),
+                                   pl->stream) != hipSuccess)
+            return -THISTLE_EDEVICE;
+        if (hipMemcpyAsync(out, tmp.p, sizeof(int32_t) * 6 * pl->nquart, hipMemcpyDeviceToHost,
+                           pl->stream) != hipSuccess)
+            return -THISTLE_EDEVICE;
+    }
+    if (hipStreamSynchronize(pl->stream) != hipSuccess) return -THISTLE_EDEVICE;
+    return pl->nquart;
+}
+
+int64_t thistle_quartet_list_screened(int32_t type, const ThistleBasis* basis, int64_t nmat,
+                                      const int32_t* mat, const double* total, int32_t norb,
+                                      const double* onerdm, int32_t* out, int64_t cap,
+                                      const ThistleOpts* opts) {
+    ThistlePlan* raw = nullptr;
+    int rc = thistle_plan_create(type, basis, nmat, mat, total, norb, onerdm, opts, &raw);
+    if (rc) return -rc;
+    std::unique_ptr<ThistlePlan> pl(raw);
+    rc = thistle_plan_execute(pl.get(), nullptr);
+    if (rc) return -rc;
+    int64_t cnt = 0;
+    if (hipMemcpy(&cnt, pl->count.p, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return -THISTLE_EDEVICE;
+    if (out && cap >= cnt && cnt > 0) {
+        DevBuf tmp;
+        if (dalloc(tmp, sizeof(int32_t) * 6 * cnt)) return -THISTLE_ENOMEM;
+        if (launch_unpack_quartets(pl->qcomp.as<uint64_t>(), cnt, tmp.as<int32_t>(), pl->stream) !=
+            hipSuccess)
+            return -THISTLE_EDEVICE;
+        if (hipMemcpyAsync(out, tmp.p, sizeof(int32_t) * 6 * cnt, hipMemcpyDeviceToHost,
+                           pl->stream) != hipSuccess)
+            return -THISTLE_EDEVICE;
+        if (hipStreamSynchronize(pl->stream) != hipSuccess) return -THISTLE_EDEVICE;
+    }
+    return cnt;
+}
+
+int64_t thistle_unique_total(const int32_t* matrix, int64_t nrow, int32_t ncol, const double* total,
+                             int32_t* rmat, double* rtot, int32_t* iuni, int32_t* irec,
+                             const ThistleOpts* opts) {
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    if (nrow < 0 || ncol < 1 || ncol > 8 || (nrow > 0 && (!matrix || !total))) return -THISTLE_EINVAL;
+    int lim = ncol <= 4 ? 65535 : 255;
+    for (int64_t k = 0; k < nrow * ncol; ++k)
+        if (matrix[k] < 0 || matrix[k] > lim) return -THISTLE_EINVAL;
+    if (hipSetDevice(o->device) != hipSuccess) return -THISTLE_EDEVICE;
+    hipStream_t s = (hipStream_t)o->stream;
+    int64_t hs = 16;
+    while (hs < 2 * std::max<int64_t>(nrow, 1)) hs <<= 1;
+    DevBuf dm, dt, keys, mn, cn, sl, sc, drm, drt, diu, dir, cnt;
+    int64_t nb = (nrow + 2047) / 2048 + 1;
+    if (dalloc(dm, sizeof(int32_t) * std::max<int64_t>(nrow * ncol, 1)) ||
+        dalloc(dt, sizeof(double) * std::max<int64_t>(nrow, 1)) || dalloc(keys, 8 * hs) ||
+        dalloc(mn, 4 * hs) || dalloc(cn, 4 * hs) || dalloc(sl, 4 * std::max<int64_t>(nrow, 1)) ||
+        dalloc(sc, 8 * (nrow + nb + 1)) ||
+        dalloc(drm, sizeof(int32_t) * std::max<int64_t>(nrow * ncol, 1)) ||
+        dalloc(drt, sizeof(double) * std::max<int64_t>(nrow, 1)) ||
+        dalloc(diu, 4 * std::max<int64_t>(nrow, 1)) || dalloc(dir, 4 * std::max<int64_t>(nrow, 1)) ||
+        dalloc(cnt, 16))
+        return -THISTLE_ENOMEM;
+    if (nrow > 0) {
+        if (hipMemcpyAsync(dm.p, matrix, sizeof(int32_t) * nrow * ncol, hipMemcpyHostToDevice, s) ||
+            hipMemcpyAsync(dt.p, total, sizeof(double) * nrow, hipMemcpyHostToDevice, s))
+            return -THISTLE_EDEVICE;
+    }
+    UniqueWork w{keys.as<uint64_t>(), mn.as<int32_t>(), cn.as<int32_t>(), sl.as<int32_t>(),
+                 sc.as<int64_t>(), hs};
+    if (launch_unique(dm.as<int32_t>(), nrow, ncol, dt.as<double>(), w, drm.as<int32_t>(),
+                      drt.as<double>(), diu.as<int32_t>(), dir.as<int32_t>(), cnt.as<int64_t>(),
+                      s) != hipSuccess)
+        return -THISTLE_EDEVICE;
+    int64_t u = 0;
+    if (hipMemcpyAsync(&u, cnt.p, 8, hipMemcpyDeviceToHost, s) != hipSuccess) return -THISTLE_EDEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return -THISTLE_EDEVICE;
+    // rmat comes back compacted to (u, ncol) column-major
+    if (rmat && u > 0) {
+        std::vector<int32_t> tmp((size_t)nrow * ncol);
+        if (hipMemcpy(tmp.data(), drm.p, sizeof(int32_t) * nrow * ncol, hipMemcpyDeviceToHost))
+            return -THISTLE_EDEVICE;
+        for (int c = 0; c < ncol; ++c)
+            for (int64_t k = 0; k < u; ++k) rmat[k + u * c] = tmp[k + nrow * c];
+    }
+    if (rtot && u > 0 && hipMemcpy(rtot, drt.p, sizeof(double) * u, hipMemcpyDeviceToHost))
+        return -THISTLE_EDEVICE;
+    if (iuni && u > 0 && hipMemcpy(iuni, diu.p, sizeof(int32_t) * u, hipMemcpyDeviceToHost))
+        return -THISTLE_EDEVICE;
+    if (irec && nrow > 0 && hipMemcpy(irec, dir.p, sizeof(int32_t) * nrow, hipMemcpyDeviceToHost))
+        return -THISTLE_EDEVICE;
+    return u;
+}
+
+int32_t thistle_pair_tables(const ThistleBasis* basis, double* E, double* P, double* e12,
+                            const ThistleOpts* opts) {
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    std::unique_ptr<ThistlePlan> pl(new ThistlePlan());
+    int rc = setup_common(pl.get(), basis, o);
+    if (rc) return rc;
+    rc = run_tables_and_enum(pl.get());
+    if (rc) return rc;
+    int L1 = pl->maxl + 1, T1 = 2 * pl->maxl + 1;
+    if (E)
+        HCHECK(hipMemcpyAsync(E, pl->E.p, sizeof(double) * (size_t)pl->npair * 3 * L1 * L1 * T1,
+                              hipMemcpyDeviceToHost, pl->stream));
+    if (e12)
+        HCHECK(hipMemcpyAsync(e12, pl->e12.p, sizeof(double) * (size_t)pl->npair * pl->nq,
+                              hipMemcpyDeviceToHost, pl->stream));
+    std::vector<PairInfo> pi(pl->npair);
+    HCHECK(hipMemcpyAsync(pi.data(), pl->P.p, sizeof(PairInfo) * pl->npair, hipMemcpyDeviceToHost,
+                          pl->stream));
+    HCHECK(hipStreamSynchronize(pl->stream));
+    if (P)
+        for (int p = 0; p < pl->npair; ++p) {
+            P[3 * p + 0] = pi[p].Px;
+            P[3 * p + 1] = pi[p].Py;
+            P[3 * p + 2] = pi[p].Pz;
+        }
+    return THISTLE_OK;
+}
+
+}  // extern "C"

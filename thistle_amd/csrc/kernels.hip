@@ -1,0 +1,855 @@
+// Device kernels of the MI355X total / elastic j0 scattering path (gfx950).
+//
+// Stages (reference routine each restates in parentheses):
+//   pair tables   -- McMurchie-Davidson E tables, pair centres, e12
+//                    (fill_md_table / variables_total, src/Variables.f90)
+//   enumeration   -- the five quartet nests of tot_integration
+//                    (src/Integrals.f90:79-82,178-180,258-260,340-341,416)
+//   transform     -- unique 2-RDM rows, MO-pair matrix, T matrix
+//                    (unique_total, z11/z22, src/Variables.f90:1000-1061)
+//   series        -- per quartet: 2-RDM block x Hermite contraction -> h(beta),
+//                    then the j_beta series over the q grid
+//                    (tot_integral_k_ijkr / _pzero, BesselDeriv, rrdj0,
+//                     bessels0rr, dphrec; src/Integrals.f90:4708-5226,
+//                     6662-6739,7062-7184)
+//   reduction     -- fixed-order compensated sum of per-wave partial S(q)
+#include "thistle_internal.hpp"
+
+#include <math.h>
+
+#define WAVE 64
+
+// pi = acos(-1.0000) in single precision, widened (src/Variables.f90:994)
+__constant__ double kPiSp = 3.1415927410125732421875;
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ void kahan_add(double& s, double& c, double v) {
+#pragma clang fp contract(off)
+    double y = v - c;
+    double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// ------------------------------------------------------------ pair tables
+// One top-level E^{l1,l2}_t slice: raise A to l1 with PA, then B to l2 with
+// PB, in the operation order of fill_md_table (src/Variables.f90:31-944).
+__device__ void md_slice(double PA, double PB, double a, int l1, int l2,
+                         double* out) {
+#pragma clang fp contract(off)
+    double cur[17], nxt[17];
+    for (int t = 0; t < 17; ++t) cur[t] = 0.0;
+    cur[0] = 1.0;
+    int top = 0;
+    for (int pass = 0; pass < l1 + l2; ++pass) {
+        double PX = pass < l1 ? PA : PB;
+        for (int t = 0; t <= top + 1; ++t) {
+            double em1 = t > 0 ? cur[t - 1] : 0.0;
+            double e0 = t <= top ? cur[t] : 0.0;
+            double ep1 = t + 1 <= top ? cur[t + 1] : 0.0;
+            double v;
+            if (t == 0)
+                v = PX * e0 + ep1;
+            else if (t == top + 1)
+                v = a * em1;
+            else if (t == top)
+                v = a * em1 + PX * e0;
+            else
+                v = a * em1 + PX * e0 + (double)(t + 1) * ep1;
+            nxt[t] = v;
+        }
+        top += 1;
+        for (int t = 0; t <= top; ++t) cur[t] = nxt[t];
+    }
+    for (int t = 0; t <= l1 + l2; ++t) out[t] = cur[t];
+}
+
+__global__ void k_pair_info(const GroupInfo* __restrict__ G,
+                            PairInfo* __restrict__ P, int npair) {
+#pragma clang fp contract(off)
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npair) return;
+    PairInfo pi = P[p];
+    GroupInfo A = G[pi.gi], B = G[pi.gj];
+    double gaP = A.ga + B.ga;
+    pi.gaP = gaP;
+    pi.Px = (A.ga * A.x + B.ga * B.x) / gaP;
+    pi.Py = (A.ga * A.y + B.ga * B.y) / gaP;
+    pi.Pz = (A.ga * A.z + B.ga * B.z) / gaP;
+    double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    pi.pre = pow(kPiSp / gaP, 1.5);
+    pi.kab = exp(-(A.ga * B.ga) / gaP * r2);
+    P[p] = pi;
+}
+
+// thread per (pair, axis, la, lb)
+__global__ void k_pair_E(const GroupInfo* __restrict__ G,
+                         const PairInfo* __restrict__ P, int npair, int maxl,
+                         double* __restrict__ E) {
+#pragma clang fp contract(off)
+    const int L1 = maxl + 1, T1 = 2 * maxl + 1;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)npair * 3 * L1 * L1;
+    if (idx >= total) return;
+    int lb = (int)(idx % L1);
+    int64_t t = idx / L1;
+    int la = (int)(t % L1);
+    t /= L1;
+    int ax = (int)(t % 3);
+    int p = (int)(t / 3);
+    double* out = E + idx * T1;
+    for (int k = 0; k < T1; ++k) out[k] = 0.0;
+    PairInfo pi = P[p];
+    GroupInfo A = G[pi.gi], B = G[pi.gj];
+    if (la > A.L || lb > B.L) return;
+    double xa = ax == 0 ? A.x : (ax == 1 ? A.y : A.z);
+    double xb = ax == 0 ? B.x : (ax == 1 ? B.y : B.z);
+    double gaP = A.ga + B.ga;
+    double Px = (A.ga * xa + B.ga * xb) / gaP;
+    double a = 0.5 / gaP;
+    // the higher-l member is raised first; ties go to the lower GTO index,
+    // which is the A side because gi <= gj
+    if (la >= lb)
+        md_slice(Px - xa, Px - xb, a, la, lb, out);
+    else
+        md_slice(Px - xb, Px - xa, a, lb, la, out);
+}
+
+// e12(q,i,j) = (pi_sp/p)^1.5 exp(-q^2/(4p)) exp(-ab/p |AB|^2)
+__global__ void k_e12(const PairInfo* __restrict__ P, int npair,
+                      const double* __restrict__ q, int nq,
+                      double* __restrict__ e12, float* __restrict__ e12f) {
+#pragma clang fp contract(off)
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)npair * nq) return;
+    int p = (int)(idx / nq), iq = (int)(idx % nq);
+    PairInfo pi = P[p];
+    double v = pi.pre * exp(-(q[iq] * q[iq]) * 0.25 / pi.gaP) * pi.kab;
+    e12[idx] = v;
+    if (e12f) e12f[idx] = (float)v;
+}
+
+// ------------------------------------------------------------ enumeration
+// One 64-thread block per inner loop; entries written in reference order.
+__global__ void k_enumerate(const EnumRow* __restrict__ rows,
+                            const PairInfo* __restrict__ P, int ng,
+                            double cutoffcentre, uint64_t* __restrict__ out) {
+#pragma clang fp contract(off)
+    EnumRow R = rows[blockIdx.x];
+    for (int u = threadIdx.x; u < R.count; u += blockDim.x) {
+        int x = R.r0 + u;
+        int gi, gj, gk, gr, mult;
+        switch (R.nest) {
+            case 1: gi = R.i; gj = R.j; gk = R.k; gr = x; mult = 8; break;
+            case 2: gi = R.i; gj = R.j; gk = R.i; gr = x; mult = 4; break;
+            case 3: gi = R.i; gj = R.i; gk = R.k; gr = x; mult = 4; break;
+            case 4: gi = R.i; gj = R.i; gk = x; gr = x; mult = 2; break;
+            default: gi = x; gj = x; gk = x; gr = x; mult = 1; break;
+        }
+        int br = 1;
+        if (R.nest != 5) {
+            const PairInfo& b = P[pair_index(gi, gj, ng)];
+            const PairInfo& k = P[pair_index(gk, gr, ng)];
+            double hx = k.Px - b.Px, hy = k.Py - b.Py, hz = k.Pz - b.Pz;
+            double h = sqrt(hx * hx + hy * hy + hz * hz);
+            br = h < cutoffcentre ? 1 : 0;
+        }
+        out[R.out + u] = (uint64_t)gi | ((uint64_t)gj << 10) | ((uint64_t)gk << 20) |
+                         ((uint64_t)gr << 30) | ((uint64_t)mult << 40) |
+                         ((uint64_t)br << 44);
+    }
+}
+
+__global__ void k_unpack_quartets(const uint64_t* __restrict__ in, int64_t n,
+                                  int32_t* __restrict__ out) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    uint64_t v = in[e];
+    int32_t* o = out + 6 * e;
+    o[0] = QE_GI(v) + 1;
+    o[1] = QE_GJ(v) + 1;
+    o[2] = QE_GK(v) + 1;
+    o[3] = QE_GR(v) + 1;
+    o[4] = QE_MULT(v);
+    o[5] = QE_BR(v);
+}
+
+// ------------------------------------------------------------------- scan
+// Exclusive scan of 0/1 bytes into int64 (three passes, fixed order).
+#define SCAN_T 256
+#define SCAN_I 8
+__global__ void k_scan_blocks(const uint8_t* __restrict__ in, int64_t n,
+                              int64_t* __restrict__ out,
+                              int64_t* __restrict__ bsum) {
+    __shared__ int64_t s[SCAN_T];
+    int64_t base = (int64_t)blockIdx.x * SCAN_T * SCAN_I + (int64_t)threadIdx.x * SCAN_I;
+    int64_t v[SCAN_I];
+    int64_t run = 0;
+    for (int k = 0; k < SCAN_I; ++k) {
+        int64_t i = base + k;
+        v[k] = run;
+        run += (i < n) ? (int64_t)in[i] : 0;
+    }
+    s[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < SCAN_T; off <<= 1) {
+        int64_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int64_t excl = s[threadIdx.x] - run;
+    for (int k = 0; k < SCAN_I; ++k) {
+        int64_t i = base + k;
+        if (i < n) out[i] = v[k] + excl;
+    }
+    if (threadIdx.x == SCAN_T - 1) bsum[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ void k_scan_sums(int64_t* __restrict__ bsum, int64_t nb,
+                            int64_t* __restrict__ total) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t run = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+        int64_t t = bsum[b];
+        bsum[b] = run;
+        run += t;
+    }
+    *total = run;
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out, int64_t n,
+                           const int64_t* __restrict__ bsum) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] += bsum[i / (SCAN_T * SCAN_I)];
+}
+
+__global__ void k_compact(const uint8_t* __restrict__ flags,
+                          const uint64_t* __restrict__ in, int64_t n,
+                          const int64_t* __restrict__ pos,
+                          uint64_t* __restrict__ out) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    if (flags[e]) out[pos[e]] = in[e];
+}
+
+// ----------------------------------------------------------------- unique
+static __device__ __forceinline__ uint64_t mix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+static __device__ __forceinline__ uint64_t row_key(const int32_t* m, int64_t nrow,
+                                                   int ncol, int64_t r) {
+    uint64_t key = 0;
+    int bits = ncol <= 4 ? 16 : 8;
+    for (int c = 0; c < ncol; ++c)
+        key |= ((uint64_t)(uint32_t)m[r + nrow * c] & ((1ull << bits) - 1)) << (bits * c);
+    return key;
+}
+
+#define HEMPTY 0xffffffffffffffffull
+
+__global__ void k_hash_init(uint64_t* keys, int32_t* minrow, int32_t* cnt,
+                            int64_t hsize) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hsize) return;
+    keys[i] = HEMPTY;
+    minrow[i] = 0x7fffffff;
+    cnt[i] = 0;
+}
+
+__global__ void k_hash_insert(const int32_t* __restrict__ m, int64_t nrow,
+                              int ncol, uint64_t* keys, int32_t* minrow,
+                              int32_t* cnt, int32_t* slot, int64_t hsize) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    uint64_t key = row_key(m, nrow, ncol, r);
+    uint64_t h = mix64(key) & (uint64_t)(hsize - 1);
+    while (true) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&keys[h],
+                                            (unsigned long long)HEMPTY,
+                                            (unsigned long long)key);
+        if (prev == HEMPTY || prev == key) {
+            atomicMin(&minrow[h], (int32_t)r);
+            atomicAdd(&cnt[h], 1);
+            slot[r] = (int32_t)h;
+            return;
+        }
+        h = (h + 1) & (uint64_t)(hsize - 1);
+    }
+}
+
+__global__ void k_first_flags(const int32_t* __restrict__ slot,
+                              const int32_t* __restrict__ minrow, int64_t nrow,
+                              uint8_t* __restrict__ flag) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    flag[r] = minrow[slot[r]] == (int32_t)r ? 1 : 0;
+}
+
+// rtot(u) = sum of total over the rows equal to the u-th unique row, in row
+// order (sum(same*total) of src/unique_module.f90:126-131).
+__global__ void k_unique_out(const int32_t* __restrict__ m, int64_t nrow,
+                             int ncol, const double* __restrict__ total,
+                             const int32_t* __restrict__ slot,
+                             const int32_t* __restrict__ minrow,
+                             const int32_t* __restrict__ cnt,
+                             const int64_t* __restrict__ pos, int32_t* rmat,
+                             int64_t ldr, double* rtot, int32_t* iuni,
+                             int32_t* irec) {
+#pragma clang fp contract(off)
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    int s = slot[r];
+    int64_t first = minrow[s];
+    if (irec) irec[r] = (int32_t)(pos[first] + 1);
+    if (first != r) return;
+    int64_t u = pos[r];
+    if (iuni) iuni[u] = (int32_t)(r + 1);
+    if (rmat)
+        for (int c = 0; c < ncol; ++c) rmat[u + ldr * c] = m[r + nrow * c];
+    if (rtot) {
+        double acc = 0.0;
+        int left = cnt[s];
+        for (int64_t k = r; k < nrow && left > 0; ++k)
+            if (slot[k] == s) {
+                acc = acc + total[k];
+                --left;
+            }
+        rtot[u] = acc;
+    }
+}
+
+// MO-pair 2-RDM: V[p,s,q,r] from unique rows, then
+// Gs(P,Q) = (G(P,Q) + G(Q,P))/8 with G summed over the orderings of each
+// unordered pair (P = {m1,m2}, Q = {m3,m4}).
+__global__ void k_scatter_v(const int32_t* __restrict__ rmat,
+                            const double* __restrict__ rtot, int64_t nu,
+                            const int64_t* __restrict__ nu_dev, int64_t ldr,
+                            int norb, double* __restrict__ V) {
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu || u >= *nu_dev) return;
+    int64_t N = norb;
+    int p = rmat[u] - 1, s = rmat[u + ldr] - 1, q = rmat[u + 2 * ldr] - 1,
+        r = rmat[u + 3 * ldr] - 1;
+    V[((p * N + s) * N + q) * N + r] = rtot[u];
+}
+
+__device__ __forceinline__ double g_unordered(const double* V, int N, int p,
+                                              int s, int q, int r) {
+#pragma clang fp contract(off)
+    auto at = [&](int a, int b, int c, int d) {
+        return V[(((int64_t)a * N + b) * N + c) * N + d];
+    };
+    double v = at(p, s, q, r);
+    if (p != s) v = v + at(s, p, q, r);
+    if (q != r) {
+        v = v + at(p, s, r, q);
+        if (p != s) v = v + at(s, p, r, q);
+    }
+    return v;
+}
+
+__global__ void k_gs(const double* __restrict__ V, int norb,
+                     const int2* __restrict__ mopairs, int npmo,
+                     double* __restrict__ Gs) {
+#pragma clang fp contract(off)
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)npmo * npmo) return;
+    int P = (int)(idx % npmo), Q = (int)(idx / npmo);
+    int2 a = mopairs[P], b = mopairs[Q];
+    double g1 = g_unordered(V, norb, a.x, a.y, b.x, b.y);
+    double g2 = g_unordered(V, norb, b.x, b.y, a.x, a.y);
+    Gs[P + (int64_t)npmo * Q] = (g1 + g2) * 0.125;
+}
+
+// T(P, x) = C(p,a)C(s,b) + C(p,b)C(s,a) for pair-space element x = (a,b);
+// column-major with leading dimension npmo.
+__global__ void k_tmat(const double* __restrict__ mmod, int nmo,
+                       const int2* __restrict__ mopairs, int npmo,
+                       const int32_t* __restrict__ xa,
+                       const int32_t* __restrict__ xb, int64_t npp,
+                       double* __restrict__ T) {
+#pragma clang fp contract(off)
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)npmo * npp) return;
+    int P = (int)(idx % npmo);
+    int64_t x = idx / npmo;
+    int2 ps = mopairs[P];
+    int a = xa[x], b = xb[x];
+    double ca_p = mmod[ps.x + (int64_t)nmo * a], cb_s = mmod[ps.y + (int64_t)nmo * b];
+    double cb_p = mmod[ps.x + (int64_t)nmo * b], ca_s = mmod[ps.y + (int64_t)nmo * a];
+    T[idx] = ca_p * cb_s + cb_p * ca_s;
+}
+
+// Z(ii,jj) of variables_elastic, src/Variables.f90:1199-1216
+__global__ void k_zel(const double* __restrict__ mmod, int nmo, int norb,
+                      const double* __restrict__ g, int ngto,
+                      double* __restrict__ Z) {
+#pragma clang fp contract(off)
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)ngto * ngto) return;
+    int ii = (int)(idx % ngto), jj = (int)(idx / ngto);
+    double temp = 0.0;
+    for (int cc = 0; cc < norb; ++cc)
+        for (int cc2 = 0; cc2 < norb; ++cc2) {
+            double gv = g[cc + (int64_t)norb * cc2];
+            temp = temp + gv * mmod[cc + (int64_t)nmo * ii] * mmod[cc2 + (int64_t)nmo * jj];
+            temp = temp + gv * mmod[cc + (int64_t)nmo * jj] * mmod[cc2 + (int64_t)nmo * ii];
+        }
+    Z[idx] = temp / 2.0;
+}
+
+__global__ void k_to_float(const double* __restrict__ in, float* __restrict__ out,
+                           int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+// ----------------------------------------------------------------- series
+#define QPL 8                 // q points per lane per launch (nq chunk <= 512)
+#define XDPH 74.0             // q*h from which dphrec's start-100 recurrence
+                              // is followed step by step (it departs from
+                              // the exact j_beta above x ~ 76)
+#define SCR_STRIDE 24         // per-lane LDS scratch length (T <= 4*4*... )
+
+// rrdj0 (src/Integrals.f90:7062-7091) into a (LL+1)^2 row-major table.
+template <int LL>
+__device__ void build_rrdj0(double x, double* a) {
+#pragma clang fp contract(off)
+    const int D = LL + 1;
+    for (int k = 0; k < D * D; ++k) a[k] = 0.0;
+    a[0] = 1.0;
+    if (LL >= 1) a[1 * D + 1] = -x;
+    for (int l = 2; l <= LL; ++l) a[l * D + 0] = (double)(-(l - 1)) * a[(l - 2) * D + 0];
+    for (int l = 2; l <= LL; ++l)
+        for (int p = 1; p <= l; ++p)
+            a[l * D + p] = (double)(-(l - 1)) * a[(l - 2) * D + p] - x * a[(l - 1) * D + p - 1];
+}
+
+// Per-axis polynomial of one GTO quadruple:
+//   c(T)  = sum_t (-1)^t Ebra(t) Eket(T-t)
+//   px(m) = sum_{T=m}^{min(2m,Lx)} c(T) a(T, 2m-T)
+// so that sum_T c(T) h_pre2(beta,T,..) factorises over the axes with
+// beta = mx + my + mz (BesselDeriv's hOrder = (T+i)/2 for even T-i).
+template <int LL>
+__device__ __forceinline__ void axis_poly(const double* __restrict__ Eb, int lab,
+                                          const double* __restrict__ Ek, int lcd,
+                                          const double* __restrict__ At,
+                                          double* __restrict__ scr, int lane,
+                                          double (&px)[LL + 1]) {
+    const int D = LL + 1;
+    int Lx = lab + lcd;
+    for (int T = 0; T <= Lx; ++T) scr[T * WAVE + lane] = 0.0;
+    for (int t = 0; t <= lab; ++t) {
+        double sb = (t & 1) ? -Eb[t] : Eb[t];
+        for (int t2 = 0; t2 <= lcd; ++t2) scr[(t + t2) * WAVE + lane] += sb * Ek[t2];
+    }
+#pragma unroll
+    for (int m = 0; m <= LL; ++m) {
+        double s = 0.0;
+        if (m <= Lx) {
+            int tmax = 2 * m < Lx ? 2 * m : Lx;
+            for (int T = m; T <= tmax; ++T) s += scr[T * WAVE + lane] * At[T * D + (2 * m - T)];
+        }
+        px[m] = s;
+    }
+}
+
+// F(q) for the h = 0 branch: sum_beta h0(beta) q^(2 beta) / (2 beta + 1)!!
+template <int LL>
+__device__ __forceinline__ double eval_pzero(const double (&hs)[LL + 1], double q) {
+    double q2 = q * q;
+    double dfi[LL + 1];
+    dfi[0] = 1.0;
+#pragma unroll
+    for (int b = 1; b <= LL; ++b) dfi[b] = dfi[b - 1] / (double)(2 * b + 1);
+    double F = hs[LL] * dfi[LL];
+#pragma unroll
+    for (int b = LL - 1; b >= 0; --b) F = F * q2 + hs[b] * dfi[b];
+    return F;
+}
+
+// dphrec with start order 100 (src/Integrals.f90:7147-7184) + bessels0rr
+template <int LL>
+__device__ double eval_dphrec(const double (&hs)[LL + 1], double q, double h,
+                              double x, double sx, double cx) {
+#pragma clang fp contract(off)
+    const double XX = x * x;
+    // 0-based R[i] = RBES(i+1); R[99] = R[98] = 1
+    double r2 = 1.0, r1 = 1.0;  // R[i+2], R[i+1]
+    int i = 97;
+    for (; i > LL + 1; --i) {
+        double LU = (double)(i + 2);
+        double r0 = r1 - XX / (4.0 * LU * LU - 1.0) * r2;
+        r2 = r1;
+        r1 = r0;
+    }
+    double R[LL + 2];
+#pragma unroll
+    for (int k = LL + 1; k >= 0; --k) {
+        double LU = (double)(k + 2);
+        double r0 = r1 - XX / (4.0 * LU * LU - 1.0) * r2;
+        r2 = r1;
+        r1 = r0;
+        R[k] = r0;
+    }
+    double A = R[0] * (cx + x * sx) - XX / 3.0 * R[1] * cx;
+    double CU = 1.0 / x;
+    double F = sx / x * hs[0];
+    double invp = q / h;
+    double rb = 1.0;
+#pragma unroll
+    for (int K = 1; K <= LL + 1; ++K) {
+        double W = 2.0 * (K - 1);
+        CU = x / (W + 1.0) * CU;
+        if (K >= 2) {
+            double jb = CU * (R[K - 1] / A);
+            rb = rb * invp;
+            F = F + jb * rb * hs[K - 1];
+        }
+    }
+    return F;
+}
+
+// F(q) = sum_beta h(beta) (q/h)^beta j_beta(q h)
+template <int LL>
+__device__ __forceinline__ double eval_series(const double (&hs)[LL + 1], double q,
+                                              double h) {
+    double x = q * h;
+    double sx, cx;
+    sincos(x, &sx, &cx);
+    double j0 = sx / x;
+    if (x >= XDPH) return eval_dphrec<LL>(hs, q, h, x, sx, cx);
+    if (x > (double)LL + 6.0) {
+        // upward recurrence: stable while beta < x
+        double inv = 1.0 / x, r = q / h;
+        double F = j0 * hs[0];
+        if (LL >= 1) {
+            double jm = j0, jc = (j0 - cx) * inv, rb = r;
+            F += jc * rb * hs[1];
+#pragma unroll
+            for (int b = 1; b < LL; ++b) {
+                double jn = (double)(2 * b + 1) * inv * jc - jm;
+                jm = jc;
+                jc = jn;
+                rb *= r;
+                F += jc * rb * hs[b + 1];
+            }
+        }
+        return F;
+    }
+    // Miller backward recurrence for g_b = j_b / x^b,
+    // g_{b-1} = (2b+1) g_b - x^2 g_{b+1}, normalised by g_0 = sin(x)/x
+    double x2 = x * x, q2 = q * q;
+    int N = LL + 16 + (int)x;
+    double vp1 = 0.0, v = 1.0;
+    for (int b = N; b > LL; --b) {
+        double vm = (double)(2 * b + 1) * v - x2 * vp1;
+        vp1 = v;
+        v = vm;
+    }
+    double acc = hs[LL] * v;
+#pragma unroll
+    for (int b = LL; b >= 1; --b) {
+        double vm = (double)(2 * b + 1) * v - x2 * vp1;
+        vp1 = v;
+        v = vm;
+        acc = acc * q2 + hs[b - 1] * v;
+    }
+    return acc * (j0 / v);
+}
+
+template <int LL>
+__device__ void quartet_work(const SeriesArgs& A, int64_t e, double* __restrict__ scr,
+                             double* __restrict__ At, double (&Sacc)[QPL],
+                             double (&Scmp)[QPL], int lane) {
+    const uint64_t ent = A.qlist[e];
+    const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
+    const int mult = QE_MULT(ent), br = QE_BR(ent);
+    const int pb = pair_index(gi, gj, A.ng), pk = pair_index(gk, gr, A.ng);
+    const PairInfo PB = A.P[pb], PK = A.P[pk];
+    const double hx = PK.Px - PB.Px, hy = PK.Py - PB.Py, hz = PK.Pz - PB.Pz;
+    const double h = sqrt(hx * hx + hy * hy + hz * hz);
+    const int D = LL + 1;
+
+    __syncthreads();
+    if (lane < 3) {
+        double xv = br ? 0.0 : (lane == 0 ? hx : (lane == 1 ? hy : hz));
+        build_rrdj0<LL>(xv, At + lane * D * D);
+    }
+    __syncthreads();
+
+    const GroupInfo Gi = A.G[gi], Gj = A.G[gj], Gk = A.G[gk], Gr = A.G[gr];
+    const int cj = Gj.count, ck = Gk.count, cr = Gr.count;
+    const int nquads = Gi.count * cj * ck * cr;
+    const int L1 = A.maxl + 1, T1 = 2 * A.maxl + 1;
+    const double* Eb0 = A.E + (int64_t)pb * 3 * L1 * L1 * T1;
+    const double* Ek0 = A.E + (int64_t)pk * 3 * L1 * L1 * T1;
+
+    double hs[LL + 1];
+#pragma unroll
+    for (int b = 0; b <= LL; ++b) hs[b] = 0.0;
+    int any = 0;
+    for (int u = lane; u < nquads; u += WAVE) {
+        int d_ = u % cr;
+        int t = u / cr;
+        int c_ = t % ck;
+        t /= ck;
+        int b_ = t % cj;
+        int a_ = t / cj;
+        const int ia = Gi.start + a_, ib = Gj.start + b_, ic = Gk.start + c_, id = Gr.start + d_;
+        double z;
+        if (!A.elastic) {
+            int64_t xb = PB.off + a_ * cj + b_;
+            int64_t xk = PK.off + c_ * cr + d_;
+            z = A.Z[xk + A.ldz * xb];
+        } else {
+            z = A.Z[ic + A.ldz * id] * A.Z[ia + A.ldz * ib];
+        }
+        if (fabs(z) < A.cutoffz) continue;
+        any = 1;
+        const char4 la = A.lxyz[ia], lb = A.lxyz[ib], lc = A.lxyz[ic], ld = A.lxyz[id];
+        double px[LL + 1], py[LL + 1], pz[LL + 1];
+        axis_poly<LL>(Eb0 + ((0 * L1 + la.x) * L1 + lb.x) * T1, la.x + lb.x,
+                      Ek0 + ((0 * L1 + lc.x) * L1 + ld.x) * T1, lc.x + ld.x, At,
+                      scr, lane, px);
+        axis_poly<LL>(Eb0 + ((1 * L1 + la.y) * L1 + lb.y) * T1, la.y + lb.y,
+                      Ek0 + ((1 * L1 + lc.y) * L1 + ld.y) * T1, lc.y + ld.y,
+                      At + D * D, scr, lane, py);
+        axis_poly<LL>(Eb0 + ((2 * L1 + la.z) * L1 + lb.z) * T1, la.z + lb.z,
+                      Ek0 + ((2 * L1 + lc.z) * L1 + ld.z) * T1, lc.z + ld.z,
+                      At + 2 * D * D, scr, lane, pz);
+        double pyz[LL + 1];
+#pragma unroll
+        for (int k = 0; k <= LL; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int my = 0; my <= k; ++my) s += py[my] * pz[k - my];
+            pyz[k] = s;
+        }
+#pragma unroll
+        for (int b = 0; b <= LL; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int mx = 0; mx <= b; ++mx) s += px[mx] * pyz[b - mx];
+            hs[b] += z * s;
+        }
+    }
+#pragma unroll
+    for (int b = 0; b <= LL; ++b) hs[b] = wave_sum(hs[b]);
+    int anyw = __any(any);
+    if (lane == 0) A.flags[e] = anyw ? 1 : 0;
+
+    const double dm = (double)mult;
+#pragma unroll
+    for (int s = 0; s < QPL; ++s) {
+        int iql = lane + WAVE * s;
+        if (iql < A.nqc) {
+            int iq = A.q0 + iql;
+            double q = A.q[iq];
+            double F = br ? eval_pzero<LL>(hs, q) : eval_series<LL>(hs, q, h);
+            double v = dm * F;
+            v = v * A.e12[(int64_t)pb * A.nq + iq];
+            v = v * A.e12[(int64_t)pk * A.nq + iq];
+            kahan_add(Sacc[s], Scmp[s], v);
+        }
+    }
+}
+
+// One launch per LL class: waves stride over the class's quartets (a list
+// of positions in reference order) and add their sums into the per-wave
+// partial rows, so the summation order is fixed for a given wave count.
+template <int LL>
+__global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __restrict__ idx,
+                                                 int64_t nidx, int first) {
+    __shared__ double scr[SCR_STRIDE * WAVE];
+    __shared__ double At[3 * (LL + 1) * (LL + 1)];
+    const int lane = threadIdx.x;
+    double Sacc[QPL], Scmp[QPL];
+#pragma unroll
+    for (int s = 0; s < QPL; ++s) Sacc[s] = Scmp[s] = 0.0;
+    const int64_t W = gridDim.x;
+    for (int64_t k = blockIdx.x; k < nidx; k += W)
+        quartet_work<LL>(A, idx[k], scr, At, Sacc, Scmp, lane);
+#pragma unroll
+    for (int s = 0; s < QPL; ++s) {
+        int iql = lane + WAVE * s;
+        if (iql < A.nqc) {
+            int64_t o = (int64_t)blockIdx.x * A.nqc + iql;
+            if (first) {
+                A.part[o] = Sacc[s];
+                A.partc[o] = Scmp[s];
+            } else {
+                double ps = A.part[o], pc = A.partc[o];
+                kahan_add(ps, pc, Sacc[s]);
+                kahan_add(ps, pc, -Scmp[s]);
+                A.part[o] = ps;
+                A.partc[o] = pc;
+            }
+        }
+    }
+}
+
+__global__ void k_reduce(const double* __restrict__ part,
+                         const double* __restrict__ partc, int waves, int nqc,
+                         double* __restrict__ out) {
+    int iq = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iq >= nqc) return;
+    double s = 0.0, c = 0.0;
+    for (int w = 0; w < waves; ++w) {
+        kahan_add(s, c, part[(int64_t)w * nqc + iq]);
+        kahan_add(s, c, -partc[(int64_t)w * nqc + iq]);
+    }
+    out[iq] = s;
+}
+
+// ---------------------------------------------------------------- launchers
+static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t launch_pair_info(const GroupInfo* G, PairInfo* P, int npair, hipStream_t s) {
+    hipLaunchKernelGGL(k_pair_info, dim3(nblk(npair, 128)), dim3(128), 0, s, G, P, npair);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_E(const GroupInfo* G, const PairInfo* P, int npair, int maxl,
+                         double* E, hipStream_t s) {
+    int64_t n = (int64_t)npair * 3 * (maxl + 1) * (maxl + 1);
+    hipLaunchKernelGGL(k_pair_E, dim3(nblk(n, 128)), dim3(128), 0, s, G, P, npair, maxl, E);
+    return hipGetLastError();
+}
+
+hipError_t launch_e12(const PairInfo* P, int npair, const double* q, int nq, double* e12,
+                      float* e12f, hipStream_t s) {
+    int64_t n = (int64_t)npair * nq;
+    hipLaunchKernelGGL(k_e12, dim3(nblk(n, 256)), dim3(256), 0, s, P, npair, q, nq, e12, e12f);
+    return hipGetLastError();
+}
+
+hipError_t launch_enumerate(const EnumRow* rows, int nrows, const PairInfo* P, int ng,
+                            double cutoffcentre, uint64_t* out, hipStream_t s) {
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_enumerate, dim3(nrows), dim3(64), 0, s, rows, P, ng, cutoffcentre, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_quartets(const uint64_t* in, int64_t n, int32_t* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack_quartets, dim3(nblk(n, 256)), dim3(256), 0, s, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_series(const SeriesArgs& a, int LL, const int32_t* idx, int64_t nidx,
+                         int waves, int first, hipStream_t s) {
+    if (nidx <= 0) return hipSuccess;
+    switch (LL) {
+#define CASE(k)                                                                     \
+    case k:                                                                         \
+        hipLaunchKernelGGL(k_series<k>, dim3(waves), dim3(WAVE), 0, s, a, idx, nidx, first); \
+        break;
+        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* part, const double* partc, int waves, int nqc,
+                         double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3(nblk(nqc, 128)), dim3(128), 0, s, part, partc, waves, nqc, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_exclusive_scan_u8(const uint8_t* in, int64_t n, int64_t* out,
+                                    int64_t* total, hipStream_t s) {
+    // bsum lives right after `out` (caller allocates n + nb + 1 int64)
+    int64_t nb = (n + SCAN_T * SCAN_I - 1) / (SCAN_T * SCAN_I);
+    if (nb == 0) nb = 1;
+    int64_t* bsum = out + n;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, n, out, bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1), 0, s, bsum, nb, total);
+    if (n > 0)
+        hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, n, bsum);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const uint8_t* flags, const uint64_t* in, int64_t n,
+                          int64_t* scratch, uint64_t* out, int64_t* count_dev,
+                          hipStream_t s) {
+    hipError_t err = launch_exclusive_scan_u8(flags, n, scratch, count_dev, s);
+    if (err != hipSuccess) return err;
+    if (n > 0)
+        hipLaunchKernelGGL(k_compact, dim3(nblk(n, 256)), dim3(256), 0, s, flags, in, n, scratch, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_unique(const int32_t* mat, int64_t nrow, int ncol, const double* total,
+                         UniqueWork& w, int32_t* rmat, double* rtot, int32_t* iuni,
+                         int32_t* irec, int64_t* count_dev, hipStream_t s) {
+    hipLaunchKernelGGL(k_hash_init, dim3(nblk(w.hsize, 256)), dim3(256), 0, s, w.keys,
+                       w.minrow, w.cnt, w.hsize);
+    if (nrow > 0)
+        hipLaunchKernelGGL(k_hash_insert, dim3(nblk(nrow, 256)), dim3(256), 0, s, mat, nrow,
+                           ncol, w.keys, w.minrow, w.cnt, w.slot, w.hsize);
+    uint8_t* flags = reinterpret_cast<uint8_t*>(w.keys);  // keys are not needed any more
+    if (nrow > 0)
+        hipLaunchKernelGGL(k_first_flags, dim3(nblk(nrow, 256)), dim3(256), 0, s, w.slot,
+                           w.minrow, nrow, flags);
+    hipError_t err = launch_exclusive_scan_u8(flags, nrow, w.scan, count_dev, s);
+    if (err != hipSuccess) return err;
+    if (nrow > 0)
+        hipLaunchKernelGGL(k_unique_out, dim3(nblk(nrow, 256)), dim3(256), 0, s, mat, nrow, ncol,
+                           total, w.slot, w.minrow, w.cnt, w.scan, rmat, nrow, rtot, iuni, irec);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs(const int32_t* rmat, const double* rtot, int64_t nu,
+                     const int64_t* nu_dev, int64_t ldr, int norb, double* V, hipStream_t s) {
+    if (nu > 0)
+        hipLaunchKernelGGL(k_scatter_v, dim3(nblk(nu, 256)), dim3(256), 0, s, rmat, rtot, nu,
+                           nu_dev, ldr, norb, V);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs_matrix(const double* V, int norb, const int2* mopairs, int npmo,
+                            double* Gs, hipStream_t s) {
+    int64_t n = (int64_t)npmo * npmo;
+    hipLaunchKernelGGL(k_gs, dim3(nblk(n, 256)), dim3(256), 0, s, V, norb, mopairs, npmo, Gs);
+    return hipGetLastError();
+}
+
+hipError_t launch_tmat2(const double* mmod, int nmo, const int2* mopairs, int npmo,
+                        const int32_t* xa, const int32_t* xb, int64_t npp, double* T,
+                        hipStream_t s) {
+    int64_t n = (int64_t)npmo * npp;
+    hipLaunchKernelGGL(k_tmat, dim3(nblk(n, 256)), dim3(256), 0, s, mmod, nmo, mopairs, npmo,
+                       xa, xb, npp, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_zel(const double* mmod, int nmo, int norb, const double* onerdm, int ngto,
+                      double* Z, hipStream_t s) {
+    int64_t n = (int64_t)ngto * ngto;
+    hipLaunchKernelGGL(k_zel, dim3(nblk(n, 256)), dim3(256), 0, s, mmod, nmo, norb, onerdm,
+                       ngto, Z);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_float(const double* in, float* out, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_to_float, dim3(nblk(n, 256)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}

@@ -1,0 +1,123 @@
+// Internal declarations shared by the host and device translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/thistle_scatter.h"
+
+// Primitive shell ("group"): all cartesian components of one primitive share
+// centre and exponent (src/molden_reader_nikola.py:195-251).
+struct GroupInfo {
+    double x, y, z, ga;
+    int32_t start, count, L, pad;
+};
+
+// Group pair gi <= gj, the bra or ket of a quartet.
+struct PairInfo {
+    double Px, Py, Pz, gaP;
+    double pre, kab;           // (pi_sp/gaP)^1.5 and exp(-ga_i ga_j/gaP |AB|^2)
+    int32_t gi, gj, off, nab;  // off: start of the pair block in pair space
+};
+
+// Quartet entry packing: 10 bits per group index (ng <= 1024).
+#define QE_GI(e) ((int)((e)&1023u))
+#define QE_GJ(e) ((int)(((e) >> 10) & 1023u))
+#define QE_GK(e) ((int)(((e) >> 20) & 1023u))
+#define QE_GR(e) ((int)(((e) >> 30) & 1023u))
+#define QE_MULT(e) ((int)(((e) >> 40) & 15u))
+#define QE_BR(e) ((int)(((e) >> 44) & 1u))
+
+// Enumeration row: one inner loop of one of the five nests.
+struct EnumRow {
+    int32_t nest, i, j, k;  // fixed outer indices (unused ones = -1)
+    int32_t r0, count;      // inner index runs r0 .. r0+count-1
+    int64_t out;            // first output slot
+};
+
+static inline __host__ __device__ int pair_index(int gi, int gj, int ng) {
+    // gi <= gj, row-major upper triangle
+    return gi * ng - (gi * (gi - 1)) / 2 + (gj - gi);
+}
+
+// Host helpers (rdm_host.cpp)
+int thistle_twordm_dense_host(const int32_t* confs, int nconf, int lconfs,
+                              const double* civs, int nstates, int state1,
+                              int state2, double* tw);
+int64_t thistle_twordm_select_host(const double* tw, int norb, double cutoff,
+                                   int32_t* mat, double* total, int64_t cap);
+int thistle_onerdm_host(const int32_t* confs, int nconf, int lconfs,
+                        const double* civs, int nstates, int state1, int state2,
+                        double* onerdm);
+
+// Device launchers (kernels.hip)
+struct SeriesArgs {
+    const uint64_t* qlist;
+    int64_t nquart;
+    const GroupInfo* G;
+    const PairInfo* P;
+    const double* E;
+    int maxl, ng, ngto;
+    const char4* lxyz;
+    const double* q;
+    int nq, q0, nqc;          // full grid size, chunk start, chunk length
+    const double* e12;        // [npair][nq] fp64
+    const float* e12f;        // [npair][nq] fp32 (mixed mode)
+    const double* Z;          // total: NPP x NPP; elastic: ngto x ngto
+    const float* Zf;          // fp32 copy (mixed mode), may be null
+    int64_t ldz;
+    int elastic, mixed;
+    double cutoffz;
+    double* part;             // [waves][nqc] sums
+    double* partc;            // [waves][nqc] Kahan compensations
+    uint8_t* flags;           // [nquart] block passes the ztot screen
+};
+
+hipError_t launch_pair_info(const GroupInfo* G, PairInfo* P, int npair,
+                            hipStream_t s);
+hipError_t launch_pair_E(const GroupInfo* G, const PairInfo* P, int npair,
+                         int maxl, double* E, hipStream_t s);
+hipError_t launch_e12(const PairInfo* P, int npair, const double* q, int nq,
+                      double* e12, float* e12f, hipStream_t s);
+hipError_t launch_enumerate(const EnumRow* rows, int nrows, const PairInfo* P,
+                            int ng, double cutoffcentre, uint64_t* out,
+                            hipStream_t s);
+hipError_t launch_unpack_quartets(const uint64_t* in, int64_t n, int32_t* out,
+                                  hipStream_t s);
+// waves of one LL class; first = 1 overwrites the partial rows
+hipError_t launch_series(const SeriesArgs& a, int LL, const int32_t* idx,
+                         int64_t nidx, int waves, int first, hipStream_t s);
+hipError_t launch_reduce(const double* part, const double* partc, int waves,
+                         int nqc, double* out, hipStream_t s);
+hipError_t launch_compact(const uint8_t* flags, const uint64_t* in, int64_t n,
+                          int64_t* scratch, uint64_t* out, int64_t* count_dev,
+                          hipStream_t s);
+hipError_t launch_tmat2(const double* mmod, int nmo, const int2* mopairs,
+                        int npmo, const int32_t* xa, const int32_t* xb,
+                        int64_t npp, double* T, hipStream_t s);
+hipError_t launch_gs_matrix(const double* V, int norb, const int2* mopairs,
+                            int npmo, double* Gs, hipStream_t s);
+hipError_t launch_zel(const double* mmod, int nmo, int norb,
+                      const double* onerdm, int ngto, double* Z, hipStream_t s);
+hipError_t launch_to_float(const double* in, float* out, int64_t n,
+                           hipStream_t s);
+
+// unique rows + 2-RDM densification
+struct UniqueWork {
+    uint64_t* keys;   // hash keys
+    int32_t* minrow;  // first row per slot
+    int32_t* cnt;     // rows per slot
+    int32_t* slot;    // slot of each row
+    int64_t* scan;    // exclusive scan of first-occurrence flags
+    int64_t hsize;
+};
+hipError_t launch_unique(const int32_t* mat, int64_t nrow, int ncol,
+                         const double* total, UniqueWork& w, int32_t* rmat,
+                         double* rtot, int32_t* iuni, int32_t* irec,
+                         int64_t* count_dev, hipStream_t s);
+hipError_t launch_gs(const int32_t* rmat, const double* rtot, int64_t nu,
+                     const int64_t* nu_dev, int64_t ldr, int norb, double* V,
+                     hipStream_t s);
+hipError_t launch_exclusive_scan_u8(const uint8_t* in, int64_t n,
+                                    int64_t* out, int64_t* total,
+                                    hipStream_t s);

@@ -1,0 +1,92 @@
+"""The ``scattering_main_routine.main()`` flow, minus Molpro and the GUI.
+
+``prepare_inputs`` goes from a molden file plus a CI punch file to the
+positional arguments of ``total_scattering_calculation``, following
+src/scattering_main_routine.py:111-187,232-238; ``main`` then runs the call
+and writes ``{'q','I'}`` (src/scattering_main_routine.py:474).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import readers
+
+# defaults of src/scattering_main_routine.py:174-179
+CUTOFF_CENTRE = 0.001
+CUTOFF_Z = 1e-15
+CUTOFF_MD = 1e-20
+
+
+@dataclass
+class ScatteringInputs:
+    Zn: np.ndarray
+    geom: np.ndarray
+    ga: np.ndarray
+    l: np.ndarray
+    m: np.ndarray
+    n: np.ndarray
+    xx: np.ndarray
+    yy: np.ndarray
+    zz: np.ndarray
+    mmod: np.ndarray      # (nmo, ngto)
+    group: np.ndarray     # 1-based
+    confs: np.ndarray     # (nconf, 2*norb) int32
+    civs: np.ndarray      # (nconf, nstates)
+
+    @property
+    def ngto(self):
+        return int(self.l.size)
+
+    @property
+    def ng(self):
+        return int(self.group.max())
+
+    @property
+    def maxl(self):
+        return int(self.l.max())
+
+    @property
+    def norb(self):
+        return int(self.confs.shape[1] // 2)
+
+
+def prepare_inputs(mldfile, punfile, closed) -> ScatteringInputs:
+    civs, confs = readers.twordmconst(closed, punfile)
+    nmo_max = len(confs[0]) / 2
+    gtos, atoms = readers.read_orbitals(mldfile, N=nmo_max, decontract=True)
+    return ScatteringInputs(
+        Zn=atoms.atomic_numbers(), geom=atoms.geometry(),
+        ga=np.asarray(gtos.ga, dtype=np.float64),
+        l=np.asarray(gtos.l, dtype=np.int32), m=np.asarray(gtos.m, dtype=np.int32),
+        n=np.asarray(gtos.n, dtype=np.int32),
+        xx=np.asarray(gtos.x, dtype=np.float64), yy=np.asarray(gtos.y, dtype=np.float64),
+        zz=np.asarray(gtos.z, dtype=np.float64),
+        mmod=np.ascontiguousarray(np.transpose(gtos.mo), dtype=np.float64),
+        group=np.asarray(gtos.group, dtype=np.int32),
+        confs=readers.confs_to_int(confs), civs=readers.civs_to_array(civs))
+
+
+def default_q(qmax=10.0, nq=200):
+    q = np.linspace(0.00000000001, qmax, nq)
+    if q[0] < 1e-10:
+        q[0] = 1e-10
+    return q
+
+
+def main(mldfile, punfile, closed, q=None, type_=1, state1=1, state2=1,
+         state3=1, out=None, **kw):
+    """Run one total/elastic call the way the reference driver does."""
+    from .scattering import total_scattering_calculation
+    inp = prepare_inputs(mldfile, punfile, closed)
+    q = default_q() if q is None else np.asarray(q, dtype=np.float64)
+    q_abs, res = total_scattering_calculation(
+        type_, inp.Zn, inp.geom, state1, state2, state3, inp.maxl, inp.ngto,
+        inp.ng, inp.ga, inp.l, inp.m, inp.n, inp.xx, inp.yy, inp.zz, inp.mmod,
+        q, q.size, inp.group, CUTOFF_Z, CUTOFF_MD, CUTOFF_CENTRE, inp.confs,
+        inp.civs, **kw)
+    if out is not None:
+        import scipy.io as sio
+        sio.savemat(out, {"q": q, "I": res})
+    return res, q, q_abs

@@ -1,0 +1,514 @@
+"""Input readers of the scattering path, behaviour-compatible with the reference.
+
+* Molden (Molpro flavour): ``read_orbitals`` restates
+  ``src/molden_reader_nikola.py:496-508`` (``_read_MO_GTOs`` :470-493) with the
+  same parsing rules, MO ordering, primitive normalisation and decontraction.
+* CI punch file: ``twordmconst`` restates ``src/twordm_red.py:12-139`` and
+  ``confs_to_int`` the string -> integer conversion of
+  ``src/scattering_main_routine.py:232-238``.
+* 2-RDM text file: ``read_twordm`` restates ``src/one_rdm.f90:2112-2146``;
+  ``write_twordm`` the writer format of ``src/twordms.f90:521-539``.
+* ``Main.f90`` text route: ``read_basis_dat``/``read_mos_dat``/
+  ``read_options_dat`` follow ``src/Main.f90:28-112`` and ``write_result``
+  ``src/Main.f90:192-198``.
+"""
+from __future__ import annotations
+
+import copy
+import math
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+AU2ANG = 0.52917721092  # src/molden_reader_nikola.py:16
+
+# cartesian component order per shell letter, src/molden_reader_nikola.py:199-238
+_ANG = {
+    "s": [(0, 0, 0)],
+    "p": [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
+    "d": [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)],
+    "f": [(3, 0, 0), (0, 3, 0), (0, 0, 3), (1, 2, 0), (2, 1, 0), (2, 0, 1),
+          (1, 0, 2), (0, 1, 2), (0, 2, 1), (1, 1, 1)],
+    "g": [(4, 0, 0), (0, 4, 0), (0, 0, 4), (3, 1, 0), (3, 0, 1), (1, 3, 0),
+          (0, 3, 1), (1, 0, 3), (0, 1, 3), (2, 2, 0), (2, 0, 2), (0, 2, 2),
+          (2, 1, 1), (1, 2, 1), (1, 1, 2)],
+}
+
+
+def factd(n: int) -> int:
+    """Double factorial with factd(n<=0) = 1 (src/molden_reader_nikola.py:347-351)."""
+    r = 1
+    while n > 0:
+        r *= n
+        n -= 2
+    return r
+
+
+def gto_norm(ga: float, l: int, m: int, n: int) -> float:
+    """Primitive cartesian normalisation (src/molden_reader_nikola.py:53-61)."""
+    normA = (2 / math.pi) ** 0.75
+    normB = 2 ** (l + m + n)
+    normC = ga ** ((2 * l + 2 * m + 2 * n + 3) / 4.0)
+    normD = factd(2 * l - 1) * factd(2 * m - 1) * factd(2 * n - 1)
+    return normA * normB * normC / normD ** 0.5
+
+
+@dataclass
+class Atom:
+    atomic_number: int
+    x: float
+    y: float
+    z: float
+
+
+@dataclass
+class Molecule:
+    """Atoms in bohr (src/molecule.py:89-104 keeps the same fields)."""
+    atoms: list = field(default_factory=list)
+
+    def atomic_numbers(self) -> np.ndarray:
+        return np.array([a.atomic_number for a in self.atoms], dtype=np.float64)
+
+    def geometry(self) -> np.ndarray:
+        return np.array([[a.x, a.y, a.z] for a in self.atoms], dtype=np.float64)
+
+
+@dataclass
+class _Prim:
+    atom_idx: int
+    contraction: int | None
+    group: int
+    ga: float
+    c: float | None
+    l: int
+    m: int
+    n: int
+    mo: object = None
+    x: float | None = None
+    y: float | None = None
+    z: float | None = None
+
+
+@dataclass
+class ArrayGTO:
+    """Primitive cartesian GTOs and their MO coefficients (mo: ngto x nMO)."""
+    mo: np.ndarray
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    ga: np.ndarray
+    l: np.ndarray
+    m: np.ndarray
+    n: np.ndarray
+    group: np.ndarray
+    occ: np.ndarray | None = None
+    energy: np.ndarray | None = None
+    syms: np.ndarray | None = None
+
+
+def _read_atoms(lines):
+    """[Atoms] block; coordinates are always divided by AU2ANG (:80-115)."""
+    it = iter(lines)
+    for line in it:
+        if "[Atoms]" in line:
+            break
+    else:
+        raise ValueError("molden: no [Atoms] section")
+    mol = Molecule()
+    for line in it:
+        if "[GTO]" in line:
+            break
+        cols = line.split()
+        mol.atoms.append(Atom(int(cols[2]), float(cols[3]) / AU2ANG,
+                              float(cols[4]) / AU2ANG, float(cols[5]) / AU2ANG))
+    else:
+        raise ValueError("molden: no [GTO] section")
+    return mol
+
+
+def _read_contractions(lines):
+    """[GTO] block -> list of primitives (src/molden_reader_nikola.py:133-255).
+
+    One group per primitive shared by its cartesian components; the
+    contraction index of each component is its MO-table row.  A single blank
+    line starts a new atom, two blank lines end the section.
+    """
+    i = 0
+    while i < len(lines) and "[GTO]" not in lines[i]:
+        i += 1
+    if i == len(lines):
+        raise ValueError("molden: no [GTO] section")
+    i += 1
+    atm = [int(s) for s in lines[i].split() if s.isdigit()][0]
+    i += 1
+    gtos = []
+    contraction_counter = 1
+    group_counter = 0
+    while i < len(lines):
+        line = lines[i]
+        i += 1
+        if "[MO]" in line:
+            break
+        if not line or line.isspace():
+            nxt = lines[i] if i < len(lines) else ""
+            i += 1
+            if not nxt or nxt.isspace():
+                continue
+            atm = [int(s) for s in nxt.split() if s.isdigit()][0]
+            line = lines[i]
+            i += 1
+        spec = line.split()
+        kind = spec[0].lower()
+        nprim = int(spec[1])
+        if kind not in _ANG:
+            raise ValueError(f"molden: unsupported shell '{spec[0]}'")
+        ang = _ANG[kind]
+        for _ in range(nprim):
+            pl = lines[i].replace("D", "E")
+            i += 1
+            g = float(pl.split()[0])
+            c = float(pl.split()[1])
+            group_counter += 1
+            for k, (l, m, n) in enumerate(ang):
+                gtos.append(_Prim(atm, contraction_counter + k, group_counter,
+                                  g, c, l, m, n))
+        contraction_counter += len(ang)
+    return gtos
+
+
+_FLOAT_RE = re.compile(r"\d+\.\d+")
+
+
+def _read_mo(lines, mo_cutoff):
+    """[MO] block (src/molden_reader_nikola.py:258-344).
+
+    Reads up to ``mo_cutoff`` orbitals, keeps the occupied subset and orders
+    it by the key frac(sym)*1000 + int(sym) with numpy's argsort.
+    """
+    i = 0
+    while i < len(lines) and "[MO]" not in lines[i]:
+        i += 1
+    if i == len(lines):
+        raise ValueError("molden: no [MO] section")
+    i += 1
+    syms, energy, occ, table = [], [], [], []
+    mo_counter = 0
+    this = []
+    while i < len(lines) and mo_counter <= mo_cutoff and "[FREQ]" not in lines[i]:
+        line = lines[i]
+        if "Sym" in line:
+            mo_counter += 1
+            if mo_counter <= mo_cutoff:
+                s = _FLOAT_RE.findall(line)
+                syms.append(s[0] if s else 1.1)
+            if this:
+                table.append(this)
+            this = []
+        elif "Ene" in line:
+            energy.append(float(_FLOAT_RE.findall(line)[0]))
+        elif "Spin" in line:
+            pass
+        elif "Occup" in line:
+            occ.append(float(_FLOAT_RE.findall(line)[0]))
+        elif line.strip():
+            this.append(float(line.split()[1]))
+        i += 1
+    if this:
+        table.append(this)
+    syms_array = np.array([float(s) for s in syms])
+    occ_array = np.array([float(o) for o in occ])
+    finalind = np.where(occ_array > 0.0)
+    keys = syms_array[finalind]
+    keys = (keys - keys.astype(int)) * 1000 + keys.astype(int)
+    idx = np.argsort(keys)
+    mo = np.array(table)[idx, :]
+    energy_array = np.array(energy)[idx]
+    occ_array = np.array(occ)[idx]
+    return mo.T, occ_array, energy_array, syms_array
+
+
+def _reverse_contraction(gtos):
+    """Merge primitives on the same atom with the same exponent and l,m,n by
+    summing MO vectors, then renumber groups 1,2,3,... by changes between
+    consecutive entries (src/molden_reader_nikola.py:382-422)."""
+    first = copy.copy(gtos[0])
+    first.mo = 0
+    contracted = [first]
+    for g in gtos:
+        for j in contracted:
+            if (g.atom_idx == j.atom_idx and g.ga == j.ga and g.l == j.l
+                    and g.m == j.m and g.n == j.n):
+                j.mo = j.mo + g.mo
+                break
+        else:
+            t = copy.copy(g)
+            t.contraction = None
+            t.c = None
+            t.mo = g.mo
+            contracted.append(t)
+    counter = 1
+    prev = contracted[0].group
+    for k in range(1, len(contracted)):
+        if contracted[k].group != prev:
+            counter += 1
+            prev = contracted[k].group
+        contracted[k].group = counter
+    return contracted
+
+
+def read_orbitals(file, N=10000, decontract=False):
+    """Primitive GTOs + MOs from a Molpro-style molden file.
+
+    Mirrors ``read_orbitals`` (src/molden_reader_nikola.py:496-508): returns
+    ``(ArrayGTO, Molecule)`` when given a file name, ``ArrayGTO`` when given an
+    open file.
+    """
+    if isinstance(file, str):
+        with open(file, "r") as f:
+            text = f.read()
+        gtos = _read_mo_gtos(text.splitlines(keepends=True), N, decontract)
+        return gtos, _read_atoms(text.splitlines(keepends=True))
+    text = file.read()
+    return _read_mo_gtos(text.splitlines(keepends=True), N, decontract)
+
+
+def _read_mo_gtos(lines, N, decontract):
+    mol = _read_atoms(lines)
+    gtos = _read_contractions(lines)
+    mo_table, occ, energy, syms = _read_mo(lines, N)
+    for g in gtos:
+        g.mo = mo_table[g.contraction - 1, :]
+    for g in gtos:
+        g.mo = g.c * gto_norm(g.ga, g.l, g.m, g.n) * g.mo
+        g.c = None
+    if decontract:
+        gtos = _reverse_contraction(gtos)
+    for g in gtos:
+        a = mol.atoms[g.atom_idx - 1]
+        g.x, g.y, g.z = a.x, a.y, a.z
+    ng = len(gtos)
+    arr = ArrayGTO(
+        mo=np.array([np.asarray(g.mo, dtype=np.float64) for g in gtos]).reshape(ng, -1),
+        x=np.array([g.x for g in gtos], dtype=np.float64),
+        y=np.array([g.y for g in gtos], dtype=np.float64),
+        z=np.array([g.z for g in gtos], dtype=np.float64),
+        ga=np.array([g.ga for g in gtos], dtype=np.float64),
+        l=np.array([g.l for g in gtos], dtype=np.int32),
+        m=np.array([g.m for g in gtos], dtype=np.int32),
+        n=np.array([g.n for g in gtos], dtype=np.int32),
+        group=np.array([g.group for g in gtos], dtype=np.int32),
+    )
+    arr.occ, arr.energy, arr.syms = occ, energy, syms
+    return arr
+
+
+# ----------------------------------------------------------------- CI punch
+def twordmconst(closed, fileread):
+    """CI vectors and determinant strings from a Molpro punch-style file.
+
+    Restates the no-symmetry branch of ``twordmconst`` (src/twordm_red.py:
+    12-86); the multi-irrep branches are not part of this path and raise.
+    Returns ``(civs, confs)``: civs as lists of coefficient strings per
+    determinant and confs as spin-orbital strings ('a' = alpha, 'b' = beta,
+    '0' = empty, two characters per orbital).
+    """
+    if isinstance(closed, list):
+        ncirr = len(closed)
+        if ncirr == 1:
+            closed = closed[0]
+    else:
+        ncirr = 1
+    firstline = None
+    with open(fileread, "r") as fh:
+        for line in fh:
+            if line[0].isupper():
+                continue
+            if line[0] not in "*?-":
+                firstline = line.strip().split()
+                break
+    if firstline is None:
+        raise ValueError("punch file holds no determinant lines")
+    nirr = -1
+    for element in firstline:
+        nirr += 1
+        if "." in element:
+            break
+    if ncirr != nirr:
+        raise ValueError(
+            f"mismatch between the closed orbital entry and the number of irreps "
+            f"in the CI vector: {nirr} entries are required")
+    if nirr != 1:
+        raise NotImplementedError("symmetry-blocked CI vectors are not supported")
+    civs, confs = [], []
+    with open(fileread, "r") as fh:
+        for line in fh:
+            if line[0].isupper() or line[0] in "*?-":
+                continue
+            NN = line.strip().split()
+            if not NN:
+                continue
+            if "." not in NN[0]:
+                sdet = "c" * closed + NN[0]
+                sdet = (sdet.replace("0", "00").replace("a", "a0")
+                        .replace("b", "0b").replace("2", "ab"))
+                sdet = sdet.replace("c", "ab")
+                confs.append(sdet)
+                civs.append(NN[1:])
+            else:
+                civs[-1].extend(NN)
+    return civs, confs
+
+
+def confs_to_int(confs) -> np.ndarray:
+    """'a' -> 1, 'b' -> 2 per character (src/scattering_main_routine.py:232-238)."""
+    out = np.zeros((len(confs), len(confs[0])), dtype=np.int32)
+    for i, s in enumerate(confs):
+        out[i, :] = np.asarray(list(s.replace("a", "1").replace("b", "2")),
+                               dtype=np.int32)
+    return out
+
+
+def civs_to_array(civs) -> np.ndarray:
+    return np.array([[float(v) for v in row] for row in civs], dtype=np.float64)
+
+
+# ----------------------------------------------------------- 2-RDM files
+def read_twordm(filename):
+    """``p s q r value`` rows (src/one_rdm.f90:2112-2146) -> (mat int32[n,4], total f64[n])."""
+    mats, tots = [], []
+    with open(filename, "r") as fh:
+        for line in fh:
+            tok = line.split()
+            if not tok:
+                continue
+            mats.append([int(t) for t in tok[:4]])
+            tots.append(float(tok[4].replace("D", "E")))
+    return np.array(mats, dtype=np.int32).reshape(-1, 4), np.array(tots, dtype=np.float64)
+
+
+def write_twordm(filename, mat, total):
+    """Same column layout as twordm_fortran.dat (src/twordms.f90:521-539)."""
+    with open(filename, "w") as fh:
+        for row, v in zip(np.asarray(mat), np.asarray(total)):
+            fh.write(f"{int(row[0]):12d}{int(row[1]):12d}{int(row[2]):12d}{int(row[3]):12d}   {float(v)!r}\n")
+
+
+# ------------------------------------------------------- Main.f90 text route
+def write_basis_dat(path, xx, yy, zz, ga, l, m, n, group):
+    with open(path, "w") as f:
+        f.write(f"{len(l)}\n")
+        for i in range(len(l)):
+            f.write(f"{float(xx[i])!r} {float(yy[i])!r} {float(zz[i])!r} {float(ga[i])!r} "
+                    f"{int(l[i])} {int(m[i])} {int(n[i])} {int(group[i])}\n")
+
+
+def read_basis_dat(path):
+    """basis.dat: ngtos, then x y z ga l m n group per line (src/Main.f90:28-36)."""
+    with open(path) as f:
+        tok = f.read().split()
+    ng = int(tok[0])
+    rows = np.array(tok[1:1 + 8 * ng], dtype=object).reshape(ng, 8)
+    xx, yy, zz, ga = (rows[:, k].astype(np.float64) for k in range(4))
+    l, m, n, group = (rows[:, k].astype(np.int32) for k in range(4, 8))
+    return xx, yy, zz, ga, l, m, n, group
+
+
+def write_mos_dat(path, mmod):
+    mmod = np.asarray(mmod)
+    with open(path, "w") as f:
+        f.write(f"{mmod.shape[0]} {mmod.shape[1]}\n")
+        for i in range(mmod.shape[0]):
+            f.write(" ".join(repr(float(v)) for v in mmod[i]) + "\n")
+
+
+def read_mos_dat(path):
+    """MOs.dat: norbs ngtos, then one MO per row (src/Main.f90:40-48)."""
+    with open(path) as f:
+        tok = f.read().split()
+    norbs, ngtos = int(tok[0]), int(tok[1])
+    return np.array(tok[2:2 + norbs * ngtos], dtype=np.float64).reshape(norbs, ngtos)
+
+
+def linspace_1(qmin, qmax, npoints):
+    """src/linspace.f90:5-22: from + range*(i-1)/(n-1)."""
+    rng = qmax - qmin
+    return np.array([qmin + rng * (i - 1) / (npoints - 1) for i in range(1, npoints + 1)])
+
+
+@dataclass
+class Options:
+    atoms: np.ndarray
+    geom: np.ndarray
+    cutoffcentre: float
+    cutoffz: float
+    cutoffmd: float
+    jeremyR: bool
+    mcci: bool
+    hf: bool
+    qmin: float
+    qmax: float
+    npoints: int
+    typec: int
+    state1: int
+    state2: int
+    state3: int
+    file_out: str
+    molpro: bool
+    molcas: bool
+    bagel: bool
+    confs: np.ndarray | None = None
+    civs: np.ndarray | None = None
+
+
+def _flog(s: str) -> bool:
+    s = s.strip().strip(".").upper()
+    return s.startswith("T")
+
+
+def read_options_dat(path) -> Options:
+    """options.dat positional records (src/Main.f90:47-112), list-directed."""
+    with open(path) as f:
+        recs = [ln.strip() for ln in f if ln.strip()]
+    k = 0
+
+    def nxt():
+        nonlocal k
+        k += 1
+        return recs[k - 1]
+
+    natoms = int(nxt().split()[0])
+    atoms = np.array(nxt().split()[:natoms], dtype=np.float64)
+    geom = np.array([nxt().split()[:3] for _ in range(natoms)], dtype=np.float64)
+    cc = float(nxt().split()[0].replace("D", "E"))
+    cz = float(nxt().split()[0].replace("D", "E"))
+    cm = float(nxt().split()[0].replace("D", "E"))
+    jr = _flog(nxt())
+    mc = _flog(nxt())
+    hf = _flog(nxt())
+    qt = nxt().split()
+    typec = int(nxt().split()[0])
+    st = [int(v) for v in nxt().split()[:3]]
+    fout = nxt().split()[0].strip("'\"")
+    molpro, molcas, bagel = _flog(nxt()), _flog(nxt()), _flog(nxt())
+    opt = Options(atoms, geom, cc, cz, cm, jr, mc, hf, float(qt[0]), float(qt[1]),
+                  int(qt[2]), typec, st[0], st[1], st[2], fout, molpro, molcas, bagel)
+    if molpro or molcas or bagel:
+        nconfs = int(nxt().split()[0])
+        lconfs = int(nxt().split()[0])
+        ncivs = int(nxt().split()[0])
+        if molpro and nconfs >= 2:
+            raise NotImplementedError("bitwise determinant files are not part of this path")
+        vals = " ".join(recs[k:]).split()
+        need = nconfs * (lconfs + ncivs)
+        arr = np.array(vals[:need], dtype=np.float64).reshape(nconfs, lconfs + ncivs)
+        opt.confs = arr[:, :lconfs].astype(np.int32)
+        opt.civs = arr[:, lconfs:]
+    return opt
+
+
+def write_result(path, q, result):
+    """'q I' per line (src/Main.f90:192-198)."""
+    with open(path, "w") as f:
+        for a, b in zip(q, result):
+            f.write(f" {float(a)!r} {float(b)!r}\n")

@@ -1,0 +1,251 @@
+"""Python mirror of ``main_calculation_mod`` for the total/elastic j0 path.
+
+``total_scattering_calculation`` takes the positional arguments of the
+Fortran procedure (src/main_calculation.f90:13-15, the form with ``state3``
+that ``src/Main.f90:171-173`` calls) and returns ``(q_abs, result)`` like the
+f2py binding used at src/scattering_main_routine.py:240-247.  Types 1 (total
+j0) and 2 (elastic j0) run on the GPU through libthistle_hip.so; the other
+types of the reference are outside this path and raise.
+
+Beyond the reference arguments, keyword ``precision`` selects FP64 (default)
+or the mixed-precision kernels, and ``rdm_cutoff`` the |Gamma| selection
+threshold of the 2-RDM (the reference hard-codes 1e-30).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+
+
+def _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
+           maxl=None, ng=None):
+    return L.make_basis(ga, l, m, n, xx, yy, zz, group, mmod, q, cutoffz, cutoffmd,
+                        cutoffcentre, maxl=maxl, ng=ng)
+
+
+def total_scattering_calculation(type, Zn, geom, state1, state2, state3, maxl, ngto, ng,
+                                 ga, l, m, n, xx, yy, zz, mmod, q, nq, group, cutoffz,
+                                 cutoffmd, cutoffcentre, confs, civecs, *, precision=L.FP64,
+                                 rdm_cutoff=1e-30, waves=0):
+    """Drop-in for main_calculation_mod::total_scattering_calculation (types 1, 2)."""
+    lib = L.load()
+    q = np.asarray(q, dtype=np.float64)
+    if int(nq) != q.size or int(ngto) != np.asarray(l).size:
+        raise ValueError("nq / ngto do not match the array sizes")
+    if int(type) not in (1, 2):
+        raise NotImplementedError(
+            f"type {type}: only 1 (total j0) and 2 (elastic j0) are on this path")
+    b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
+               maxl=maxl, ng=ng)
+    o = L.make_opts(precision, rdm_cutoff, waves=waves)
+    confs = np.require(np.asarray(confs), dtype=np.int32, requirements=["F"])
+    civ = np.require(np.asarray(civecs, dtype=np.float64), requirements=["F"])
+    if civ.ndim == 1:
+        civ = civ.reshape(-1, 1, order="F")
+    Zn = np.require(np.atleast_1d(np.asarray(Zn, dtype=np.float64)), requirements=["C"])
+    geom = np.require(np.asarray(geom, dtype=np.float64).reshape(-1, 3), requirements=["F"])
+    q_abs = np.zeros(q.size)
+    res = np.zeros(q.size)
+    rc = lib.thistle_total_scattering_calculation(
+        int(type), int(Zn.size), L._ptr(Zn), L._ptr(geom), int(state1), int(state2),
+        int(state3), C.byref(b), int(confs.shape[0]), int(confs.shape[1]), int(civ.shape[1]),
+        L._ptr(confs), L._ptr(civ), C.byref(o), L._ptr(q_abs), L._ptr(res))
+    L.check(rc, "total_scattering_calculation")
+    return q_abs, res
+
+
+def total_from_twordm(ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total, cutoffz=1e-15,
+                      cutoffmd=1e-20, cutoffcentre=0.001, *, precision=L.FP64, waves=0,
+                      maxl=None):
+    """readrdm route: S(q) of the total j0 path from 2-RDM rows (p,s,q,r) + values."""
+    lib = L.load()
+    b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
+               maxl=maxl)
+    o = L.make_opts(precision, waves=waves)
+    mat = np.require(np.asarray(mat), dtype=np.int32, requirements=["F"])
+    total = np.require(np.asarray(total, dtype=np.float64), requirements=["C"])
+    res = np.zeros(np.asarray(q).size)
+    rc = lib.thistle_total_from_twordm(C.byref(b), int(total.size), L._ptr(mat),
+                                       L._ptr(total), C.byref(o), L._ptr(res))
+    L.check(rc, "total_from_twordm")
+    return res
+
+
+def elastic_from_onerdm(ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm, cutoffz=1e-15,
+                        cutoffmd=1e-20, cutoffcentre=0.001, *, precision=L.FP64, waves=0,
+                        maxl=None):
+    lib = L.load()
+    b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
+               maxl=maxl)
+    o = L.make_opts(precision, waves=waves)
+    g = np.require(np.asarray(onerdm, dtype=np.float64), requirements=["F"])
+    res = np.zeros(np.asarray(q).size)
+    rc = lib.thistle_elastic_from_onerdm(C.byref(b), int(g.shape[0]), L._ptr(g), C.byref(o),
+                                         L._ptr(res))
+    L.check(rc, "elastic_from_onerdm")
+    return res
+
+
+# --------------------------------------------------- 2-RDM / 1-RDM (host)
+def createtwordm(confs, civs, state1=1, state2=1, cutoff=1e-30):
+    """maxcoincidence + createtwordm: rows (p,s,q,r) and values (src/twordms.f90)."""
+    lib = L.load()
+    confs = np.require(np.asarray(confs), dtype=np.int32, requirements=["F"])
+    civs = np.require(np.asarray(civs, dtype=np.float64), requirements=["F"])
+    if civs.ndim == 1:
+        civs = civs.reshape(-1, 1, order="F")
+    nconf, lconfs = confs.shape
+    norb = lconfs // 2
+    tw = np.zeros((norb,) * 4, order="F")
+    L.check(lib.thistle_twordm_dense(L._ptr(confs), nconf, lconfs, L._ptr(civs), civs.shape[1],
+                                     state1, state2, L._ptr(tw)), "twordm_dense")
+    cnt = lib.thistle_twordm_select(L._ptr(tw), norb, cutoff, None, None, 0)
+    mat = np.zeros((cnt, 4), dtype=np.int32, order="F")
+    tot = np.zeros(cnt)
+    lib.thistle_twordm_select(L._ptr(tw), norb, cutoff, L._ptr(mat), L._ptr(tot), cnt)
+    return np.ascontiguousarray(mat), tot
+
+
+def onerdm(confs, civs, state1=1, state2=1):
+    lib = L.load()
+    confs = np.require(np.asarray(confs), dtype=np.int32, requirements=["F"])
+    civs = np.require(np.asarray(civs, dtype=np.float64), requirements=["F"])
+    if civs.ndim == 1:
+        civs = civs.reshape(-1, 1, order="F")
+    norb = confs.shape[1] // 2
+    g = np.zeros((norb, norb), order="F")
+    L.check(lib.thistle_onerdm(L._ptr(confs), confs.shape[0], confs.shape[1], L._ptr(civs),
+                               civs.shape[1], state1, state2, L._ptr(g)), "onerdm")
+    return g
+
+
+# ----------------------------------------------------------- device stages
+def quartet_list(ga, l, m, n, xx, yy, zz, group, q, cutoffcentre=0.001):
+    """Reference-order quartet list computed on the GPU: int32 [Q, 6]."""
+    lib = L.load()
+    b = _basis(ga, l, m, n, xx, yy, zz, None, q, group, 0.0, 0.0, cutoffcentre)
+    o = L.make_opts()
+    cnt = lib.thistle_quartet_list(C.byref(b), None, 0, C.byref(o))
+    if cnt < 0:
+        raise L.ThistleError(-cnt, "quartet_list")
+    out = np.zeros((cnt, 6), dtype=np.int32)
+    cnt2 = lib.thistle_quartet_list(C.byref(b), L._ptr(out), cnt, C.byref(o))
+    if cnt2 < 0:
+        raise L.ThistleError(-cnt2, "quartet_list")
+    return out
+
+
+def quartet_list_screened(type_, ga, l, m, n, xx, yy, zz, mmod, q, group, mat=None,
+                          total=None, onerdm=None, cutoffz=1e-15, cutoffcentre=0.001):
+    lib = L.load()
+    b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, 0.0, cutoffcentre)
+    o = L.make_opts()
+    mat_ = np.require(np.asarray(mat if mat is not None else np.zeros((0, 4))), dtype=np.int32,
+                      requirements=["F"])
+    tot_ = np.require(np.asarray(total if total is not None else np.zeros(0),
+                                 dtype=np.float64), requirements=["C"])
+    g = None if onerdm is None else np.require(np.asarray(onerdm, dtype=np.float64),
+                                               requirements=["F"])
+    norb = 0 if g is None else g.shape[0]
+    args = (int(type_), C.byref(b), int(tot_.size), L._ptr(mat_), L._ptr(tot_), norb, L._ptr(g))
+    cnt = lib.thistle_quartet_list_screened(*args, None, 0, C.byref(o))
+    if cnt < 0:
+        raise L.ThistleError(-cnt, "quartet_list_screened")
+    out = np.zeros((cnt, 6), dtype=np.int32)
+    cnt2 = lib.thistle_quartet_list_screened(*args, L._ptr(out), cnt, C.byref(o))
+    if cnt2 < 0:
+        raise L.ThistleError(-cnt2, "quartet_list_screened")
+    return out
+
+
+def unique_total(matrix, total):
+    """GPU unique_total: (rmat, rtot, iuni, irec) with 1-based iuni/irec."""
+    lib = L.load()
+    mat = np.require(np.asarray(matrix), dtype=np.int32, requirements=["F"])
+    tot = np.require(np.asarray(total, dtype=np.float64), requirements=["C"])
+    nrow, ncol = mat.shape
+    rmat = np.zeros((max(nrow, 1), ncol), dtype=np.int32, order="F")
+    rtot = np.zeros(max(nrow, 1))
+    iuni = np.zeros(max(nrow, 1), dtype=np.int32)
+    irec = np.zeros(max(nrow, 1), dtype=np.int32)
+    o = L.make_opts()
+    u = lib.thistle_unique_total(L._ptr(mat), nrow, ncol, L._ptr(tot), L._ptr(rmat),
+                                 L._ptr(rtot), L._ptr(iuni), L._ptr(irec), C.byref(o))
+    if u < 0:
+        raise L.ThistleError(-u, "unique_total")
+    rm = np.asfortranarray(rmat.reshape(-1, order="F")[: u * ncol].reshape(u, ncol, order="F"))
+    return np.ascontiguousarray(rm), rtot[:u].copy(), iuni[:u].copy(), irec[:nrow].copy()
+
+
+def pair_tables(ga, l, m, n, xx, yy, zz, group, q, maxl=None):
+    """E[p][axis][la][lb][t], P[p][3], e12[p][nq] for group pairs gi <= gj."""
+    lib = L.load()
+    b = _basis(ga, l, m, n, xx, yy, zz, None, q, group, 0.0, 0.0, 0.001, maxl=maxl)
+    ng, mx, nq = b.ng, b.maxl, b.nq
+    mxe = max(mx, int(np.max(m)), int(np.max(n)))
+    npair = ng * (ng + 1) // 2
+    E = np.zeros((npair, 3, mxe + 1, mxe + 1, 2 * mxe + 1))
+    P = np.zeros((npair, 3))
+    e12 = np.zeros((npair, nq))
+    L.check(lib.thistle_pair_tables(C.byref(b), L._ptr(E), L._ptr(P), L._ptr(e12),
+                                    C.byref(L.make_opts())), "pair_tables")
+    return E, P, e12
+
+
+class Plan:
+    """Device-resident problem: repeated executions reuse every HBM buffer."""
+
+    def __init__(self, type_, ga, l, m, n, xx, yy, zz, mmod, q, group, mat=None, total=None,
+                 onerdm=None, cutoffz=1e-15, cutoffmd=1e-20, cutoffcentre=0.001,
+                 precision=L.FP64, waves=0, maxl=None):
+        self.lib = L.load()
+        self._b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd,
+                         cutoffcentre, maxl=maxl)
+        self._o = L.make_opts(precision, waves=waves)
+        self.nq = int(np.asarray(q).size)
+        self._mat = np.require(np.asarray(mat if mat is not None else np.zeros((0, 4))),
+                               dtype=np.int32, requirements=["F"])
+        self._tot = np.require(np.asarray(total if total is not None else np.zeros(0),
+                                          dtype=np.float64), requirements=["C"])
+        self._g = None if onerdm is None else np.require(
+            np.asarray(onerdm, dtype=np.float64), requirements=["F"])
+        h = C.c_void_p()
+        rc = self.lib.thistle_plan_create(
+            int(type_), C.byref(self._b), int(self._tot.size), L._ptr(self._mat),
+            L._ptr(self._tot), 0 if self._g is None else self._g.shape[0], L._ptr(self._g),
+            C.byref(self._o), C.byref(h))
+        L.check(rc, "plan_create")
+        self.h = h
+
+    def execute(self, fetch=True):
+        out = np.zeros(self.nq) if fetch else None
+        L.check(self.lib.thistle_plan_execute(self.h, L._ptr(out)), "plan_execute")
+        return out
+
+    def result(self):
+        out = np.zeros(self.nq)
+        L.check(self.lib.thistle_plan_result(self.h, L._ptr(out)), "plan_result")
+        return out
+
+    def timings(self):
+        ms = np.zeros(5)
+        nq = C.c_int64()
+        units = C.c_int64()
+        L.check(self.lib.thistle_plan_timings(self.h, L._ptr(ms), C.byref(nq), C.byref(units)),
+                "plan_timings")
+        return {"pair_tables": ms[0], "enumeration": ms[1], "transform": ms[2],
+                "quartets": ms[3], "reduction": ms[4], "nquart": nq.value, "units": units.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.thistle_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
