@@ -176,6 +176,11 @@ int32_t thistle_plan_timings(ThistlePlan *plan, double *ms5, int64_t *nquart,
                              int64_t *units);
 void thistle_plan_destroy(ThistlePlan *plan);
 
+/* Number of HIP devices visible to the library (negative on error), and a
+ * device-wide synchronize of the given ordinal. */
+int32_t thistle_device_count(void);
+int32_t thistle_device_synchronize(int32_t device);
+
 /* Library version string. */
 const char *thistle_version(void);
 

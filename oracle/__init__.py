@@ -51,6 +51,10 @@ def _declare(L):
     L.orc_total_j0.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                _i, _p, _p, _i, _p, _i, _d, _d, _d, _i, _i, _p]
     L.orc_total_j0.restype = _i
+    L.orc_total_j0_sample.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
+                                      _p, _i, _p, _p, _i, _p, _i, _d, _d, _d, _i,
+                                      C.c_int64, _p, _p, _p]
+    L.orc_total_j0_sample.restype = _i
     L.orc_elastic_j0.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
                                  _p, _p, _i, _d, _d, _d, _i, _p]
     L.orc_elastic_j0.restype = _i
@@ -182,6 +186,33 @@ def total_j0(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q,
     if rc:
         raise RuntimeError(f"oracle total_j0 status {rc}")
     return out
+
+
+def total_j0_sample(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q, stride,
+                    cutoffz=1e-15, cutoffmd=1e-20, cutoffcentre=0.001, nthreads=0,
+                    maxl=None):
+    """Time the per-quartet kernels on an LL-stratified 1/stride sample.
+
+    Returns (partial S(q), kernel seconds, quartets sampled)."""
+    l, m, n, group = _I(l), _I(m), _I(n), _I(group)
+    ga, xx, yy, zz, q = _D(ga), _D(xx), _D(yy), _D(zz), _D(q)
+    mmod = _D(mmod, "F")
+    mat = _I(mat, "F")
+    total = _D(total)
+    ngto = len(l)
+    ng = int(group.max())
+    maxl = int(l.max()) if maxl is None else maxl
+    out = np.zeros(len(q))
+    secs = np.zeros(1)
+    ns = np.zeros(1, dtype=np.int64)
+    rc = lib().orc_total_j0_sample(ngto, ng, maxl, _ptr(l), _ptr(m), _ptr(n), _ptr(group),
+                                   _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(mmod),
+                                   mmod.shape[0], _ptr(mat), _ptr(total), len(total),
+                                   _ptr(q), len(q), cutoffz, cutoffmd, cutoffcentre,
+                                   nthreads, int(stride), _ptr(out), _ptr(secs), _ptr(ns))
+    if rc:
+        raise RuntimeError(f"oracle total_j0_sample status {rc}")
+    return out, float(secs[0]), int(ns[0])
 
 
 def elastic_j0(l, m, n, group, ga, xx, yy, zz, Z, q, cutoffz=1e-15,

@@ -953,13 +953,14 @@ static void build_z12(const double *mmod, int nmo, int ngto, const int *mat,
 #undef MM
 }
 
-int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
-                 const int *n, const int *group, const double *ga,
-                 const double *xx, const double *yy, const double *zz,
-                 const double *mmod, int nmo, const int *mat,
-                 const double *total, int nmat, const double *q, int nq,
-                 double cutoffz, double cutoffmd, double cutoffcentre,
-                 int zmode, int nthreads, double *tsi)
+static int total_j0_impl(int ngto, int ng, int maxl, const int *l, const int *m,
+                         const int *n, const int *group, const double *ga,
+                         const double *xx, const double *yy, const double *zz,
+                         const double *mmod, int nmo, const int *mat,
+                         const double *total, int nmat, const double *q, int nq,
+                         double cutoffz, double cutoffmd, double cutoffcentre,
+                         int zmode, int nthreads, double *tsi, int64_t stride,
+                         double *kernel_seconds, int64_t *nsampled)
 {
     setup_t S;
     int rc = setup_build(&S, ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz,
@@ -968,6 +969,33 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
     int64_t Q = orc_quartet_list(ng, S.px, S.py, S.pz, cutoffcentre, NULL);
     int32_t *ql = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)Q);
     orc_quartet_list(ng, S.px, S.py, S.pz, cutoffcentre, ql);
+    /* stride > 1: an LL-stratified sample (every stride-th quartet of the
+     * list ordered stably by LLmax) */
+    int64_t *sel = NULL, nsel = Q;
+    if (stride > 1) {
+        int64_t cnt[21] = {0}, off[22] = {0};
+        int *LLv = (int *)malloc(sizeof(int) * (size_t)Q);
+        for (int64_t e = 0; e < Q; ++e) {
+            const int32_t *o = ql + 6 * e;
+            int s4 = 0;
+            for (int k = 0; k < 4; ++k) {
+                int g0 = S.gs[o[k] - 1];
+                s4 += l[g0] + m[g0] + n[g0];
+            }
+            LLv[e] = s4 > 20 ? 20 : s4;
+            cnt[LLv[e]]++;
+        }
+        for (int k = 0; k < 21; ++k) off[k + 1] = off[k] + cnt[k];
+        int64_t *ord = (int64_t *)malloc(sizeof(int64_t) * (size_t)Q);
+        for (int64_t e = 0; e < Q; ++e) ord[off[LLv[e]]++] = e;
+        nsel = (Q + stride - 1) / stride;
+        sel = (int64_t *)malloc(sizeof(int64_t) * (size_t)nsel);
+        for (int64_t k = 0; k < nsel; ++k) sel[k] = ord[k * stride];
+        free(ord);
+        free(LLv);
+    }
+    if (nsampled) *nsampled = nsel;
+    double *tker = NULL;
 
     double *z1 = NULL, *z2 = NULL, *W = NULL;
     int norb = 0;
@@ -1011,6 +1039,7 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
     nth = nthreads > 0 ? nthreads : omp_get_max_threads();
 #endif
     double *part = (double *)calloc((size_t)nth * nq, sizeof(double));
+    tker = (double *)calloc((size_t)nth, sizeof(double));
     int err = 0;
 #pragma omp parallel num_threads(nth)
     {
@@ -1022,7 +1051,8 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
         double *F = (double *)malloc(sizeof(double) * nq);
         double *zb = (double *)malloc(sizeof(double) * 15 * 15 * 15 * 15);
 #pragma omp for schedule(static)
-        for (int64_t e = 0; e < Q; ++e) {
+        for (int64_t e0 = 0; e0 < nsel; ++e0) {
+            int64_t e = sel ? sel[e0] : e0;
             const int32_t *o = ql + 6 * e;
             int gi = o[0] - 1, gj = o[1] - 1, gk = o[2] - 1, gr = o[3] - 1;
             int mult = o[4], br = o[5];
@@ -1063,6 +1093,10 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
                             }
                             zb[((a * cj + b) * ck + c) * cr + d] = v1 + v2;
                         }
+            double t0 = 0.0;
+#ifdef _OPENMP
+            t0 = omp_get_wtime();
+#endif
             if (br) {
                 ijkr_pzero(&C, gi, gj, gk, gr, zb, cutoffz, cutoffmd, S.P0, S.D0,
                            nq, F);
@@ -1085,11 +1119,21 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
             double mlt = (double)mult;
             for (int iq = 0; iq < nq; ++iq)
                 acc[iq] = acc[iq] + mlt * F[iq] * eij[iq] * ekr[iq];
+#ifdef _OPENMP
+            tker[tid] += omp_get_wtime() - t0;
+#endif
         }
         free(F);
         free(zb);
     }
 #undef MM
+    if (kernel_seconds) {
+        double mx = 0.0;
+        for (int t = 0; t < nth; ++t) mx = tker[t] > mx ? tker[t] : mx;
+        *kernel_seconds = mx;
+    }
+    free(tker);
+    free(sel);
     for (int iq = 0; iq < nq; ++iq) {
         double s = 0.0;
         for (int t = 0; t < nth; ++t) s += part[(size_t)t * nq + iq];
@@ -1102,6 +1146,35 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
     free(W);
     setup_free(&S);
     return err;
+}
+
+int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
+                 const int *n, const int *group, const double *ga,
+                 const double *xx, const double *yy, const double *zz,
+                 const double *mmod, int nmo, const int *mat,
+                 const double *total, int nmat, const double *q, int nq,
+                 double cutoffz, double cutoffmd, double cutoffcentre,
+                 int zmode, int nthreads, double *tsi)
+{
+    return total_j0_impl(ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, mmod,
+                         nmo, mat, total, nmat, q, nq, cutoffz, cutoffmd,
+                         cutoffcentre, zmode, nthreads, tsi, 1, NULL, NULL);
+}
+
+int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
+                        const int *m, const int *n, const int *group,
+                        const double *ga, const double *xx, const double *yy,
+                        const double *zz, const double *mmod, int nmo,
+                        const int *mat, const double *total, int nmat,
+                        const double *q, int nq, double cutoffz,
+                        double cutoffmd, double cutoffcentre, int nthreads,
+                        int64_t stride, double *tsi, double *kernel_seconds,
+                        int64_t *nsampled)
+{
+    return total_j0_impl(ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, mmod,
+                         nmo, mat, total, nmat, q, nq, cutoffz, cutoffmd,
+                         cutoffcentre, 1, nthreads, tsi, stride,
+                         kernel_seconds, nsampled);
 }
 
 int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,

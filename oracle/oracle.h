@@ -85,6 +85,21 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
                  const double *total, int nmat, const double *q, int nq,
                  double cutoffz, double cutoffmd, double cutoffcentre,
                  int zmode, int nthreads, double *tsi);
+/* LL-stratified sample of the total-j0 quartets (every stride-th quartet
+ * of the list ordered stably by LLmax); kernel_seconds = per-thread maximum
+ * of the time spent in the per-quartet kernels (tot_integral_k_ijkr /
+ * _pzero, bessels0rr and the e12 accumulation) with the 2-RDM blocks
+ * supplied; tsi holds the sample's partial S(q).  Used for the CPU
+ * baseline. */
+int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
+                        const int *m, const int *n, const int *group,
+                        const double *ga, const double *xx, const double *yy,
+                        const double *zz, const double *mmod, int nmo,
+                        const int *mat, const double *total, int nmat,
+                        const double *q, int nq, double cutoffz,
+                        double cutoffmd, double cutoffcentre, int nthreads,
+                        int64_t stride, double *tsi, double *kernel_seconds,
+                        int64_t *nsampled);
 int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
                    const int *n, const int *group, const double *ga,
                    const double *xx, const double *yy, const double *zz,

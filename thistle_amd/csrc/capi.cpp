@@ -450,6 +450,18 @@ extern "C" {
 
 const char* thistle_version(void) { return "thistle-mi355x 0.1 (gfx950)"; }
 
+int32_t thistle_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return -THISTLE_EDEVICE;
+    return n;
+}
+
+int32_t thistle_device_synchronize(int32_t device) {
+    HCHECK(hipSetDevice(device));
+    HCHECK(hipDeviceSynchronize());
+    return THISTLE_OK;
+}
+
 int32_t thistle_plan_create(int32_t type, const ThistleBasis* basis, int64_t nmat,
                             const int32_t* mat, const double* total, int32_t norb,
                             const double* onerdm, const ThistleOpts* opts, ThistlePlan** plan) {

@@ -26,7 +26,8 @@ EXPORTS = [
     "thistle_onerdm", "thistle_quartet_list", "thistle_quartet_list_screened",
     "thistle_unique_total", "thistle_pair_tables", "thistle_plan_create",
     "thistle_plan_execute", "thistle_plan_result", "thistle_plan_timings",
-    "thistle_plan_destroy", "thistle_version",
+    "thistle_plan_destroy", "thistle_device_count", "thistle_device_synchronize",
+    "thistle_version",
 ]
 
 
@@ -100,6 +101,10 @@ def _declare(L):
     L.thistle_plan_timings.restype = i32
     L.thistle_plan_destroy.argtypes = [p]
     L.thistle_plan_destroy.restype = None
+    L.thistle_device_count.argtypes = []
+    L.thistle_device_count.restype = i32
+    L.thistle_device_synchronize.argtypes = [i32]
+    L.thistle_device_synchronize.restype = i32
     L.thistle_version.argtypes = []
     L.thistle_version.restype = C.c_char_p
 
@@ -154,6 +159,14 @@ def make_opts(precision=FP64, rdm_cutoff=1e-30, stream=None, device=0, waves=0):
     o.device = int(device)
     o.waves = int(waves)
     return o
+
+
+def device_count():
+    return int(load().thistle_device_count())
+
+
+def synchronize(device=0):
+    check(load().thistle_device_synchronize(int(device)), "device_synchronize")
 
 
 def check(code, where):
