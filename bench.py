@@ -162,6 +162,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-stride", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--per-q-out", default=None,
+                    help="write q, S_fp64, S_mixed and the per-q relative error here")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
@@ -186,9 +188,13 @@ def main():
         resm, _, wallm, km = run_gpu(inp, q, mat, tot, "mixed", args.steps, args.warmup)
         msm = 1000.0 * wallm / args.steps
         out["mixed"] = (resm, msm, km)
-        err = float(np.max(np.abs(resm - res64) / np.abs(res64)))
+        per_q = np.abs(resm - res64) / np.abs(res64)
+        err = float(np.max(per_q))
         if err <= 1e-6:
             chosen = "mixed"
+        if args.per_q_out:
+            np.savetxt(args.per_q_out, np.column_stack([q, res64, resm, per_q]),
+                       header="q S_fp64 S_mixed rel_err")
     res, ms, kern = out[chosen]
     units = Q * q.size
     value = units / (ms / 1000.0)

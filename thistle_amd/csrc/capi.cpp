@@ -86,6 +86,8 @@ struct ThistlePlan {
     DevBuf V, Gs, T, M, Z, mopairs, xa, xb, onerdm;
     DevBuf part, partc, S, cls;
     int64_t cls_off[18] = {0};  // per-LL ranges in `cls` (positions, ref order)
+    int cls_tmax[17] = {0};     // largest exponent-tuple count per LL class
+    DevBuf hs;                  // h(beta) per quartet
     hipEvent_t ev[6] = {};
     double ms[5] = {0, 0, 0, 0, 0};
     ~ThistlePlan() {
@@ -225,10 +227,26 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
                 case 4: gi = R.i; gj = R.i; gk = x; gr = x; break;
                 default: gi = gj = gk = gr = x; break;
             }
-            return groups[gi].L + groups[gj].L + groups[gk].L + groups[gr].L;
+            int Ls[4] = {groups[gi].L, groups[gj].L, groups[gk].L, groups[gr].L};
+            int T = (Ls[0] + 1) * (Ls[1] + 1) * (Ls[2] + 1) * (Ls[3] + 1);
+            int LL = Ls[0] + Ls[1] + Ls[2] + Ls[3];
+            pl->cls_tmax[LL] = std::max(pl->cls_tmax[LL], T);
+            return LL;
         };
         for (const auto& R : rows)
             for (int u = 0; u < R.count; ++u) cnt[LLof(R, R.r0 + u)]++;
+        // the contraction kernel keeps per-exponent-tuple tables in LDS
+        hipDeviceProp_t prop;
+        HCHECK(hipGetDeviceProperties(&prop, o->device));
+        size_t need = 0;
+        for (int LL = 0; LL <= 16; ++LL)
+            if (cnt[LL] > 0) need = std::max(need, contract_lds_bytes(LL, pl->cls_tmax[LL]));
+        if (need > prop.sharedMemPerBlock) {
+            std::fprintf(stderr, "thistle: quartet tables need %zu B of LDS (device: %zu B)\n",
+                         need, (size_t)prop.sharedMemPerBlock);
+            return THISTLE_ELMAX;
+        }
+        if (need > 65536) HCHECK(set_contract_lds_limit(need));
         pl->cls_off[0] = 0;
         for (int k = 0; k < 17; ++k) pl->cls_off[k + 1] = pl->cls_off[k] + cnt[k];
         std::vector<int64_t> fill(pl->cls_off, pl->cls_off + 17);
@@ -252,6 +270,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if ((rc = dalloc(pl->xa, sizeof(int32_t) * pl->npp))) return rc;
     if ((rc = dalloc(pl->xb, sizeof(int32_t) * pl->npp))) return rc;
     if ((rc = dalloc(pl->cls, sizeof(int32_t) * Q))) return rc;
+    if ((rc = dalloc(pl->hs, sizeof(double) * 17 * (size_t)Q))) return rc;
     HCHECK(hipMemcpyAsync(pl->cls.p, cls.data(), sizeof(int32_t) * Q, hipMemcpyHostToDevice,
                           pl->stream));
     HCHECK(hipStreamSynchronize(pl->stream));
@@ -267,7 +286,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if (waves <= 0) {
         hipDeviceProp_t prop;
         HCHECK(hipGetDeviceProperties(&prop, o->device));
-        waves = prop.multiProcessorCount * 8;
+        waves = prop.multiProcessorCount * 4;
     }
     pl->waves = (int)std::max<int64_t>(1, std::min<int64_t>(waves, Q));
     int nqc = std::min(b->nq, 512);
@@ -320,11 +339,11 @@ int run_transform_total(ThistlePlan* pl) {
     BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_none, rocblas_operation_none, npmo,
                          (rocblas_int)pl->npp, npmo, &one, pl->Gs.as<double>(), npmo,
                          pl->T.as<double>(), npmo, &zero, pl->M.as<double>(), npmo));
-    // Z = T^T * M  (npp x npp)
-    BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_transpose, rocblas_operation_none,
-                         (rocblas_int)pl->npp, (rocblas_int)pl->npp, npmo, &one,
-                         pl->T.as<double>(), npmo, pl->M.as<double>(), npmo, &zero,
-                         pl->Z.as<double>(), (rocblas_int)pl->npp));
+    // Z = T^T * M  (npp x npp, symmetric): upper triangle only
+    BCHECK(rocblas_dsyrkx(pl->blas, rocblas_fill_upper, rocblas_operation_transpose,
+                          (rocblas_int)pl->npp, npmo, &one, pl->T.as<double>(), npmo,
+                          pl->M.as<double>(), npmo, &zero, pl->Z.as<double>(),
+                          (rocblas_int)pl->npp));
     return THISTLE_OK;
 }
 
@@ -351,12 +370,21 @@ int run_series(ThistlePlan* pl, double* result_host) {
     a.part = pl->part.as<double>();
     a.partc = pl->partc.as<double>();
     a.flags = pl->flags.as<uint8_t>();
+    a.hs = pl->hs.as<double>();
+    a.nqc = std::min(512, pl->nq);
+    // h(beta) of every quartet (q independent), heaviest classes first
+    for (int LL = 16; LL >= 0; --LL) {
+        int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
+        if (n <= 0) continue;
+        int blocks = (int)std::min<int64_t>(LL < 10 ? 16384 : 2048, n);
+        HCHECK(launch_contract(a, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
+                               n, blocks, pl->stream));
+    }
     for (int q0 = 0; q0 < pl->nq; q0 += 512) {
         a.q0 = q0;
         a.nqc = std::min(512, pl->nq - q0);
         HCHECK(hipMemsetAsync(pl->part.p, 0, pl->part.bytes, pl->stream));
         HCHECK(hipMemsetAsync(pl->partc.p, 0, pl->partc.bytes, pl->stream));
-        // heaviest classes first
         for (int LL = 16; LL >= 0; --LL) {
             int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
             if (n <= 0) continue;

@@ -419,291 +419,6 @@ __global__ void k_to_float(const double* __restrict__ in, float* __restrict__ ou
     if (i < n) out[i] = (float)in[i];
 }
 
-// ----------------------------------------------------------------- series
-#define QPL 8                 // q points per lane per launch (nq chunk <= 512)
-#define XDPH 74.0             // q*h from which dphrec's start-100 recurrence
-                              // is followed step by step (it departs from
-                              // the exact j_beta above x ~ 76)
-#define SCR_STRIDE 24         // per-lane LDS scratch length (T <= 4*4*... )
-
-// rrdj0 (src/Integrals.f90:7062-7091) into a (LL+1)^2 row-major table.
-template <int LL>
-__device__ void build_rrdj0(double x, double* a) {
-#pragma clang fp contract(off)
-    const int D = LL + 1;
-    for (int k = 0; k < D * D; ++k) a[k] = 0.0;
-    a[0] = 1.0;
-    if (LL >= 1) a[1 * D + 1] = -x;
-    for (int l = 2; l <= LL; ++l) a[l * D + 0] = (double)(-(l - 1)) * a[(l - 2) * D + 0];
-    for (int l = 2; l <= LL; ++l)
-        for (int p = 1; p <= l; ++p)
-            a[l * D + p] = (double)(-(l - 1)) * a[(l - 2) * D + p] - x * a[(l - 1) * D + p - 1];
-}
-
-// Per-axis polynomial of one GTO quadruple:
-//   c(T)  = sum_t (-1)^t Ebra(t) Eket(T-t)
-//   px(m) = sum_{T=m}^{min(2m,Lx)} c(T) a(T, 2m-T)
-// so that sum_T c(T) h_pre2(beta,T,..) factorises over the axes with
-// beta = mx + my + mz (BesselDeriv's hOrder = (T+i)/2 for even T-i).
-template <int LL>
-__device__ __forceinline__ void axis_poly(const double* __restrict__ Eb, int lab,
-                                          const double* __restrict__ Ek, int lcd,
-                                          const double* __restrict__ At,
-                                          double* __restrict__ scr, int lane,
-                                          double (&px)[LL + 1]) {
-    const int D = LL + 1;
-    int Lx = lab + lcd;
-    for (int T = 0; T <= Lx; ++T) scr[T * WAVE + lane] = 0.0;
-    for (int t = 0; t <= lab; ++t) {
-        double sb = (t & 1) ? -Eb[t] : Eb[t];
-        for (int t2 = 0; t2 <= lcd; ++t2) scr[(t + t2) * WAVE + lane] += sb * Ek[t2];
-    }
-#pragma unroll
-    for (int m = 0; m <= LL; ++m) {
-        double s = 0.0;
-        if (m <= Lx) {
-            int tmax = 2 * m < Lx ? 2 * m : Lx;
-            for (int T = m; T <= tmax; ++T) s += scr[T * WAVE + lane] * At[T * D + (2 * m - T)];
-        }
-        px[m] = s;
-    }
-}
-
-// F(q) for the h = 0 branch: sum_beta h0(beta) q^(2 beta) / (2 beta + 1)!!
-template <int LL>
-__device__ __forceinline__ double eval_pzero(const double (&hs)[LL + 1], double q) {
-    double q2 = q * q;
-    double dfi[LL + 1];
-    dfi[0] = 1.0;
-#pragma unroll
-    for (int b = 1; b <= LL; ++b) dfi[b] = dfi[b - 1] / (double)(2 * b + 1);
-    double F = hs[LL] * dfi[LL];
-#pragma unroll
-    for (int b = LL - 1; b >= 0; --b) F = F * q2 + hs[b] * dfi[b];
-    return F;
-}
-
-// dphrec with start order 100 (src/Integrals.f90:7147-7184) + bessels0rr
-template <int LL>
-__device__ double eval_dphrec(const double (&hs)[LL + 1], double q, double h,
-                              double x, double sx, double cx) {
-#pragma clang fp contract(off)
-    const double XX = x * x;
-    // 0-based R[i] = RBES(i+1); R[99] = R[98] = 1
-    double r2 = 1.0, r1 = 1.0;  // R[i+2], R[i+1]
-    int i = 97;
-    for (; i > LL + 1; --i) {
-        double LU = (double)(i + 2);
-        double r0 = r1 - XX / (4.0 * LU * LU - 1.0) * r2;
-        r2 = r1;
-        r1 = r0;
-    }
-    double R[LL + 2];
-#pragma unroll
-    for (int k = LL + 1; k >= 0; --k) {
-        double LU = (double)(k + 2);
-        double r0 = r1 - XX / (4.0 * LU * LU - 1.0) * r2;
-        r2 = r1;
-        r1 = r0;
-        R[k] = r0;
-    }
-    double A = R[0] * (cx + x * sx) - XX / 3.0 * R[1] * cx;
-    double CU = 1.0 / x;
-    double F = sx / x * hs[0];
-    double invp = q / h;
-    double rb = 1.0;
-#pragma unroll
-    for (int K = 1; K <= LL + 1; ++K) {
-        double W = 2.0 * (K - 1);
-        CU = x / (W + 1.0) * CU;
-        if (K >= 2) {
-            double jb = CU * (R[K - 1] / A);
-            rb = rb * invp;
-            F = F + jb * rb * hs[K - 1];
-        }
-    }
-    return F;
-}
-
-// F(q) = sum_beta h(beta) (q/h)^beta j_beta(q h)
-template <int LL>
-__device__ __forceinline__ double eval_series(const double (&hs)[LL + 1], double q,
-                                              double h) {
-    double x = q * h;
-    double sx, cx;
-    sincos(x, &sx, &cx);
-    double j0 = sx / x;
-    if (x >= XDPH) return eval_dphrec<LL>(hs, q, h, x, sx, cx);
-    if (x > (double)LL + 6.0) {
-        // upward recurrence: stable while beta < x
-        double inv = 1.0 / x, r = q / h;
-        double F = j0 * hs[0];
-        if (LL >= 1) {
-            double jm = j0, jc = (j0 - cx) * inv, rb = r;
-            F += jc * rb * hs[1];
-#pragma unroll
-            for (int b = 1; b < LL; ++b) {
-                double jn = (double)(2 * b + 1) * inv * jc - jm;
-                jm = jc;
-                jc = jn;
-                rb *= r;
-                F += jc * rb * hs[b + 1];
-            }
-        }
-        return F;
-    }
-    // Miller backward recurrence for g_b = j_b / x^b,
-    // g_{b-1} = (2b+1) g_b - x^2 g_{b+1}, normalised by g_0 = sin(x)/x
-    double x2 = x * x, q2 = q * q;
-    int N = LL + 16 + (int)x;
-    double vp1 = 0.0, v = 1.0;
-    for (int b = N; b > LL; --b) {
-        double vm = (double)(2 * b + 1) * v - x2 * vp1;
-        vp1 = v;
-        v = vm;
-    }
-    double acc = hs[LL] * v;
-#pragma unroll
-    for (int b = LL; b >= 1; --b) {
-        double vm = (double)(2 * b + 1) * v - x2 * vp1;
-        vp1 = v;
-        v = vm;
-        acc = acc * q2 + hs[b - 1] * v;
-    }
-    return acc * (j0 / v);
-}
-
-template <int LL>
-__device__ void quartet_work(const SeriesArgs& A, int64_t e, double* __restrict__ scr,
-                             double* __restrict__ At, double (&Sacc)[QPL],
-                             double (&Scmp)[QPL], int lane) {
-    const uint64_t ent = A.qlist[e];
-    const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
-    const int mult = QE_MULT(ent), br = QE_BR(ent);
-    const int pb = pair_index(gi, gj, A.ng), pk = pair_index(gk, gr, A.ng);
-    const PairInfo PB = A.P[pb], PK = A.P[pk];
-    const double hx = PK.Px - PB.Px, hy = PK.Py - PB.Py, hz = PK.Pz - PB.Pz;
-    const double h = sqrt(hx * hx + hy * hy + hz * hz);
-    const int D = LL + 1;
-
-    __syncthreads();
-    if (lane < 3) {
-        double xv = br ? 0.0 : (lane == 0 ? hx : (lane == 1 ? hy : hz));
-        build_rrdj0<LL>(xv, At + lane * D * D);
-    }
-    __syncthreads();
-
-    const GroupInfo Gi = A.G[gi], Gj = A.G[gj], Gk = A.G[gk], Gr = A.G[gr];
-    const int cj = Gj.count, ck = Gk.count, cr = Gr.count;
-    const int nquads = Gi.count * cj * ck * cr;
-    const int L1 = A.maxl + 1, T1 = 2 * A.maxl + 1;
-    const double* Eb0 = A.E + (int64_t)pb * 3 * L1 * L1 * T1;
-    const double* Ek0 = A.E + (int64_t)pk * 3 * L1 * L1 * T1;
-
-    double hs[LL + 1];
-#pragma unroll
-    for (int b = 0; b <= LL; ++b) hs[b] = 0.0;
-    int any = 0;
-    for (int u = lane; u < nquads; u += WAVE) {
-        int d_ = u % cr;
-        int t = u / cr;
-        int c_ = t % ck;
-        t /= ck;
-        int b_ = t % cj;
-        int a_ = t / cj;
-        const int ia = Gi.start + a_, ib = Gj.start + b_, ic = Gk.start + c_, id = Gr.start + d_;
-        double z;
-        if (!A.elastic) {
-            int64_t xb = PB.off + a_ * cj + b_;
-            int64_t xk = PK.off + c_ * cr + d_;
-            z = A.Z[xk + A.ldz * xb];
-        } else {
-            z = A.Z[ic + A.ldz * id] * A.Z[ia + A.ldz * ib];
-        }
-        if (fabs(z) < A.cutoffz) continue;
-        any = 1;
-        const char4 la = A.lxyz[ia], lb = A.lxyz[ib], lc = A.lxyz[ic], ld = A.lxyz[id];
-        double px[LL + 1], py[LL + 1], pz[LL + 1];
-        axis_poly<LL>(Eb0 + ((0 * L1 + la.x) * L1 + lb.x) * T1, la.x + lb.x,
-                      Ek0 + ((0 * L1 + lc.x) * L1 + ld.x) * T1, lc.x + ld.x, At,
-                      scr, lane, px);
-        axis_poly<LL>(Eb0 + ((1 * L1 + la.y) * L1 + lb.y) * T1, la.y + lb.y,
-                      Ek0 + ((1 * L1 + lc.y) * L1 + ld.y) * T1, lc.y + ld.y,
-                      At + D * D, scr, lane, py);
-        axis_poly<LL>(Eb0 + ((2 * L1 + la.z) * L1 + lb.z) * T1, la.z + lb.z,
-                      Ek0 + ((2 * L1 + lc.z) * L1 + ld.z) * T1, lc.z + ld.z,
-                      At + 2 * D * D, scr, lane, pz);
-        double pyz[LL + 1];
-#pragma unroll
-        for (int k = 0; k <= LL; ++k) {
-            double s = 0.0;
-#pragma unroll
-            for (int my = 0; my <= k; ++my) s += py[my] * pz[k - my];
-            pyz[k] = s;
-        }
-#pragma unroll
-        for (int b = 0; b <= LL; ++b) {
-            double s = 0.0;
-#pragma unroll
-            for (int mx = 0; mx <= b; ++mx) s += px[mx] * pyz[b - mx];
-            hs[b] += z * s;
-        }
-    }
-#pragma unroll
-    for (int b = 0; b <= LL; ++b) hs[b] = wave_sum(hs[b]);
-    int anyw = __any(any);
-    if (lane == 0) A.flags[e] = anyw ? 1 : 0;
-
-    const double dm = (double)mult;
-#pragma unroll
-    for (int s = 0; s < QPL; ++s) {
-        int iql = lane + WAVE * s;
-        if (iql < A.nqc) {
-            int iq = A.q0 + iql;
-            double q = A.q[iq];
-            double F = br ? eval_pzero<LL>(hs, q) : eval_series<LL>(hs, q, h);
-            double v = dm * F;
-            v = v * A.e12[(int64_t)pb * A.nq + iq];
-            v = v * A.e12[(int64_t)pk * A.nq + iq];
-            kahan_add(Sacc[s], Scmp[s], v);
-        }
-    }
-}
-
-// One launch per LL class: waves stride over the class's quartets (a list
-// of positions in reference order) and add their sums into the per-wave
-// partial rows, so the summation order is fixed for a given wave count.
-template <int LL>
-__global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __restrict__ idx,
-                                                 int64_t nidx, int first) {
-    __shared__ double scr[SCR_STRIDE * WAVE];
-    __shared__ double At[3 * (LL + 1) * (LL + 1)];
-    const int lane = threadIdx.x;
-    double Sacc[QPL], Scmp[QPL];
-#pragma unroll
-    for (int s = 0; s < QPL; ++s) Sacc[s] = Scmp[s] = 0.0;
-    const int64_t W = gridDim.x;
-    for (int64_t k = blockIdx.x; k < nidx; k += W)
-        quartet_work<LL>(A, idx[k], scr, At, Sacc, Scmp, lane);
-#pragma unroll
-    for (int s = 0; s < QPL; ++s) {
-        int iql = lane + WAVE * s;
-        if (iql < A.nqc) {
-            int64_t o = (int64_t)blockIdx.x * A.nqc + iql;
-            if (first) {
-                A.part[o] = Sacc[s];
-                A.partc[o] = Scmp[s];
-            } else {
-                double ps = A.part[o], pc = A.partc[o];
-                kahan_add(ps, pc, Sacc[s]);
-                kahan_add(ps, pc, -Scmp[s]);
-                A.part[o] = ps;
-                A.partc[o] = pc;
-            }
-        }
-    }
-}
-
 __global__ void k_reduce(const double* __restrict__ part,
                          const double* __restrict__ partc, int waves, int nqc,
                          double* __restrict__ out) {
@@ -749,22 +464,6 @@ hipError_t launch_enumerate(const EnumRow* rows, int nrows, const PairInfo* P, i
 hipError_t launch_unpack_quartets(const uint64_t* in, int64_t n, int32_t* out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_unpack_quartets, dim3(nblk(n, 256)), dim3(256), 0, s, in, n, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_series(const SeriesArgs& a, int LL, const int32_t* idx, int64_t nidx,
-                         int waves, int first, hipStream_t s) {
-    if (nidx <= 0) return hipSuccess;
-    switch (LL) {
-#define CASE(k)                                                                     \
-    case k:                                                                         \
-        hipLaunchKernelGGL(k_series<k>, dim3(waves), dim3(WAVE), 0, s, a, idx, nidx, first); \
-        break;
-        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
-#undef CASE
-        default: return hipErrorInvalidValue;
-    }
     return hipGetLastError();
 }
 

@@ -1,0 +1,586 @@
+// Quartet kernels of the total / elastic j0 path (gfx950).
+//
+// Two kernels per LLmax class (LL = L_i + L_j + L_k + L_r of the quartet's
+// four primitive shells), launched over the class's quartet positions:
+//
+//   k_contract<LL>  one 256-thread workgroup per quartet.  Contracts the
+//                   quartet's 2-RDM block with the Hermite expansions and the
+//                   j0-derivative coefficients into h(beta), beta = 0..LL
+//                   (h_saved of tot_integral_k_ijkr / integral_k_ijkr and the
+//                   P0 sum of the h = 0 branch; src/Integrals.f90:4751-4863,
+//                   4955-5226,5730-5901).  FP64.
+//   k_series<LL,M>  one wave per (quartet slice, 64-point q slice).  Evaluates
+//                   F(q) = sum_beta h(beta) (q/h)^beta j_beta(q h)
+//                   (bessels0rr / dphrec, src/Integrals.f90:6662-6739,
+//                   7147-7184) or the h = 0 polynomial, weights it with
+//                   mult * e12(q,i,j) * e12(q,k,r) and accumulates S(q) with
+//                   compensated sums.  M = true runs the large-argument
+//                   Bessel recursion in FP32 (mixed precision).
+//
+// Factorisation used by k_contract.  The reference forms, per GTO quadruple,
+//   sum_{t,u,v,t',u',v'} (-1)^{t+u+v} E^x_ab(t) E^y_ab(u) E^z_ab(v)
+//                         E^x_cd(t') E^y_cd(u') E^z_cd(v') h_pre2(beta,T,U,V)
+// with T = t+t' etc. and h_pre2(beta,T,U,V) = sum a(T,i) b(U,j) c(V,k) over
+// beta = ceil((T+U+V-i-j-k)/2)+i+j+k (BesselDeriv).  a(T,i) vanishes unless
+// T-i is even, so beta = (T+i)/2 + (U+j)/2 + (V+k)/2 and the sum factorises:
+//   h(beta) = sum_{mx+my+mz=beta} px(mx) py(my) pz(mz),
+//   px(m)   = sum_T c_x(T) a(T, 2m-T),  c_x(T) = sum_t (-1)^t E^x_ab(t) E^x_cd(T-t).
+// px depends only on the x exponents (la,lb,lc,ld) of the quadruple, so each
+// workgroup tabulates px/py/pz per exponent tuple in LDS and every quadruple
+// is a gather plus a triple convolution.  a(l,p) has the closed form
+//   a(l,p) = (-1)^((l+p)/2) l! / (k! p! 2^k) x^p,  k = (l-p)/2,
+// which is what rrdj0's recurrence (src/Integrals.f90:7062-7091) produces.
+#include "thistle_internal.hpp"
+
+#include <math.h>
+
+#define WAVE 64
+
+#define HS 17       // stride of the h(beta) buffer (LL <= 16)
+#define XDPH 74.0   // q h from which dphrec's start-100 recurrence is followed
+                    // step by step (it departs from the exact j_beta near 76)
+
+namespace {
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ void kahan(double& s, double& c, double v) {
+#pragma clang fp contract(off)
+    double y = v - c;
+    double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__device__ __forceinline__ double rcp64(double x) {
+    // hardware reciprocal estimate refined by two Newton steps
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+// Cody-Waite reduction by pi/2 split in three parts (33, 33 and 53 bits),
+// then Taylor polynomials on |r| <= pi/4.  Valid for 0 <= x < 2^20.
+__device__ __forceinline__ void sincos_cw(double x, double& s, double& c) {
+    const double C1 = 0x1.921fb54400000p+0, C2 = 0x1.0b4611a600000p-34,
+                 C3 = 0x1.3198a2e037073p-69;
+    double kf = rint(x * 0.6366197723675814);
+    double r = fma(-kf, C1, x);
+    r = fma(-kf, C2, r);
+    r = fma(-kf, C3, r);
+    double r2 = r * r;
+    // sin r = r (1 - r^2/3! + ... - r^16/17!)
+    double ps = -1.0 / 355687428096000.0;   // -1/17!
+    ps = fma(ps, r2, 1.0 / 1307674368000.0);
+    ps = fma(ps, r2, -1.0 / 6227020800.0);
+    ps = fma(ps, r2, 1.0 / 39916800.0);
+    ps = fma(ps, r2, -1.0 / 362880.0);
+    ps = fma(ps, r2, 1.0 / 5040.0);
+    ps = fma(ps, r2, -1.0 / 120.0);
+    ps = fma(ps, r2, 1.0 / 6.0);
+    double sr = fma(-r * r2, ps, r);
+    // cos r = 1 - r^2/2! + ... + r^20/20!
+    double pc = 1.0 / 2432902008176640000.0;  // 1/20!
+    pc = fma(pc, r2, -1.0 / 6402373705728000.0);
+    pc = fma(pc, r2, 1.0 / 20922789888000.0);
+    pc = fma(pc, r2, -1.0 / 87178291200.0);
+    pc = fma(pc, r2, 1.0 / 479001600.0);
+    pc = fma(pc, r2, -1.0 / 3628800.0);
+    pc = fma(pc, r2, 1.0 / 40320.0);
+    pc = fma(pc, r2, -1.0 / 720.0);
+    pc = fma(pc, r2, 1.0 / 24.0);
+    pc = fma(pc, r2, -0.5);
+    double cr = fma(pc, r2, 1.0);
+    int k = (int)kf & 3;
+    double ss = (k & 1) ? cr : sr;
+    double cc = (k & 1) ? sr : cr;
+    s = (k & 2) ? -ss : ss;
+    c = ((k + 1) & 2) ? -cc : cc;
+}
+
+// FP32 polynomial after an FP64 reduction (mixed precision).
+__device__ __forceinline__ void sincos_mixed(double x, float& s, float& c) {
+    const double C1 = 0x1.921fb54400000p+0, C2 = 0x1.0b4611a600000p-34,
+                 C3 = 0x1.3198a2e037073p-69;
+    double kf = rint(x * 0.6366197723675814);
+    double rd = fma(-kf, C1, x);
+    rd = fma(-kf, C2, rd);
+    rd = fma(-kf, C3, rd);
+    float r = (float)rd, r2 = r * r;
+    float ps = 1.0f / 362880.0f;
+    ps = fmaf(ps, r2, -1.0f / 5040.0f);
+    ps = fmaf(ps, r2, 1.0f / 120.0f);
+    ps = fmaf(ps, r2, -1.0f / 6.0f);
+    float sr = fmaf(r * r2, ps, r);
+    float pc = -1.0f / 3628800.0f;
+    pc = fmaf(pc, r2, 1.0f / 40320.0f);
+    pc = fmaf(pc, r2, -1.0f / 720.0f);
+    pc = fmaf(pc, r2, 1.0f / 24.0f);
+    pc = fmaf(pc, r2, -0.5f);
+    float cr = fmaf(pc, r2, 1.0f);
+    int k = (int)kf & 3;
+    float ss = (k & 1) ? cr : sr;
+    float cc = (k & 1) ? sr : cr;
+    s = (k & 2) ? -ss : ss;
+    c = ((k + 1) & 2) ? -cc : cc;
+}
+
+// a(l,p) of rrdj0 in closed form: coefficient table built at compile time
+struct ACoef {
+    double v[17][17];
+};
+constexpr ACoef make_acoef() {
+    ACoef t{};
+    for (int l = 0; l <= 16; ++l)
+        for (int p = 0; p <= l; ++p) {
+            if ((l - p) & 1) continue;
+            int k = (l - p) / 2;
+            double c = 1.0;
+            for (int i = 2; i <= l; ++i) c *= i;
+            for (int i = 2; i <= k; ++i) c /= i;
+            for (int i = 2; i <= p; ++i) c /= i;
+            for (int i = 0; i < k; ++i) c *= 0.5;
+            t.v[l][p] = (((l + p) / 2) & 1) ? -c : c;
+        }
+    return t;
+}
+__constant__ ACoef kAcoef = make_acoef();
+
+__device__ __forceinline__ double a_closed(int l, int p, double x) {
+    double xp = 1.0;
+    for (int i = 0; i < p; ++i) xp *= x;
+    return kAcoef.v[l][p] * xp;
+}
+
+// floor(u / d) for u < 2^16 and 1 <= d <= 15 with m = ceil(2^32 / d); d = 1
+// has no 32-bit multiplier and is passed through.
+__device__ __forceinline__ uint32_t magic_div(uint32_t d) {
+    return d == 1 ? 0u : (uint32_t)((0xffffffffull + d) / d);
+}
+__device__ __forceinline__ uint32_t div_small(uint32_t u, uint32_t d, uint32_t m) {
+    return d == 1 ? u : __umulhi(u, m);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ contraction
+template <int LL, int CT>
+__global__ void __launch_bounds__(CT) k_contract(SeriesArgs A, const int32_t* __restrict__ idx,
+                                                 int64_t nidx) {
+    constexpr int D = LL + 1;
+    constexpr int NW = CT / WAVE;
+    extern __shared__ double sm[];
+    double* At = sm;              // [3][D][D]
+    double* red = At + 3 * D * D; // [4][D]
+    double* PX = red + 4 * D;     // [3][T][D]
+    __shared__ int anyw[NW];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int L1 = A.maxl + 1, T1 = 2 * A.maxl + 1;
+
+    for (int64_t kq = blockIdx.x; kq < nidx; kq += gridDim.x) {
+        const int64_t e = idx[kq];
+        const uint64_t ent = A.qlist[e];
+        const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
+        const int br = QE_BR(ent);
+        const int pb = pair_index(gi, gj, A.ng), pk = pair_index(gk, gr, A.ng);
+        const PairInfo PB = A.P[pb], PK = A.P[pk];
+        const GroupInfo Gi = A.G[gi], Gj = A.G[gj], Gk = A.G[gk], Gr = A.G[gr];
+        const int Li = Gi.L, Lj = Gj.L, Lk = Gk.L, Lr = Gr.L;
+        const int T = (Li + 1) * (Lj + 1) * (Lk + 1) * (Lr + 1);
+        double hxyz[3] = {PK.Px - PB.Px, PK.Py - PB.Py, PK.Pz - PB.Pz};
+
+        __syncthreads();  // the previous quartet is done with the tables
+        for (int w = tid; w < 3 * D * D; w += CT) {
+            int ax = w / (D * D), l = (w / D) % D, p = w % D;
+            At[w] = a_closed(l, p, br ? 0.0 : hxyz[ax]);
+        }
+        __syncthreads();
+        const double* E0 = A.E;
+        for (int w = tid; w < 3 * T; w += CT) {
+            int ax = w / T, t = w % T;
+            int ld = t % (Lr + 1), t2 = t / (Lr + 1);
+            int lc = t2 % (Lk + 1);
+            t2 /= (Lk + 1);
+            int lb = t2 % (Lj + 1), la = t2 / (Lj + 1);
+            const double* Eb = E0 + (((int64_t)pb * 3 + ax) * L1 * L1 + la * L1 + lb) * T1;
+            const double* Ek = E0 + (((int64_t)pk * 3 + ax) * L1 * L1 + lc * L1 + ld) * T1;
+            const int lab = la + lb, lcd = lc + ld;
+            double eb[D], ek[D];
+#pragma unroll
+            for (int s = 0; s < D; ++s) {
+                double vb = s <= lab ? Eb[s] : 0.0;
+                eb[s] = (s & 1) ? -vb : vb;
+                ek[s] = s <= lcd ? Ek[s] : 0.0;
+            }
+            double cx[D];
+#pragma unroll
+            for (int T2 = 0; T2 < D; ++T2) {
+                double s = 0.0;
+#pragma unroll
+                for (int t3 = 0; t3 <= T2; ++t3) s += eb[t3] * ek[T2 - t3];
+                cx[T2] = s;
+            }
+            const double* Aax = At + ax * D * D;
+            double* out = PX + ((int64_t)ax * T + t) * D;
+#pragma unroll
+            for (int m = 0; m < D; ++m) {
+                double s = 0.0;
+#pragma unroll
+                for (int T2 = m; T2 <= 2 * m && T2 < D; ++T2) s += cx[T2] * Aax[T2 * D + (2 * m - T2)];
+                out[m] = s;
+            }
+        }
+        __syncthreads();
+
+        const int ci = Gi.count, cj = Gj.count, ck = Gk.count, cr = Gr.count;
+        const uint32_t nq4 = (uint32_t)(ci * cj * ck * cr);
+        const uint32_t Mr = magic_div(cr), Mk = magic_div(ck), Mj = magic_div(cj);
+        const int sTz = (Lj + 1) * (Lk + 1) * (Lr + 1);  // stride of la in a tuple
+        double h[D];
+#pragma unroll
+        for (int b = 0; b < D; ++b) h[b] = 0.0;
+        int any = 0;
+        for (uint32_t u = tid; u < nq4; u += CT) {
+            uint32_t t1 = div_small(u, cr, Mr);
+            int d_ = (int)(u - t1 * cr);
+            uint32_t t2 = div_small(t1, ck, Mk);
+            int c_ = (int)(t1 - t2 * ck);
+            uint32_t a_ = div_small(t2, cj, Mj);
+            int b_ = (int)(t2 - a_ * cj);
+            const int ia = Gi.start + (int)a_, ib = Gj.start + b_, ic = Gk.start + c_,
+                      id = Gr.start + d_;
+            double z;
+            if (!A.elastic) {
+                int64_t xb = PB.off + (int)a_ * cj + b_;
+                int64_t xk = PK.off + c_ * cr + d_;
+                // Z is symmetric and only its upper triangle is stored
+                z = xk <= xb ? A.Z[xk + A.ldz * xb] : A.Z[xb + A.ldz * xk];
+            } else {
+                z = A.Z[ic + A.ldz * id] * A.Z[ia + A.ldz * ib];
+            }
+            if (fabs(z) < A.cutoffz) continue;
+            any = 1;
+            const char4 la = A.lxyz[ia], lb = A.lxyz[ib], lc = A.lxyz[ic], ld = A.lxyz[id];
+            const int tx = ((la.x * (Lj + 1) + lb.x) * (Lk + 1) + lc.x) * (Lr + 1) + ld.x;
+            const int ty = ((la.y * (Lj + 1) + lb.y) * (Lk + 1) + lc.y) * (Lr + 1) + ld.y;
+            const int tz = ((la.z * (Lj + 1) + lb.z) * (Lk + 1) + lc.z) * (Lr + 1) + ld.z;
+            (void)sTz;
+            const double* px = PX + (int64_t)tx * D;
+            const double* py = PX + ((int64_t)T + ty) * D;
+            const double* pz = PX + ((int64_t)2 * T + tz) * D;
+            double vy[D], vz[D], vx[D];
+#pragma unroll
+            for (int m = 0; m < D; ++m) {
+                vx[m] = px[m];
+                vy[m] = py[m];
+                vz[m] = pz[m];
+            }
+            double yz[D];
+#pragma unroll
+            for (int k2 = 0; k2 < D; ++k2) {
+                double s = 0.0;
+#pragma unroll
+                for (int my = 0; my <= k2; ++my) s += vy[my] * vz[k2 - my];
+                yz[k2] = s;
+            }
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                double s = 0.0;
+#pragma unroll
+                for (int mx = 0; mx <= b; ++mx) s += vx[mx] * yz[b - mx];
+                h[b] = fma(z, s, h[b]);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            double v = wsum(h[b]);
+            if (lane == 0) red[wid * D + b] = v;
+        }
+        int wa = __any(any);
+        if (lane == 0) anyw[wid] = wa;
+        __syncthreads();
+        if (tid < D) {
+            double s = red[tid];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) s += red[w * D + tid];
+            A.hs[e * HS + tid] = s;
+        }
+        if (tid == 0) {
+            int f = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) f |= anyw[w];
+            A.flags[e] = f ? 1 : 0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ series
+template <int LL>
+__device__ __forceinline__ double f_pzero(const double (&hs)[LL + 1], double q) {
+    double q2 = q * q;
+    double dfi = 1.0;
+#pragma unroll
+    for (int b = 1; b <= LL; ++b) dfi /= (double)(2 * b + 1);
+    double F = hs[LL] * dfi;
+#pragma unroll
+    for (int b = LL - 1; b >= 0; --b) {
+        dfi *= (double)(2 * b + 3);
+        F = fma(F, q2, hs[b] * dfi);
+    }
+    return F;
+}
+
+// dphrec with start order 100 followed literally, then bessels0rr's sum
+template <int LL>
+__device__ __noinline__ double f_dphrec(const double (&hs)[LL + 1], double q, double h, double x, double sx,
+                           double cx) {
+#pragma clang fp contract(off)
+    const double XX = x * x;
+    double r2 = 1.0, r1 = 1.0;  // R[i+2], R[i+1] with R[99] = R[98] = 1
+    int i = 97;
+    for (; i > LL + 1; --i) {
+        double LU = (double)(i + 2);
+        double r0 = r1 - XX / (4.0 * LU * LU - 1.0) * r2;
+        r2 = r1;
+        r1 = r0;
+    }
+    double R[LL + 2];
+#pragma unroll
+    for (int k = LL + 1; k >= 0; --k) {
+        double LU = (double)(k + 2);
+        double r0 = r1 - XX / (4.0 * LU * LU - 1.0) * r2;
+        r2 = r1;
+        r1 = r0;
+        R[k] = r0;
+    }
+    double Aw = R[0] * (cx + x * sx) - XX / 3.0 * R[1] * cx;
+    double CU = 1.0 / x;
+    double F = sx / x * hs[0];
+    double invp = q / h, rb = 1.0;
+#pragma unroll
+    for (int K = 1; K <= LL + 1; ++K) {
+        double W = 2.0 * (K - 1);
+        CU = x / (W + 1.0) * CU;
+        if (K >= 2) {
+            double jb = CU * (R[K - 1] / Aw);
+            rb = rb * invp;
+            F = F + jb * rb * hs[K - 1];
+        }
+    }
+    return F;
+}
+
+// Miller backward recurrence for g_b = j_b(x)/x^b, normalised by
+// g_0 = sin(x)/x:  F = sum_b h(b) q^(2b) g_b.  FP64 (small arguments, where
+// the terms cancel).
+template <int LL>
+__device__ __forceinline__ double f_miller(const double (&hs)[LL + 1], double q, double x,
+                                           double j0) {
+    const double x2 = x * x, q2 = q * q;
+    const int N = LL + 16 + (int)x;
+    double vp1 = 0.0, v = 1.0, tb = 2.0 * N + 1.0;
+    for (int b = N; b > LL; --b) {
+        double vm = fma(tb, v, -x2 * vp1);
+        vp1 = v;
+        v = vm;
+        tb -= 2.0;
+    }
+    double acc = hs[LL] * v;
+#pragma unroll
+    for (int b = LL; b >= 1; --b) {
+        double vm = fma((double)(2 * b + 1), v, -x2 * vp1);
+        vp1 = v;
+        v = vm;
+        acc = fma(acc, q2, hs[b - 1] * v);
+    }
+    return acc * (j0 * rcp64(v));
+}
+
+template <int LL>
+__device__ __forceinline__ double f_up64(const double (&hs)[LL + 1], double q, double invh,
+                                         double inv, double j0, double cx) {
+    double F = j0 * hs[0];
+    if (LL >= 1) {
+        const double r = q * invh;
+        double jm = j0, jc = (j0 - cx) * inv, rb = r;
+        F = fma(jc * rb, hs[1], F);
+#pragma unroll
+        for (int b = 1; b < LL; ++b) {
+            double jn = fma((double)(2 * b + 1) * inv, jc, -jm);
+            jm = jc;
+            jc = jn;
+            rb *= r;
+            F = fma(jc * rb, hs[b + 1], F);
+        }
+    }
+    return F;
+}
+
+template <int LL>
+__device__ __forceinline__ double f_up32(const float (&hf)[LL + 1], double q, double invh,
+                                         double x) {
+    float s, c;
+    sincos_mixed(x, s, c);
+    const float inv = __builtin_amdgcn_rcpf((float)x);
+    const float j0 = s * inv;
+    float F = j0 * hf[0];
+    if (LL >= 1) {
+        const float r = (float)(q * invh);
+        float jm = j0, jc = (j0 - c) * inv, rb = r;
+        F = fmaf(jc * rb, hf[1], F);
+#pragma unroll
+        for (int b = 1; b < LL; ++b) {
+            float jn = fmaf((float)(2 * b + 1) * inv, jc, -jm);
+            jm = jc;
+            jc = jn;
+            rb *= r;
+            F = fmaf(jc * rb, hf[b + 1], F);
+        }
+    }
+    return (double)F;
+}
+
+#define SQ 4   // q slices (of 64 points) per wave
+
+template <int LL, bool MIXED>
+__device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], double q, double h,
+                                         double invh, int br) {
+    if (br) return f_pzero<LL>(hs, q);
+    const double x = q * h;
+    if (MIXED && x > (double)LL + 6.0 && x < XDPH) {
+        float hf[LL + 1];
+#pragma unroll
+        for (int b = 0; b <= LL; ++b) hf[b] = (float)hs[b];
+        return f_up32<LL>(hf, q, invh, x);
+    }
+    double sx, cx;
+    sincos_cw(x, sx, cx);
+    const double inv = rcp64(x);
+    const double j0 = sx * inv;
+    if (x >= XDPH) return f_dphrec<LL>(hs, q, h, x, sx, cx);
+    if (x > (double)LL + 6.0) return f_up64<LL>(hs, q, invh, inv, j0, cx);
+    return f_miller<LL>(hs, q, x, j0);
+}
+
+template <int LL, bool MIXED>
+__global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __restrict__ idx,
+                                                 int64_t nidx, int first) {
+    const int lane = threadIdx.x;
+    int iqs[SQ];
+    double qv[SQ], S[SQ], Sc[SQ];
+#pragma unroll
+    for (int s = 0; s < SQ; ++s) {
+        const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
+        iqs[s] = A.q0 + (iql < A.nqc ? iql : 0);
+        qv[s] = A.q[iqs[s]];
+        S[s] = Sc[s] = 0.0;
+    }
+    for (int64_t kq = blockIdx.x; kq < nidx; kq += gridDim.x) {
+        const int64_t e = idx[kq];
+        const uint64_t ent = A.qlist[e];
+        const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
+        const int br = QE_BR(ent);
+        const double mult = (double)QE_MULT(ent);
+        const int pb = pair_index(gi, gj, A.ng), pk = pair_index(gk, gr, A.ng);
+        const double hx = A.P[pk].Px - A.P[pb].Px, hy = A.P[pk].Py - A.P[pb].Py,
+                     hz = A.P[pk].Pz - A.P[pb].Pz;
+        const double h = sqrt(hx * hx + hy * hy + hz * hz);
+        const double invh = br ? 0.0 : rcp64(h);
+        double hs[LL + 1];
+#pragma unroll
+        for (int b = 0; b <= LL; ++b) hs[b] = A.hs[e * HS + b];
+        const double* eb = A.e12 + (int64_t)pb * A.nq;
+        const double* ek = A.e12 + (int64_t)pk * A.nq;
+#pragma unroll
+        for (int s = 0; s < SQ; ++s) {
+            double v = mult * f_eval<LL, MIXED>(hs, qv[s], h, invh, br);
+            v = v * eb[iqs[s]];
+            v = v * ek[iqs[s]];
+            kahan(S[s], Sc[s], v);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SQ; ++s) {
+        const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
+        if (iql >= A.nqc) continue;
+        int64_t o = (int64_t)blockIdx.x * A.nqc + iql;
+        if (first) {
+            A.part[o] = S[s];
+            A.partc[o] = Sc[s];
+        } else {
+            double ps = A.part[o], pc = A.partc[o];
+            kahan(ps, pc, S[s]);
+            kahan(ps, pc, -Sc[s]);
+            A.part[o] = ps;
+            A.partc[o] = pc;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- launchers
+size_t contract_lds_bytes(int LL, int Tmax) {
+    int D = LL + 1;
+    return sizeof(double) * (size_t)(3 * D * D + 4 * D + 3 * Tmax * D);
+}
+
+hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t* idx,
+                           int64_t nidx, int blocks, hipStream_t s) {
+    if (nidx <= 0) return hipSuccess;
+    size_t lds = contract_lds_bytes(LL, Tmax);
+    // one wave per quartet up to LL = 9 (tens to hundreds of quadruples),
+    // four waves per quartet above (thousands)
+    switch (LL) {
+#define CASE(k, nt)                                                                          \
+    case k:                                                                                  \
+        hipLaunchKernelGGL((k_contract<k, nt>), dim3(blocks), dim3(nt), lds, s, a, idx, nidx); \
+        break;
+        CASE(0, 64) CASE(1, 64) CASE(2, 64) CASE(3, 64) CASE(4, 64) CASE(5, 64) CASE(6, 64)
+        CASE(7, 64) CASE(8, 64) CASE(9, 64) CASE(10, 256) CASE(11, 256) CASE(12, 256)
+        CASE(13, 256) CASE(14, 256) CASE(15, 256) CASE(16, 256)
+#undef CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int contract_threads(int LL) { return LL < 10 ? 64 : 256; }
+
+hipError_t set_contract_lds_limit(size_t bytes) {
+#define SETA(k, nt)                                                                      \
+    {                                                                                    \
+        hipError_t e = hipFuncSetAttribute((const void*)k_contract<k, nt>,               \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                           (int)bytes);                                  \
+        if (e != hipSuccess) return e;                                                   \
+    }
+    SETA(0, 64) SETA(1, 64) SETA(2, 64) SETA(3, 64) SETA(4, 64) SETA(5, 64) SETA(6, 64)
+    SETA(7, 64) SETA(8, 64) SETA(9, 64) SETA(10, 256) SETA(11, 256) SETA(12, 256)
+    SETA(13, 256) SETA(14, 256) SETA(15, 256) SETA(16, 256)
+#undef SETA
+    return hipSuccess;
+}
+
+hipError_t launch_series(const SeriesArgs& a, int LL, const int32_t* idx, int64_t nidx,
+                         int waves, int first, hipStream_t s) {
+    if (nidx <= 0) return hipSuccess;
+    dim3 grid(waves, (a.nqc + SQ * WAVE - 1) / (SQ * WAVE));
+    switch (LL) {
+#define CASE(k)                                                                            \
+    case k:                                                                                \
+        if (a.mixed)                                                                       \
+            hipLaunchKernelGGL((k_series<k, true>), grid, dim3(WAVE), 0, s, a, idx, nidx, first); \
+        else                                                                               \
+            hipLaunchKernelGGL((k_series<k, false>), grid, dim3(WAVE), 0, s, a, idx, nidx, first); \
+        break;
+        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}

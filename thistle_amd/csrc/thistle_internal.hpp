@@ -71,6 +71,7 @@ struct SeriesArgs {
     double* part;             // [waves][nqc] sums
     double* partc;            // [waves][nqc] Kahan compensations
     uint8_t* flags;           // [nquart] block passes the ztot screen
+    double* hs;               // [nquart][17] h(beta) per quartet
 };
 
 hipError_t launch_pair_info(const GroupInfo* G, PairInfo* P, int npair,
@@ -84,7 +85,12 @@ hipError_t launch_enumerate(const EnumRow* rows, int nrows, const PairInfo* P,
                             hipStream_t s);
 hipError_t launch_unpack_quartets(const uint64_t* in, int64_t n, int32_t* out,
                                   hipStream_t s);
-// waves of one LL class; first = 1 overwrites the partial rows
+// k_contract<LL>: h(beta) of one LL class (series.hip)
+size_t contract_lds_bytes(int LL, int Tmax);
+hipError_t set_contract_lds_limit(size_t bytes);
+hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t* idx,
+                           int64_t nidx, int blocks, hipStream_t s);
+// k_series<LL>: S(q) of one LL class; first = 1 overwrites the partial rows
 hipError_t launch_series(const SeriesArgs& a, int LL, const int32_t* idx,
                          int64_t nidx, int waves, int first, hipStream_t s);
 hipError_t launch_reduce(const double* part, const double* partc, int waves,
