@@ -130,10 +130,15 @@ def createtwordm(confs, civs, state1=1, state2=1, cutoff=1e-30):
 
 
 def pair_phase(confs, c1, c2):
+    """(ep2(c1,c2), ndiff(c1,c2)) of maxcoincidence for 1-based determinant
+    indices, as the reference's matrices are indexed."""
     confs = _I(confs, "F")
+    nconf = confs.shape[0]
+    if not (1 <= c1 <= nconf and 1 <= c2 <= nconf):
+        raise IndexError(f"determinant index out of 1..{nconf}: {c1}, {c2}")
     ep = np.zeros(1, dtype=np.int32)
     nd = np.zeros(1, dtype=np.int32)
-    lib().orc_pair_phase(_ptr(confs), confs.shape[0], confs.shape[1], c1, c2,
+    lib().orc_pair_phase(_ptr(confs), nconf, confs.shape[1], c1 - 1, c2 - 1,
                          _ptr(ep), _ptr(nd))
     return int(ep[0]), int(nd[0])
 

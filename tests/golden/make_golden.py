@@ -125,7 +125,32 @@ def save(name, inp, q, ref, cutoffs=(1e-15, 1e-20, 0.001)):
     return path
 
 
+REF_SRC = "/root/reference/src"
+
+
+def save_reference_rdm_files():
+    """The reference's two 2-RDM data files as (mat, total) fixtures.
+
+    HF_NonZero2RM_MO_Coeffs: closed-shell determinant, 7 doubly occupied MOs
+    labelled 1-5, 11, 15 (91 rows).  NonZero2RDM_MO_Coeffs: 13 MOs, 10
+    electrons, 4549 rows in first-index-fastest order (src/Main.f90:148).
+    """
+    from thistle_amd import readers
+    out = {}
+    for key, fname in (("hf", "HF_NonZero2RM_MO_Coeffs"), ("cas", "NonZero2RDM_MO_Coeffs")):
+        mat, tot = readers.read_twordm(os.path.join(REF_SRC, fname))
+        out[f"{key}_mat"] = mat.astype(np.int16)
+        out[f"{key}_total"] = tot
+    path = os.path.join(HERE, "ref_rdm_files.npz")
+    np.savez_compressed(path, **out)
+    print(f"ref_rdm_files: hf {len(out['hf_total'])} rows, cas {len(out['cas_total'])} rows, "
+          f"size={os.path.getsize(path)}B")
+
+
 def main(names):
+    if "ref_rdm_files" in names:
+        save_reference_rdm_files()
+        names = [n for n in names if n != "ref_rdm_files"]
     for name in names:
         w, inp, q, ref = run_case(name)
         path = save(name, inp, q, ref)

@@ -1,0 +1,91 @@
+"""The CPU oracle against golden vectors written by the reference's own
+Fortran kernels (flang build of src/twordms.f90, src/one_rdm.f90,
+src/Integrals.f90, src/unique_module.f90, src/calculate_form_factors.f90;
+fixtures made by tests/golden/make_golden.py).
+
+These tests pin the oracle before it is used as the checker of the HIP path.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+CASES = ["h2", "sp3", "spd2", "f2", "xyx_cas44"]
+PI_SP = 3.1415927410125732421875   # acos(-1.0000) in single precision (src/Variables.f90:994)
+
+
+@pytest.fixture(scope="module", params=CASES)
+def case(request, golden):
+    return request.param, golden(request.param)
+
+
+def _cut(g):
+    cz, cmd, cc = (float(v) for v in g["cutoffs"])
+    return dict(cutoffz=cz, cutoffmd=cmd, cutoffcentre=cc)
+
+
+def test_twordm_selection_bit_exact(case):
+    """L1: nonzero 2-RDM rows and values of createtwordm (src/twordms.f90:175-556)."""
+    name, g = case
+    mat, tot = oracle.createtwordm(g["confs"], g["civs"])
+    assert np.array_equal(mat, g["mat"].astype(np.int32)), name
+    assert np.array_equal(tot, g["total"]), name
+
+
+def test_onerdm(case):
+    """createonerdm (src/one_rdm.f90:109-222)."""
+    name, g = case
+    gam = oracle.onerdm(g["confs"], g["civs"])
+    ref = g["onerdm"]
+    n = gam.shape[0]
+    np.testing.assert_allclose(gam, ref[:n, :n], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("zmode", [0, 1])
+def test_total_j0(case, zmode):
+    """tot_integration (src/Integrals.f90:8-472) with the per-quartet dgemm
+    (zmode 0) and with the MO-pair factorised transform (zmode 1)."""
+    name, g = case
+    mat, tot = g["mat"].astype(np.int32), g["total"]
+    S = oracle.total_j0(g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"], g["yy"],
+                        g["zz"], g["mmod"], mat, tot, g["q"], zmode=zmode, **_cut(g))
+    rel = np.max(np.abs(S - g["S_total"]) / np.abs(g["S_total"]))
+    assert rel < 1e-12, (name, rel)
+
+
+def test_elastic_j0(case):
+    """elastic_integration (src/Integrals.f90:1575-1939) with variables_elastic's Z."""
+    name, g = case
+    gam = oracle.onerdm(g["confs"], g["civs"])
+    Z = oracle.elastic_z(gam, g["mmod"])
+    S = oracle.elastic_j0(g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"], g["yy"],
+                          g["zz"], Z, g["q"], **_cut(g))
+    rel = np.max(np.abs(S - g["S_elastic"]) / np.abs(g["S_elastic"]))
+    assert rel < 1e-12, (name, rel)
+
+
+def test_unique_total(case):
+    """L3: unique_total (src/unique_module.f90:90-145) on the 2-RDM rows."""
+    name, g = case
+    rmat, rtot, iuni, irec = oracle.unique_total(g["mat"], g["total"])
+    assert np.array_equal(rmat, g["rmat"].astype(np.int32)), name
+    assert np.array_equal(rtot, g["rtot"]), name
+
+
+def test_p0_table(case):
+    """set_P0 (src/Integrals.f90:3001-3031): sum and absolute sum of the table."""
+    name, g = case
+    P0 = oracle.set_p0(4 * int(g["l"].max()), g["q"])
+    s, a = P0.sum(), np.abs(P0).sum()
+    np.testing.assert_allclose([s, a], g["P0_sum"], rtol=1e-13)
+
+
+def test_forward_limits(case):
+    """S(q -> 0): total = N(N-1), elastic = N^2, both times (pi_sp/pi)^3 from
+    e12's single-precision pi, for Loewdin-orthonormal MOs."""
+    name, g = case
+    N = int(np.count_nonzero(g["confs"][0]))
+    f = (PI_SP / np.pi) ** 3
+    assert g["q"][0] <= 1e-10
+    np.testing.assert_allclose(g["S_total"][0], N * (N - 1) * f, rtol=1e-9)
+    np.testing.assert_allclose(g["S_elastic"][0], N * N * f, rtol=1e-9)

@@ -411,19 +411,24 @@ int run_series(ThistlePlan* pl, double* result_host) {
 int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const int32_t* mat,
                      const double* total, int32_t norb, const double* onerdm,
                      const ThistleOpts* o, ThistlePlan** out) {
+    // every host-side argument check comes before the first HIP call
     if (type != 1 && type != 2) return THISTLE_ETYPE;
-    std::unique_ptr<ThistlePlan> pl(new ThistlePlan());
-    pl->type = type;
-    int rc = setup_common(pl.get(), b, o);
-    if (rc) return rc;
-    if (!b->mmod || b->nmo <= 0) return THISTLE_EINVAL;
+    if (!b || !b->mmod || b->nmo <= 0) return THISTLE_EINVAL;
+    int nrb = 0;
     if (type == 1) {
         if (nmat < 0 || (nmat > 0 && (!mat || !total))) return THISTLE_EINVAL;
-        int nrb = 0;
         for (int64_t k = 0; k < 4 * nmat; ++k) {
             if (mat[k] < 1 || mat[k] > b->nmo || mat[k] > 65535) return THISTLE_EINVAL;
             nrb = std::max(nrb, (int)mat[k]);
         }
+    } else if (norb <= 0 || !onerdm || norb > b->nmo) {
+        return THISTLE_EINVAL;
+    }
+    std::unique_ptr<ThistlePlan> pl(new ThistlePlan());
+    pl->type = type;
+    int rc = setup_common(pl.get(), b, o);
+    if (rc) return rc;
+    if (type == 1) {
         pl->norb = std::max(nrb, 1);
         pl->nmat = nmat;
         pl->npmo = pl->norb * (pl->norb + 1) / 2;
@@ -460,7 +465,6 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
         if (rocblas_create_handle(&pl->blas) != rocblas_status_success) return THISTLE_EDEVICE;
         BCHECK(rocblas_set_stream(pl->blas, pl->stream));
     } else {
-        if (norb <= 0 || !onerdm || norb > b->nmo) return THISTLE_EINVAL;
         pl->norb = norb;
         if ((rc = dalloc(pl->onerdm, sizeof(double) * norb * norb))) return rc;
         if ((rc = dalloc(pl->Z, sizeof(double) * (size_t)b->ngto * b->ngto))) return rc;
