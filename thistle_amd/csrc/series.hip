@@ -375,20 +375,31 @@ __device__ __noinline__ double f_dphrec(const double (&hs)[LL + 1], double q, do
     return F;
 }
 
+// Upward recurrence from j0, j1 keeps every j_b (b <= LL) within 1e-14 of
+// its envelope (|j_b| below the first zero, the spherical Hankel modulus
+// above it) for x >= 0.9 LL + 1; below that Miller's backward recurrence is
+// used.
+template <int LL>
+struct SeriesRegime {
+    static constexpr double xup = LL == 0 ? 0.0 : 0.9 * LL + 1.0;
+    // backward start: at least 16 orders above max(LL, x) for x < xup
+    static constexpr int nstart = LL + 16 + (int)(0.9 * LL + 2.0);
+};
+
 // Miller backward recurrence for g_b = j_b(x)/x^b, normalised by
 // g_0 = sin(x)/x:  F = sum_b h(b) q^(2b) g_b.  FP64 (small arguments, where
-// the terms cancel).
+// the terms cancel); fixed trip count so that a wave does not diverge.
 template <int LL>
 __device__ __forceinline__ double f_miller(const double (&hs)[LL + 1], double q, double x,
                                            double j0) {
     const double x2 = x * x, q2 = q * q;
-    const int N = LL + 16 + (int)x;
-    double vp1 = 0.0, v = 1.0, tb = 2.0 * N + 1.0;
+    constexpr int N = SeriesRegime<LL>::nstart;
+    double vp1 = 0.0, v = 1.0;
+#pragma unroll
     for (int b = N; b > LL; --b) {
-        double vm = fma(tb, v, -x2 * vp1);
+        double vm = fma((double)(2 * b + 1), v, -x2 * vp1);
         vp1 = v;
         v = vm;
-        tb -= 2.0;
     }
     double acc = hs[LL] * v;
 #pragma unroll
@@ -461,9 +472,10 @@ __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], double q, d
     double sx, cx;
     sincos_cw(x, sx, cx);
     const double inv = rcp64(x);
-    const double j0 = sx * inv;
+    // q = 0 (x = 0): the j0 limit 1 instead of 0/0
+    const double j0 = x > 0.0 ? sx * inv : 1.0;
     if (x >= XDPH) return f_dphrec<LL>(hs, q, h, x, sx, cx);
-    if (x > (double)LL + 6.0) return f_up64<LL>(hs, q, invh, inv, j0, cx);
+    if (LL == 0 || x >= SeriesRegime<LL>::xup) return f_up64<LL>(hs, q, invh, inv, j0, cx);
     return f_miller<LL>(hs, q, x, j0);
 }
 
@@ -480,7 +492,14 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
         qv[s] = A.q[iqs[s]];
         S[s] = Sc[s] = 0.0;
     }
-    for (int64_t kq = blockIdx.x; kq < nidx; kq += gridDim.x) {
+    // a contiguous chunk of the class list per wave: consecutive quartets
+    // of one nest share their bra pair, whose e12 row stays in registers
+    const int64_t per = (nidx + gridDim.x - 1) / gridDim.x;
+    const int64_t k0 = (int64_t)blockIdx.x * per;
+    const int64_t k1 = k0 + per < nidx ? k0 + per : nidx;
+    int pbl = -1;
+    double ebv[SQ];
+    for (int64_t kq = k0; kq < k1; ++kq) {
         const int64_t e = idx[kq];
         const uint64_t ent = A.qlist[e];
         const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
@@ -494,12 +513,17 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
         double hs[LL + 1];
 #pragma unroll
         for (int b = 0; b <= LL; ++b) hs[b] = A.hs[e * HS + b];
-        const double* eb = A.e12 + (int64_t)pb * A.nq;
+        if (pb != pbl) {
+            const double* eb = A.e12 + (int64_t)pb * A.nq;
+#pragma unroll
+            for (int s = 0; s < SQ; ++s) ebv[s] = eb[iqs[s]];
+            pbl = pb;
+        }
         const double* ek = A.e12 + (int64_t)pk * A.nq;
 #pragma unroll
         for (int s = 0; s < SQ; ++s) {
             double v = mult * f_eval<LL, MIXED>(hs, qv[s], h, invh, br);
-            v = v * eb[iqs[s]];
+            v = v * ebv[s];
             v = v * ek[iqs[s]];
             kahan(S[s], Sc[s], v);
         }
