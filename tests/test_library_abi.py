@@ -142,6 +142,9 @@ def test_status_codes_before_device(lib):
     assert _status(1, ga, hl, hm, n, xyz, group, mmod, q, mat, tot) == 3       # ELLMAX
     assert _status(1, ga, l, m, n, xyz, np.array([1, 2, 1, 2]), mmod, q, mat, tot) == 4
     assert _status(1, ga, l, m, n, xyz, np.array([1, 1, 1, 3]), mmod, q, mat, tot) == 4
+    mixed_l = n.copy()
+    mixed_l[2] = 2                                     # a d component in a p shell
+    assert _status(1, ga, l, m, mixed_l, xyz, group, mmod, q, mat, tot) == 4
     assert _status(1, ga, l, m, n, xyz, group, mmod, q, np.array([[1, 1, 3, 1]]), tot) == 1
     assert _status(2, ga, l, m, n, xyz, group, mmod, q, None, None, 3, np.eye(3)) == 1
     assert _status(2, ga, l, m, n, xyz, group, mmod, q, None, None, 0, None) == 1

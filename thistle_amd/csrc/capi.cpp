@@ -88,6 +88,8 @@ struct ThistlePlan {
     int64_t cls_off[18] = {0};  // per-LL ranges in `cls` (positions, ref order)
     int cls_tmax[17] = {0};     // largest exponent-tuple count per LL class
     DevBuf hs;                  // h(beta) per quartet
+    DevBuf quadtab, comboff;    // GTO-quadruple tables per shell-signature combination
+    int nsig = 0;
     hipEvent_t ev[6] = {};
     double ms[5] = {0, 0, 0, 0, 0};
     ~ThistlePlan() {
@@ -100,7 +102,10 @@ struct ThistlePlan {
 
 namespace {
 
-int check_basis(const ThistleBasis* b, std::vector<GroupInfo>& groups, int& maxl_eff) {
+// Shell signature: the ordered (l,m,n) list of a group's GTOs.  Quartets
+// whose four groups have the same signatures share one GTO-quadruple table.
+int check_basis(const ThistleBasis* b, std::vector<GroupInfo>& groups, int& maxl_eff,
+                std::vector<std::vector<int32_t>>* sigs = nullptr) {
     if (!b || b->ngto <= 0 || b->ng <= 0 || b->nq <= 0 || !b->ga || !b->l || !b->m ||
         !b->n || !b->xx || !b->yy || !b->zz || !b->group || !b->q)
         return THISTLE_EINVAL;
@@ -133,10 +138,33 @@ int check_basis(const ThistleBasis* b, std::vector<GroupInfo>& groups, int& maxl
         for (int j = groups[g].start; j < groups[g].start + groups[g].count; ++j)
             if (b->group[j] != g + 1) return THISTLE_EGROUP;
     }
+    if (sigs) {
+        sigs->clear();
+        for (int g = 0; g < b->ng; ++g) {
+            std::vector<int32_t> key;
+            for (int j = groups[g].start; j < groups[g].start + groups[g].count; ++j)
+                key.push_back((b->l[j] << 8) | (b->m[j] << 4) | b->n[j]);
+            int id = -1;
+            for (size_t k = 0; k < sigs->size(); ++k)
+                if ((*sigs)[k] == key) id = (int)k;
+            if (id < 0) {
+                if (sigs->size() >= 32) return THISTLE_EGROUP;
+                id = (int)sigs->size();
+                sigs->push_back(key);
+            }
+            groups[g].pad = id;
+        }
+    }
     // LLmax + 1 > 20 for the largest quartet (src/Integrals.f90:5117-5120)
     int Lmax = 0;
     for (auto& g : groups) Lmax = std::max(Lmax, g.L);
     if (4 * Lmax + 1 > 20) return THISTLE_ELLMAX;
+    // one primitive shell per group: at most 15 cartesian components of one L
+    for (int g = 0; g < b->ng; ++g) {
+        if (groups[g].count > 15) return THISTLE_EGROUP;
+        for (int j = groups[g].start; j < groups[g].start + groups[g].count; ++j)
+            if (b->l[j] + b->m[j] + b->n[j] != groups[g].L) return THISTLE_EGROUP;
+    }
     return THISTLE_OK;
 }
 
@@ -164,9 +192,11 @@ std::vector<EnumRow> enum_rows(int ng, int64_t& total) {
 
 int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     std::vector<GroupInfo> groups;
+    std::vector<std::vector<int32_t>> sigs;
     int maxl = 0;
-    int rc = check_basis(b, groups, maxl);
+    int rc = check_basis(b, groups, maxl, &sigs);
     if (rc) return rc;
+    const int S = (int)sigs.size();
     pl->precision = o->precision;
     pl->ngto = b->ngto;
     pl->ng = b->ng;
@@ -216,6 +246,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     pl->nrows = (int)rows.size();
     // positions of each LL = L_i+L_j+L_k+L_r class, in reference order
     std::vector<int32_t> cls(Q);
+    std::vector<char> used((size_t)S * S * S * S, 0);
     {
         std::vector<int64_t> cnt(18, 0);
         auto LLof = [&](const EnumRow& R, int x) {
@@ -228,6 +259,8 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
                 default: gi = gj = gk = gr = x; break;
             }
             int Ls[4] = {groups[gi].L, groups[gj].L, groups[gk].L, groups[gr].L};
+            used[((groups[gi].pad * S + groups[gj].pad) * S + groups[gk].pad) * S +
+                 groups[gr].pad] = 1;
             int T = (Ls[0] + 1) * (Ls[1] + 1) * (Ls[2] + 1) * (Ls[3] + 1);
             int LL = Ls[0] + Ls[1] + Ls[2] + Ls[3];
             pl->cls_tmax[LL] = std::max(pl->cls_tmax[LL], T);
@@ -253,6 +286,49 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
         for (const auto& R : rows)
             for (int u = 0; u < R.count; ++u) cls[fill[LLof(R, R.r0 + u)]++] = (int32_t)(R.out + u);
     }
+    // GTO-quadruple tables of the signature combinations in use.  Entry u of
+    // a combination (d fastest): bits 0-7 a*cj+b, 8-15 c*cr+d (offsets in the
+    // bra / ket pair blocks), 16-25 / 26-35 / 36-45 the x / y / z exponent
+    // tuple index, 46-61 a, b, c, d (4 bits each).
+    std::vector<int64_t> comboff((size_t)S * S * S * S, 0);
+    std::vector<uint64_t> qtab;
+    for (int si = 0; si < S; ++si)
+        for (int sj = 0; sj < S; ++sj)
+            for (int sk = 0; sk < S; ++sk)
+                for (int sr = 0; sr < S; ++sr) {
+                    size_t c = ((size_t)(si * S + sj) * S + sk) * S + sr;
+                    if (!used[c]) continue;
+                    comboff[c] = (int64_t)qtab.size();
+                    const std::vector<int32_t>* sg[4] = {&sigs[si], &sigs[sj], &sigs[sk],
+                                                         &sigs[sr]};
+                    int cnt4[4], L4[4];
+                    for (int k = 0; k < 4; ++k) {
+                        cnt4[k] = (int)sg[k]->size();
+                        int v = (*sg[k])[0];
+                        L4[k] = (v >> 8) + ((v >> 4) & 15) + (v & 15);
+                    }
+                    for (int a = 0; a < cnt4[0]; ++a)
+                        for (int bb = 0; bb < cnt4[1]; ++bb)
+                            for (int cc = 0; cc < cnt4[2]; ++cc)
+                                for (int d = 0; d < cnt4[3]; ++d) {
+                                    int e4[4] = {(*sg[0])[a], (*sg[1])[bb], (*sg[2])[cc],
+                                                 (*sg[3])[d]};
+                                    uint64_t t3[3];
+                                    for (int ax = 0; ax < 3; ++ax) {
+                                        int sh = 8 - 4 * ax, t = 0;
+                                        for (int k = 0; k < 4; ++k)
+                                            t = t * (L4[k] + 1) + ((e4[k] >> sh) & 15);
+                                        t3[ax] = (uint64_t)t;
+                                    }
+                                    uint64_t w = (uint64_t)(a * cnt4[1] + bb) |
+                                                 ((uint64_t)(cc * cnt4[3] + d) << 8) |
+                                                 (t3[0] << 16) | (t3[1] << 26) | (t3[2] << 36) |
+                                                 ((uint64_t)a << 46) | ((uint64_t)bb << 50) |
+                                                 ((uint64_t)cc << 54) | ((uint64_t)d << 58);
+                                    qtab.push_back(w);
+                                }
+                }
+    pl->nsig = S;
     int L1 = maxl + 1, T1 = 2 * maxl + 1;
     if ((rc = dalloc(pl->G, sizeof(GroupInfo) * ng))) return rc;
     if ((rc = dalloc(pl->P, sizeof(PairInfo) * pl->npair))) return rc;
@@ -271,6 +347,12 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if ((rc = dalloc(pl->xb, sizeof(int32_t) * pl->npp))) return rc;
     if ((rc = dalloc(pl->cls, sizeof(int32_t) * Q))) return rc;
     if ((rc = dalloc(pl->hs, sizeof(double) * 17 * (size_t)Q))) return rc;
+    if ((rc = dalloc(pl->quadtab, sizeof(uint64_t) * qtab.size()))) return rc;
+    if ((rc = dalloc(pl->comboff, sizeof(int64_t) * comboff.size()))) return rc;
+    HCHECK(hipMemcpyAsync(pl->quadtab.p, qtab.data(), sizeof(uint64_t) * qtab.size(),
+                          hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(pl->comboff.p, comboff.data(), sizeof(int64_t) * comboff.size(),
+                          hipMemcpyHostToDevice, pl->stream));
     HCHECK(hipMemcpyAsync(pl->cls.p, cls.data(), sizeof(int32_t) * Q, hipMemcpyHostToDevice,
                           pl->stream));
     HCHECK(hipStreamSynchronize(pl->stream));
@@ -339,8 +421,8 @@ int run_transform_total(ThistlePlan* pl) {
     BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_none, rocblas_operation_none, npmo,
                          (rocblas_int)pl->npp, npmo, &one, pl->Gs.as<double>(), npmo,
                          pl->T.as<double>(), npmo, &zero, pl->M.as<double>(), npmo));
-    // Z = T^T * M  (npp x npp, symmetric): upper triangle only
-    BCHECK(rocblas_dsyrkx(pl->blas, rocblas_fill_upper, rocblas_operation_transpose,
+    // Z = T^T * M  (npp x npp, symmetric): lower triangle only
+    BCHECK(rocblas_dsyrkx(pl->blas, rocblas_fill_lower, rocblas_operation_transpose,
                           (rocblas_int)pl->npp, npmo, &one, pl->T.as<double>(), npmo,
                           pl->M.as<double>(), npmo, &zero, pl->Z.as<double>(),
                           (rocblas_int)pl->npp));
@@ -371,6 +453,9 @@ int run_series(ThistlePlan* pl, double* result_host) {
     a.partc = pl->partc.as<double>();
     a.flags = pl->flags.as<uint8_t>();
     a.hs = pl->hs.as<double>();
+    a.quadtab = pl->quadtab.as<uint64_t>();
+    a.comboff = pl->comboff.as<int64_t>();
+    a.nsig = pl->nsig;
     a.nqc = std::min(512, pl->nq);
     // h(beta) of every quartet (q independent), heaviest classes first
     for (int LL = 16; LL >= 0; --LL) {

@@ -34,6 +34,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 #define WAVE 64
 
 #define HS 17       // stride of the h(beta) buffer (LL <= 16)
@@ -151,39 +153,37 @@ constexpr ACoef make_acoef() {
 }
 __constant__ ACoef kAcoef = make_acoef();
 
-__device__ __forceinline__ double a_closed(int l, int p, double x) {
-    double xp = 1.0;
-    for (int i = 0; i < p; ++i) xp *= x;
-    return kAcoef.v[l][p] * xp;
-}
-
-// floor(u / d) for u < 2^16 and 1 <= d <= 15 with m = ceil(2^32 / d); d = 1
-// has no 32-bit multiplier and is passed through.
-__device__ __forceinline__ uint32_t magic_div(uint32_t d) {
-    return d == 1 ? 0u : (uint32_t)((0xffffffffull + d) / d);
-}
-__device__ __forceinline__ uint32_t div_small(uint32_t u, uint32_t d, uint32_t m) {
-    return d == 1 ? u : __umulhi(u, m);
-}
-
 }  // namespace
 
 // ------------------------------------------------------------ contraction
-template <int LL, int CT>
-__global__ void __launch_bounds__(CT) k_contract(SeriesArgs A, const int32_t* __restrict__ idx,
-                                                 int64_t nidx) {
+// G lanes per quartet: small classes (a few GTO quadruples and exponent
+// tuples per quartet) share one wave between 64/G quartets; from LL = 10 on
+// a quartet has thousands of quadruples and takes a 256-thread workgroup.
+template <int LL>
+struct ContractShape {
+    static constexpr int CT = LL < 10 ? 64 : 256;
+    static constexpr int G = LL <= 1 ? 4 : LL == 2 ? 8 : LL == 3 ? 16 : LL == 4 ? 32 : CT;
+    static constexpr int NG = CT / G;
+};
+
+template <int LL>
+__global__ void __launch_bounds__(ContractShape<LL>::CT)
+    k_contract(SeriesArgs A, const int32_t* __restrict__ idx, int64_t nidx, int tmax) {
     constexpr int D = LL + 1;
+    constexpr int CT = ContractShape<LL>::CT, G = ContractShape<LL>::G, NG = ContractShape<LL>::NG;
     constexpr int NW = CT / WAVE;
     extern __shared__ double sm[];
-    double* At = sm;              // [3][D][D]
-    double* red = At + 3 * D * D; // [4][D]
-    double* PX = red + 4 * D;     // [3][T][D]
     __shared__ int anyw[NW];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int grp = tid / G, lg = tid % G;
+    double* PX = sm + (size_t)grp * 3 * tmax * D;  // [3][T][D] of this group's quartet
+    double* red = sm + (size_t)NG * 3 * tmax * D;  // [NW][D] when a quartet spans waves
     const int L1 = A.maxl + 1, T1 = 2 * A.maxl + 1;
 
-    for (int64_t kq = blockIdx.x; kq < nidx; kq += gridDim.x) {
-        const int64_t e = idx[kq];
+    for (int64_t base = (int64_t)blockIdx.x * NG; base < nidx; base += (int64_t)gridDim.x * NG) {
+        const int64_t kq = base + grp;
+        const bool valid = kq < nidx;
+        const int64_t e = idx[valid ? kq : base];
         const uint64_t ent = A.qlist[e];
         const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
         const int br = QE_BR(ent);
@@ -191,17 +191,15 @@ __global__ void __launch_bounds__(CT) k_contract(SeriesArgs A, const int32_t* __
         const PairInfo PB = A.P[pb], PK = A.P[pk];
         const GroupInfo Gi = A.G[gi], Gj = A.G[gj], Gk = A.G[gk], Gr = A.G[gr];
         const int Li = Gi.L, Lj = Gj.L, Lk = Gk.L, Lr = Gr.L;
-        const int T = (Li + 1) * (Lj + 1) * (Lk + 1) * (Lr + 1);
-        double hxyz[3] = {PK.Px - PB.Px, PK.Py - PB.Py, PK.Pz - PB.Pz};
+        const int T = valid ? (Li + 1) * (Lj + 1) * (Lk + 1) * (Lr + 1) : 0;
+        const double hx = br ? 0.0 : PK.Px - PB.Px, hy = br ? 0.0 : PK.Py - PB.Py,
+                     hz = br ? 0.0 : PK.Pz - PB.Pz;
 
-        __syncthreads();  // the previous quartet is done with the tables
-        for (int w = tid; w < 3 * D * D; w += CT) {
-            int ax = w / (D * D), l = (w / D) % D, p = w % D;
-            At[w] = a_closed(l, p, br ? 0.0 : hxyz[ax]);
-        }
-        __syncthreads();
+        __syncthreads();  // the previous quartets are done with the tables
+        // px(m) = sum_T c(T) a(T, 2m - T) per axis and exponent tuple, with
+        // a(l,p) = kAcoef(l,p) h^p
         const double* E0 = A.E;
-        for (int w = tid; w < 3 * T; w += CT) {
+        for (int w = lg; w < 3 * T; w += G) {
             int ax = w / T, t = w % T;
             int ld = t % (Lr + 1), t2 = t / (Lr + 1);
             int lc = t2 % (Lk + 1);
@@ -225,51 +223,51 @@ __global__ void __launch_bounds__(CT) k_contract(SeriesArgs A, const int32_t* __
                 for (int t3 = 0; t3 <= T2; ++t3) s += eb[t3] * ek[T2 - t3];
                 cx[T2] = s;
             }
-            const double* Aax = At + ax * D * D;
+            const double hax = ax == 0 ? hx : (ax == 1 ? hy : hz);
+            double hp[D];
+            hp[0] = 1.0;
+#pragma unroll
+            for (int k = 1; k < D; ++k) hp[k] = hp[k - 1] * hax;
             double* out = PX + ((int64_t)ax * T + t) * D;
 #pragma unroll
             for (int m = 0; m < D; ++m) {
                 double s = 0.0;
 #pragma unroll
-                for (int T2 = m; T2 <= 2 * m && T2 < D; ++T2) s += cx[T2] * Aax[T2 * D + (2 * m - T2)];
+                for (int T2 = m; T2 <= 2 * m && T2 < D; ++T2)
+                    s += cx[T2] * (kAcoef.v[T2][2 * m - T2] * hp[2 * m - T2]);
                 out[m] = s;
             }
         }
         __syncthreads();
 
-        const int ci = Gi.count, cj = Gj.count, ck = Gk.count, cr = Gr.count;
-        const uint32_t nq4 = (uint32_t)(ci * cj * ck * cr);
-        const uint32_t Mr = magic_div(cr), Mk = magic_div(ck), Mj = magic_div(cj);
-        const int sTz = (Lj + 1) * (Lk + 1) * (Lr + 1);  // stride of la in a tuple
+        const uint32_t nq4 = valid ? (uint32_t)(Gi.count * Gj.count * Gk.count * Gr.count) : 0u;
+        const int S = A.nsig;
+        const uint64_t* qt =
+            A.quadtab + A.comboff[((Gi.pad * S + Gj.pad) * S + Gk.pad) * S + Gr.pad];
         double h[D];
 #pragma unroll
         for (int b = 0; b < D; ++b) h[b] = 0.0;
         int any = 0;
-        for (uint32_t u = tid; u < nq4; u += CT) {
-            uint32_t t1 = div_small(u, cr, Mr);
-            int d_ = (int)(u - t1 * cr);
-            uint32_t t2 = div_small(t1, ck, Mk);
-            int c_ = (int)(t1 - t2 * ck);
-            uint32_t a_ = div_small(t2, cj, Mj);
-            int b_ = (int)(t2 - a_ * cj);
-            const int ia = Gi.start + (int)a_, ib = Gj.start + b_, ic = Gk.start + c_,
-                      id = Gr.start + d_;
+        for (uint32_t u = lg; u < nq4; u += G) {
+            const uint64_t w = qt[u];
             double z;
             if (!A.elastic) {
-                int64_t xb = PB.off + (int)a_ * cj + b_;
-                int64_t xk = PK.off + c_ * cr + d_;
-                // Z is symmetric and only its upper triangle is stored
-                z = xk <= xb ? A.Z[xk + A.ldz * xb] : A.Z[xb + A.ldz * xk];
+                const int64_t xb = PB.off + (int)(w & 255u);
+                const int64_t xk = PK.off + (int)((w >> 8) & 255u);
+                // Z is symmetric and only its lower triangle is stored; ket
+                // pairs mostly follow their bra, so xk >= xb reads contiguous rows
+                z = xk >= xb ? A.Z[xk + A.ldz * xb] : A.Z[xb + A.ldz * xk];
             } else {
+                const int ia = Gi.start + (int)((w >> 46) & 15u),
+                          ib = Gj.start + (int)((w >> 50) & 15u),
+                          ic = Gk.start + (int)((w >> 54) & 15u),
+                          id = Gr.start + (int)((w >> 58) & 15u);
                 z = A.Z[ic + A.ldz * id] * A.Z[ia + A.ldz * ib];
             }
             if (fabs(z) < A.cutoffz) continue;
             any = 1;
-            const char4 la = A.lxyz[ia], lb = A.lxyz[ib], lc = A.lxyz[ic], ld = A.lxyz[id];
-            const int tx = ((la.x * (Lj + 1) + lb.x) * (Lk + 1) + lc.x) * (Lr + 1) + ld.x;
-            const int ty = ((la.y * (Lj + 1) + lb.y) * (Lk + 1) + lc.y) * (Lr + 1) + ld.y;
-            const int tz = ((la.z * (Lj + 1) + lb.z) * (Lk + 1) + lc.z) * (Lr + 1) + ld.z;
-            (void)sTz;
+            const int tx = (int)((w >> 16) & 1023u), ty = (int)((w >> 26) & 1023u),
+                      tz = (int)((w >> 36) & 1023u);
             const double* px = PX + (int64_t)tx * D;
             const double* py = PX + ((int64_t)T + ty) * D;
             const double* pz = PX + ((int64_t)2 * T + tz) * D;
@@ -296,25 +294,47 @@ __global__ void __launch_bounds__(CT) k_contract(SeriesArgs A, const int32_t* __
                 h[b] = fma(z, s, h[b]);
             }
         }
+        if constexpr (G <= WAVE) {
+            // butterfly over the group's lanes; every lane ends with the sums
 #pragma unroll
-        for (int b = 0; b < D; ++b) {
-            double v = wsum(h[b]);
-            if (lane == 0) red[wid * D + b] = v;
-        }
-        int wa = __any(any);
-        if (lane == 0) anyw[wid] = wa;
-        __syncthreads();
-        if (tid < D) {
-            double s = red[tid];
+            for (int b = 0; b < D; ++b) {
+                double v = h[b];
 #pragma unroll
-            for (int w = 1; w < NW; ++w) s += red[w * D + tid];
-            A.hs[e * HS + tid] = s;
-        }
-        if (tid == 0) {
-            int f = 0;
+                for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+                h[b] = v;
+            }
+            const uint64_t bal = __ballot(any);
+            const uint64_t gmask =
+                G == WAVE ? ~0ull : (((1ull << G) - 1ull) << ((lane / G) * G));
+            if (valid) {
+                double mine = 0.0;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) f |= anyw[w];
-            A.flags[e] = f ? 1 : 0;
+                for (int b = 0; b < D; ++b)
+                    if (lg == b) mine = h[b];
+                if (lg < D) A.hs[e * HS + lg] = mine;
+                if (lg == 0) A.flags[e] = (bal & gmask) ? 1 : 0;
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                double v = wsum(h[b]);
+                if (lane == 0) red[wid * D + b] = v;
+            }
+            int wa = __any(any);
+            if (lane == 0) anyw[wid] = wa;
+            __syncthreads();
+            if (valid && tid < D) {
+                double s = red[tid];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) s += red[w * D + tid];
+                A.hs[e * HS + tid] = s;
+            }
+            if (valid && tid == 0) {
+                int f = 0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) f |= anyw[w];
+                A.flags[e] = f ? 1 : 0;
+            }
         }
     }
 }
@@ -547,26 +567,42 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
 }
 
 // ----------------------------------------------------------------- launchers
+namespace {
+template <int LL>
+size_t contract_lds_for(int Tmax) {
+    constexpr int D = LL + 1, NG = ContractShape<LL>::NG, NW = ContractShape<LL>::CT / WAVE;
+    return sizeof(double) * ((size_t)NG * 3 * Tmax * D + (NG == 1 ? (size_t)NW * D : 0));
+}
+}  // namespace
+
+#define ALL_LL(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) \
+    X(14) X(15) X(16)
+
 size_t contract_lds_bytes(int LL, int Tmax) {
-    int D = LL + 1;
-    return sizeof(double) * (size_t)(3 * D * D + 4 * D + 3 * Tmax * D);
+    switch (LL) {
+#define C_(k) \
+    case k: return contract_lds_for<k>(Tmax);
+        ALL_LL(C_)
+#undef C_
+        default: return 0;
+    }
 }
 
 hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t* idx,
-                           int64_t nidx, int blocks, hipStream_t s) {
+                           int64_t nidx, int max_blocks, hipStream_t s) {
     if (nidx <= 0) return hipSuccess;
     size_t lds = contract_lds_bytes(LL, Tmax);
-    // one wave per quartet up to LL = 9 (tens to hundreds of quadruples),
-    // four waves per quartet above (thousands)
     switch (LL) {
-#define CASE(k, nt)                                                                          \
-    case k:                                                                                  \
-        hipLaunchKernelGGL((k_contract<k, nt>), dim3(blocks), dim3(nt), lds, s, a, idx, nidx); \
-        break;
-        CASE(0, 64) CASE(1, 64) CASE(2, 64) CASE(3, 64) CASE(4, 64) CASE(5, 64) CASE(6, 64)
-        CASE(7, 64) CASE(8, 64) CASE(9, 64) CASE(10, 256) CASE(11, 256) CASE(12, 256)
-        CASE(13, 256) CASE(14, 256) CASE(15, 256) CASE(16, 256)
-#undef CASE
+#define C_(k)                                                                                  \
+    case k: {                                                                                  \
+        constexpr int NG = ContractShape<k>::NG;                                               \
+        int blocks = (int)std::min<int64_t>(max_blocks, (nidx + NG - 1) / NG);                 \
+        hipLaunchKernelGGL((k_contract<k>), dim3(blocks), dim3(ContractShape<k>::CT), lds, s, a, \
+                           idx, nidx, Tmax);                                                   \
+        break;                                                                                 \
+    }
+        ALL_LL(C_)
+#undef C_
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -575,17 +611,15 @@ hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t*
 int contract_threads(int LL) { return LL < 10 ? 64 : 256; }
 
 hipError_t set_contract_lds_limit(size_t bytes) {
-#define SETA(k, nt)                                                                      \
+#define C_(k)                                                                            \
     {                                                                                    \
-        hipError_t e = hipFuncSetAttribute((const void*)k_contract<k, nt>,               \
+        hipError_t e = hipFuncSetAttribute((const void*)k_contract<k>,                   \
                                            hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                            (int)bytes);                                  \
         if (e != hipSuccess) return e;                                                   \
     }
-    SETA(0, 64) SETA(1, 64) SETA(2, 64) SETA(3, 64) SETA(4, 64) SETA(5, 64) SETA(6, 64)
-    SETA(7, 64) SETA(8, 64) SETA(9, 64) SETA(10, 256) SETA(11, 256) SETA(12, 256)
-    SETA(13, 256) SETA(14, 256) SETA(15, 256) SETA(16, 256)
-#undef SETA
+    ALL_LL(C_)
+#undef C_
     return hipSuccess;
 }
 

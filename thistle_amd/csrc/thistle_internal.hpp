@@ -10,7 +10,7 @@
 // centre and exponent (src/molden_reader_nikola.py:195-251).
 struct GroupInfo {
     double x, y, z, ga;
-    int32_t start, count, L, pad;
+    int32_t start, count, L, pad;  // pad: shell-signature id (see setup_common)
 };
 
 // Group pair gi <= gj, the bra or ket of a quartet.
@@ -72,6 +72,9 @@ struct SeriesArgs {
     double* partc;            // [waves][nqc] Kahan compensations
     uint8_t* flags;           // [nquart] block passes the ztot screen
     double* hs;               // [nquart][17] h(beta) per quartet
+    const uint64_t* quadtab;  // GTO-quadruple tables (packed, see setup_common)
+    const int64_t* comboff;   // [nsig^4] table start per signature combination
+    int nsig;
 };
 
 hipError_t launch_pair_info(const GroupInfo* G, PairInfo* P, int npair,
