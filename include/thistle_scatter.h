@@ -42,7 +42,8 @@ enum {
                                 src/Variables.f90:681-682,939-940)          */
     THISTLE_ELLMAX = 3,      /* quartet LLmax + 1 > 20
                                 (src/Integrals.f90:5117-5120)               */
-    THISTLE_EGROUP = 4,      /* a group is not one contiguous run of GTOs    */
+    THISTLE_EGROUP = 4,      /* a group is not one contiguous run of GTOs of
+                                one shell (same l+m+n, at most 15 GTOs)     */
     THISTLE_EDEVICE = 5,     /* HIP or rocBLAS failure                       */
     THISTLE_ENOMEM = 6,      /* device allocation failed                     */
     THISTLE_ETYPE = 7        /* calculation type other than 1 (total j0) or

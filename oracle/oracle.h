@@ -19,7 +19,9 @@
  *   - e12[(gi*ng+gj)*nq + iq], px/py/pz[gi*ng+gj].
  *   - P0[((L*D+M)*D+N)*nq + iq], D = 4*maxl+1.
  *
- * Parity pins for each function are listed in tests/test_oracle.py.
+ * Parity pins for each function: tests/test_oracle_reference.py (against the
+ * flang build of the reference) and tests/test_oracle_math.py (closed forms,
+ * quadrature, brute force, the reference's own test vectors and data files).
  */
 #ifndef THISTLE_ORACLE_H
 #define THISTLE_ORACLE_H

@@ -5,10 +5,14 @@
 //   1. pair tables   (k_pair_info, k_pair_E, k_e12)
 //   2. enumeration   (k_enumerate: the reference's five nests)
 //   3. transform     (unique rows -> MO-pair matrix Gs -> T -> Z = T^T Gs T
-//                     with two rocBLAS GEMMs on the matrix cores), or the
+//                     with a rocBLAS GEMM and a symmetric rank-k update that
+//                     writes Z's lower triangle, on the matrix cores), or the
 //                     elastic Z of the 1-RDM
-//   4. quartet kernel (k_series: contraction + Bessel series per quartet)
-//   5. reduction + stable compaction of the screened quartet list
+//   4. quartet kernels, per LL class: k_contract (h(beta) of each quartet)
+//                     then k_series (Bessel series times e12 weights into
+//                     per-wave compensated partial sums of S(q))
+//   5. reduction of the partial sums + stable compaction of the screened
+//                     quartet list
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
