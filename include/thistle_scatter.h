@@ -95,6 +95,22 @@ int32_t thistle_total_from_twordm(const ThistleBasis *basis, int64_t nmat,
                                   const int32_t *mat, const double *total,
                                   const ThistleOpts *opts, double *result_out);
 
+/* Adjoint of thistle_total_from_twordm.  No counterpart in the reference,
+ * which has no derivatives.  Given dL_dS[nq] = dL/dS(q), returns S(q)
+ * (S_out, optional), dL/d(nuclear coordinates) (dL_dR, (natom,3)
+ * column-major like geom, optional) and dL/d(2-RDM values) (dL_dtotal[nmat],
+ * optional).  Every shell centre must coincide with a row of geom; the MO
+ * coefficients are held fixed.  The forward is recomputed; Hermite tables
+ * are rebuilt with their derivatives rather than stored per quartet. */
+int32_t thistle_total_scattering_adjoint(const ThistleBasis *basis,
+                                         int64_t nmat, const int32_t *mat,
+                                         const double *total, int32_t natom,
+                                         const double *geom,
+                                         const double *dL_dS,
+                                         const ThistleOpts *opts,
+                                         double *S_out, double *dL_dR,
+                                         double *dL_dtotal);
+
 /* Elastic j0 from a given 1-RDM (norb x norb, column-major): the body of
  * case 2 after onerdm_creat (src/main_calculation.f90:137-152). */
 int32_t thistle_elastic_from_onerdm(const ThistleBasis *basis, int32_t norb,

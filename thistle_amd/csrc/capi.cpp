@@ -433,7 +433,7 @@ int run_transform_total(ThistlePlan* pl) {
     return THISTLE_OK;
 }
 
-int run_series(ThistlePlan* pl, double* result_host) {
+SeriesArgs series_args(const ThistlePlan* pl) {
     const bool elastic = pl->type == 2;
     SeriesArgs a{};
     a.qlist = pl->qlist.as<uint64_t>();
@@ -461,6 +461,11 @@ int run_series(ThistlePlan* pl, double* result_host) {
     a.comboff = pl->comboff.as<int64_t>();
     a.nsig = pl->nsig;
     a.nqc = std::min(512, pl->nq);
+    return a;
+}
+
+int run_series(ThistlePlan* pl, double* result_host) {
+    SeriesArgs a = series_args(pl);
     // h(beta) of every quartet (q independent), heaviest classes first
     for (int LL = 16; LL >= 0; --LL) {
         int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
@@ -565,9 +570,181 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
     return THISTLE_OK;
 }
 
+// Adjoint of one executed type-1 plan: w = dL/dS(q) -> dL/d(shell centre)
+// [ng][3] (gg_host, may be null) and dL/dtotal [nmat] (dtot_host, may be null).
+int run_adjoint(ThistlePlan* pl, const double* w_host, double* gg_host, double* dtot_host) {
+    HCHECK(hipSetDevice(pl->device));
+    const int L1 = pl->maxl + 1, T1 = 2 * pl->maxl + 1;
+    DevBuf dE, w, gadj, Wz, qgrad, goff, gent, gg, dtot, scratch, Wg;
+    int rc;
+    if ((rc = dalloc(dE, sizeof(double) * (size_t)pl->npair * 3 * L1 * L1 * T1))) return rc;
+    if ((rc = dalloc(w, sizeof(double) * pl->nq))) return rc;
+    if ((rc = dalloc(gadj, sizeof(double) * 18 * (size_t)pl->nquart))) return rc;
+    if ((rc = dalloc(qgrad, sizeof(double) * 12 * (size_t)pl->nquart))) return rc;
+    if ((rc = dalloc(Wz, sizeof(double) * (size_t)pl->npp * pl->npp))) return rc;
+    HCHECK(hipMemcpyAsync(w.p, w_host, sizeof(double) * pl->nq, hipMemcpyHostToDevice,
+                          pl->stream));
+    HCHECK(hipMemsetAsync(Wz.p, 0, Wz.bytes, pl->stream));
+    HCHECK(launch_pair_dE(pl->G.as<GroupInfo>(), pl->P.as<PairInfo>(), pl->npair, pl->maxl,
+                          dE.as<double>(), pl->stream));
+    AdjointArgs j{};
+    j.s = series_args(pl);
+    j.dE = dE.as<double>();
+    j.w = w.as<double>();
+    j.gadj = gadj.as<double>();
+    j.Wz = Wz.as<double>();
+    j.ldw = pl->npp;
+    j.qgrad = qgrad.as<double>();
+    for (int LL = 16; LL >= 0; --LL) {
+        int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
+        HCHECK(launch_series_adj(j, LL, pl->cls.as<int32_t>() + pl->cls_off[LL], n, pl->waves,
+                                 pl->stream));
+    }
+    // px tables and their derivatives: LDS up to 64 KB per workgroup, else a
+    // global slice per quartet group
+    const size_t lds_cap = 65536;
+    size_t scratch_doubles = 0;
+    auto shape = [&](int LL, int64_t n, bool& glob, int& blocks) {
+        const int NG = contract_adj_groups(LL);
+        const size_t per = contract_adj_table_doubles(LL, pl->cls_tmax[LL]) * NG;
+        glob = per * sizeof(double) > lds_cap;
+        blocks = (int)std::min<int64_t>(glob ? 1024 : (LL < 10 ? 16384 : 2048), (n + NG - 1) / NG);
+        return per;
+    };
+    for (int LL = 0; LL <= 16; ++LL) {
+        int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
+        if (n <= 0) continue;
+        bool glob;
+        int blocks;
+        size_t per = shape(LL, n, glob, blocks);
+        if (glob) scratch_doubles = std::max(scratch_doubles, per * blocks);
+    }
+    if (scratch_doubles) {
+        if ((rc = dalloc(scratch, sizeof(double) * scratch_doubles))) return rc;
+        j.scratch = scratch.as<double>();
+    }
+    for (int LL = 16; LL >= 0; --LL) {
+        int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
+        if (n <= 0) continue;
+        bool glob;
+        int blocks;
+        shape(LL, n, glob, blocks);
+        HCHECK(launch_contract_adj(j, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
+                                   n, blocks, glob, lds_cap, pl->stream));
+    }
+    if (gg_host) {
+        // quartet -> shell entries (e*4 + slot), grouped by shell, in list order
+        int64_t Q = 0;
+        std::vector<EnumRow> rows = enum_rows(pl->ng, Q);
+        std::vector<int64_t> cnt(pl->ng + 1, 0);
+        auto quad = [](const EnumRow& R, int x, int g[4]) {
+            switch (R.nest) {
+                case 1: g[0] = R.i; g[1] = R.j; g[2] = R.k; g[3] = x; break;
+                case 2: g[0] = R.i; g[1] = R.j; g[2] = R.i; g[3] = x; break;
+                case 3: g[0] = R.i; g[1] = R.i; g[2] = R.k; g[3] = x; break;
+                case 4: g[0] = R.i; g[1] = R.i; g[2] = x; g[3] = x; break;
+                default: g[0] = g[1] = g[2] = g[3] = x; break;
+            }
+        };
+        for (const auto& R : rows)
+            for (int u = 0; u < R.count; ++u) {
+                int g[4];
+                quad(R, R.r0 + u, g);
+                for (int k = 0; k < 4; ++k) cnt[g[k] + 1]++;
+            }
+        for (int g = 0; g < pl->ng; ++g) cnt[g + 1] += cnt[g];
+        std::vector<int64_t> ent(4 * (size_t)Q), fill(cnt.begin(), cnt.end() - 1);
+        for (const auto& R : rows)
+            for (int u = 0; u < R.count; ++u) {
+                int g[4];
+                quad(R, R.r0 + u, g);
+                for (int k = 0; k < 4; ++k) ent[fill[g[k]]++] = (R.out + u) * 4 + k;
+            }
+        if ((rc = dalloc(goff, sizeof(int64_t) * cnt.size()))) return rc;
+        if ((rc = dalloc(gent, sizeof(int64_t) * ent.size()))) return rc;
+        if ((rc = dalloc(gg, sizeof(double) * 3 * pl->ng))) return rc;
+        HCHECK(hipMemcpyAsync(goff.p, cnt.data(), sizeof(int64_t) * cnt.size(),
+                              hipMemcpyHostToDevice, pl->stream));
+        HCHECK(hipMemcpyAsync(gent.p, ent.data(), sizeof(int64_t) * ent.size(),
+                              hipMemcpyHostToDevice, pl->stream));
+        HCHECK(launch_group_grad(qgrad.as<double>(), goff.as<int64_t>(), gent.as<int64_t>(),
+                                 pl->ng, gg.as<double>(), pl->stream));
+        HCHECK(hipMemcpyAsync(gg_host, gg.p, sizeof(double) * 3 * pl->ng, hipMemcpyDeviceToHost,
+                              pl->stream));
+        // the host vectors above must outlive the copies
+        HCHECK(hipStreamSynchronize(pl->stream));
+    }
+    if (dtot_host && pl->nmat > 0) {
+        const int npmo = pl->npmo;
+        const double one = 1.0, zero = 0.0;
+        if ((rc = dalloc(Wg, sizeof(double) * (size_t)npmo * npmo))) return rc;
+        if ((rc = dalloc(dtot, sizeof(double) * pl->nmat))) return rc;
+        // Y = T Wz (Wz stored transposed: element (xk, xb) holds dL/dZ(xb, xk))
+        BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_none, rocblas_operation_transpose, npmo,
+                             (rocblas_int)pl->npp, (rocblas_int)pl->npp, &one, pl->T.as<double>(),
+                             npmo, Wz.as<double>(), (rocblas_int)pl->npp, &zero,
+                             pl->M.as<double>(), npmo));
+        // W = Y T^T  (npmo x npmo) = dL/dGs
+        BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_none, rocblas_operation_transpose, npmo,
+                             npmo, (rocblas_int)pl->npp, &one, pl->M.as<double>(), npmo,
+                             pl->T.as<double>(), npmo, &zero, Wg.as<double>(), npmo));
+        HCHECK(launch_dtotal(pl->mat.as<int32_t>(), pl->nmat, pl->norb, Wg.as<double>(), npmo,
+                             dtot.as<double>(), pl->stream));
+        HCHECK(hipMemcpyAsync(dtot_host, dtot.p, sizeof(double) * pl->nmat,
+                              hipMemcpyDeviceToHost, pl->stream));
+    }
+    HCHECK(hipStreamSynchronize(pl->stream));
+    return THISTLE_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int32_t thistle_total_scattering_adjoint(const ThistleBasis* basis, int64_t nmat,
+                                         const int32_t* mat, const double* total, int32_t natom,
+                                         const double* geom, const double* dL_dS,
+                                         const ThistleOpts* opts, double* S_out, double* dL_dR,
+                                         double* dL_dtotal) {
+    if (!basis || !dL_dS) return THISTLE_EINVAL;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    // shell -> atom by centre (host, before any device call)
+    std::vector<int> gatom;
+    if (dL_dR) {
+        if (natom <= 0 || !geom) return THISTLE_EINVAL;
+        std::vector<GroupInfo> groups;
+        int maxl = 0;
+        int rc = check_basis(basis, groups, maxl);
+        if (rc) return rc;
+        gatom.assign(basis->ng, -1);
+        for (int g = 0; g < basis->ng; ++g) {
+            const double c[3] = {groups[g].x, groups[g].y, groups[g].z};
+            for (int a = 0; a < natom && gatom[g] < 0; ++a) {
+                bool same = true;
+                for (int k = 0; k < 3; ++k)
+                    same = same && std::fabs(geom[a + (int64_t)natom * k] - c[k]) <=
+                                       1e-10 * (1.0 + std::fabs(c[k]));
+                if (same) gatom[g] = a;
+            }
+            if (gatom[g] < 0) return THISTLE_EINVAL;
+        }
+    }
+    ThistlePlan* raw = nullptr;
+    int rc = thistle_plan_create(1, basis, nmat, mat, total, 0, nullptr, o, &raw);
+    if (rc) return rc;
+    std::unique_ptr<ThistlePlan> pl(raw);
+    rc = thistle_plan_execute(pl.get(), S_out);
+    if (rc) return rc;
+    std::vector<double> gg(dL_dR ? 3 * (size_t)basis->ng : 0);
+    rc = run_adjoint(pl.get(), dL_dS, dL_dR ? gg.data() : nullptr, dL_dtotal);
+    if (rc) return rc;
+    if (dL_dR) {
+        for (int64_t k = 0; k < (int64_t)natom * 3; ++k) dL_dR[k] = 0.0;
+        for (int g = 0; g < basis->ng; ++g)
+            for (int k = 0; k < 3; ++k) dL_dR[gatom[g] + (int64_t)natom * k] += gg[3 * g + k];
+    }
+    return THISTLE_OK;
+}
 
 const char* thistle_version(void) { return "thistle-mi355x 0.1 (gfx950)"; }
 

@@ -30,142 +30,12 @@
 // is a gather plus a triple convolution.  a(l,p) has the closed form
 //   a(l,p) = (-1)^((l+p)/2) l! / (k! p! 2^k) x^p,  k = (l-p)/2,
 // which is what rrdj0's recurrence (src/Integrals.f90:7062-7091) produces.
-#include "thistle_internal.hpp"
-
-#include <math.h>
+#include "series_math.hpp"
 
 #include <algorithm>
 
-#define WAVE 64
-
-#define HS 17       // stride of the h(beta) buffer (LL <= 16)
-#define XDPH 74.0   // q h from which dphrec's start-100 recurrence is followed
-                    // step by step (it departs from the exact j_beta near 76)
-
-namespace {
-
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
-}
-
-__device__ __forceinline__ void kahan(double& s, double& c, double v) {
-#pragma clang fp contract(off)
-    double y = v - c;
-    double t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
-
-__device__ __forceinline__ double rcp64(double x) {
-    // hardware reciprocal estimate refined by two Newton steps
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(fma(-x, r, 1.0), r, r);
-    r = fma(fma(-x, r, 1.0), r, r);
-    return r;
-}
-
-// Cody-Waite reduction by pi/2 split in three parts (33, 33 and 53 bits),
-// then Taylor polynomials on |r| <= pi/4.  Valid for 0 <= x < 2^20.
-__device__ __forceinline__ void sincos_cw(double x, double& s, double& c) {
-    const double C1 = 0x1.921fb54400000p+0, C2 = 0x1.0b4611a600000p-34,
-                 C3 = 0x1.3198a2e037073p-69;
-    double kf = rint(x * 0.6366197723675814);
-    double r = fma(-kf, C1, x);
-    r = fma(-kf, C2, r);
-    r = fma(-kf, C3, r);
-    double r2 = r * r;
-    // sin r = r (1 - r^2/3! + ... - r^16/17!)
-    double ps = -1.0 / 355687428096000.0;   // -1/17!
-    ps = fma(ps, r2, 1.0 / 1307674368000.0);
-    ps = fma(ps, r2, -1.0 / 6227020800.0);
-    ps = fma(ps, r2, 1.0 / 39916800.0);
-    ps = fma(ps, r2, -1.0 / 362880.0);
-    ps = fma(ps, r2, 1.0 / 5040.0);
-    ps = fma(ps, r2, -1.0 / 120.0);
-    ps = fma(ps, r2, 1.0 / 6.0);
-    double sr = fma(-r * r2, ps, r);
-    // cos r = 1 - r^2/2! + ... + r^20/20!
-    double pc = 1.0 / 2432902008176640000.0;  // 1/20!
-    pc = fma(pc, r2, -1.0 / 6402373705728000.0);
-    pc = fma(pc, r2, 1.0 / 20922789888000.0);
-    pc = fma(pc, r2, -1.0 / 87178291200.0);
-    pc = fma(pc, r2, 1.0 / 479001600.0);
-    pc = fma(pc, r2, -1.0 / 3628800.0);
-    pc = fma(pc, r2, 1.0 / 40320.0);
-    pc = fma(pc, r2, -1.0 / 720.0);
-    pc = fma(pc, r2, 1.0 / 24.0);
-    pc = fma(pc, r2, -0.5);
-    double cr = fma(pc, r2, 1.0);
-    int k = (int)kf & 3;
-    double ss = (k & 1) ? cr : sr;
-    double cc = (k & 1) ? sr : cr;
-    s = (k & 2) ? -ss : ss;
-    c = ((k + 1) & 2) ? -cc : cc;
-}
-
-// FP32 polynomial after an FP64 reduction (mixed precision).
-__device__ __forceinline__ void sincos_mixed(double x, float& s, float& c) {
-    const double C1 = 0x1.921fb54400000p+0, C2 = 0x1.0b4611a600000p-34,
-                 C3 = 0x1.3198a2e037073p-69;
-    double kf = rint(x * 0.6366197723675814);
-    double rd = fma(-kf, C1, x);
-    rd = fma(-kf, C2, rd);
-    rd = fma(-kf, C3, rd);
-    float r = (float)rd, r2 = r * r;
-    float ps = 1.0f / 362880.0f;
-    ps = fmaf(ps, r2, -1.0f / 5040.0f);
-    ps = fmaf(ps, r2, 1.0f / 120.0f);
-    ps = fmaf(ps, r2, -1.0f / 6.0f);
-    float sr = fmaf(r * r2, ps, r);
-    float pc = -1.0f / 3628800.0f;
-    pc = fmaf(pc, r2, 1.0f / 40320.0f);
-    pc = fmaf(pc, r2, -1.0f / 720.0f);
-    pc = fmaf(pc, r2, 1.0f / 24.0f);
-    pc = fmaf(pc, r2, -0.5f);
-    float cr = fmaf(pc, r2, 1.0f);
-    int k = (int)kf & 3;
-    float ss = (k & 1) ? cr : sr;
-    float cc = (k & 1) ? sr : cr;
-    s = (k & 2) ? -ss : ss;
-    c = ((k + 1) & 2) ? -cc : cc;
-}
-
-// a(l,p) of rrdj0 in closed form: coefficient table built at compile time
-struct ACoef {
-    double v[17][17];
-};
-constexpr ACoef make_acoef() {
-    ACoef t{};
-    for (int l = 0; l <= 16; ++l)
-        for (int p = 0; p <= l; ++p) {
-            if ((l - p) & 1) continue;
-            int k = (l - p) / 2;
-            double c = 1.0;
-            for (int i = 2; i <= l; ++i) c *= i;
-            for (int i = 2; i <= k; ++i) c /= i;
-            for (int i = 2; i <= p; ++i) c /= i;
-            for (int i = 0; i < k; ++i) c *= 0.5;
-            t.v[l][p] = (((l + p) / 2) & 1) ? -c : c;
-        }
-    return t;
-}
-__constant__ ACoef kAcoef = make_acoef();
-
-}  // namespace
 
 // ------------------------------------------------------------ contraction
-// G lanes per quartet: small classes (a few GTO quadruples and exponent
-// tuples per quartet) share one wave between 64/G quartets; from LL = 10 on
-// a quartet has thousands of quadruples and takes a 256-thread workgroup.
-template <int LL>
-struct ContractShape {
-    static constexpr int CT = LL < 10 ? 64 : 256;
-    static constexpr int G = LL <= 1 ? 4 : LL == 2 ? 8 : LL == 3 ? 16 : LL == 4 ? 32 : CT;
-    static constexpr int NG = CT / G;
-};
-
 template <int LL>
 __global__ void __launch_bounds__(ContractShape<LL>::CT)
     k_contract(SeriesArgs A, const int32_t* __restrict__ idx, int64_t nidx, int tmax) {
@@ -394,17 +264,6 @@ __device__ __noinline__ double f_dphrec(const double (&hs)[LL + 1], double q, do
     }
     return F;
 }
-
-// Upward recurrence from j0, j1 keeps every j_b (b <= LL) within 1e-14 of
-// its envelope (|j_b| below the first zero, the spherical Hankel modulus
-// above it) for x >= 0.9 LL + 1; below that Miller's backward recurrence is
-// used.
-template <int LL>
-struct SeriesRegime {
-    static constexpr double xup = LL == 0 ? 0.0 : 0.9 * LL + 1.0;
-    // backward start: at least 16 orders above max(LL, x) for x < xup
-    static constexpr int nstart = LL + 16 + (int)(0.9 * LL + 2.0);
-};
 
 // Miller backward recurrence for g_b = j_b(x)/x^b, normalised by
 // g_0 = sin(x)/x:  F = sum_b h(b) q^(2b) g_b.  FP64 (small arguments, where

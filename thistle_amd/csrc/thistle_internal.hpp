@@ -77,6 +77,31 @@ struct SeriesArgs {
     int nsig;
 };
 
+// Adjoint of the total j0 path (adjoint.hip)
+struct AdjointArgs {
+    SeriesArgs s;             // the forward's tables (Z, hs, flags, ...)
+    const double* dE;         // dE/dX_AB, same layout as SeriesArgs::E
+    const double* w;          // [nq] dL/dS(q)
+    double* gadj;             // [nquart][18] G_beta, beta <= LL+1
+    double* Wz;               // [npp][npp] dL/dZ (column-major, all reads)
+    int64_t ldw;
+    double* qgrad;            // [nquart][12] dL/d(centre) of the four shells
+    double* scratch;          // global px tables when they exceed LDS
+};
+hipError_t launch_pair_dE(const GroupInfo* G, const PairInfo* P, int npair, int maxl,
+                          double* dE, hipStream_t s);
+hipError_t launch_series_adj(const AdjointArgs& j, int LL, const int32_t* idx,
+                             int64_t nidx, int waves, hipStream_t s);
+size_t contract_adj_table_doubles(int LL, int Tmax);
+int contract_adj_groups(int LL);
+hipError_t launch_contract_adj(const AdjointArgs& j, int LL, int Tmax, const int32_t* idx,
+                               int64_t nidx, int blocks, bool glob, size_t lds_limit,
+                               hipStream_t s);
+hipError_t launch_group_grad(const double* qgrad, const int64_t* goff,
+                             const int64_t* gent, int ng, double* out, hipStream_t s);
+hipError_t launch_dtotal(const int32_t* mat, int64_t nmat, int norb, const double* W,
+                         int npmo, double* dtot, hipStream_t s);
+
 hipError_t launch_pair_info(const GroupInfo* G, PairInfo* P, int npair,
                             hipStream_t s);
 hipError_t launch_pair_E(const GroupInfo* G, const PairInfo* P, int npair,

@@ -22,6 +22,7 @@ FP64, MIXED = 0, 1
 
 EXPORTS = [
     "thistle_total_scattering_calculation", "thistle_total_from_twordm",
+    "thistle_total_scattering_adjoint",
     "thistle_elastic_from_onerdm", "thistle_twordm_dense", "thistle_twordm_select",
     "thistle_onerdm", "thistle_quartet_list", "thistle_quartet_list_screened",
     "thistle_unique_total", "thistle_pair_tables", "thistle_plan_create",
@@ -75,6 +76,8 @@ def _declare(L):
     L.thistle_total_scattering_calculation.restype = i32
     L.thistle_total_from_twordm.argtypes = [p, i64, p, p, p, p]
     L.thistle_total_from_twordm.restype = i32
+    L.thistle_total_scattering_adjoint.argtypes = [p, i64, p, p, i32, p, p, p, p, p, p]
+    L.thistle_total_scattering_adjoint.restype = i32
     L.thistle_elastic_from_onerdm.argtypes = [p, i32, p, p, p]
     L.thistle_elastic_from_onerdm.restype = i32
     L.thistle_twordm_dense.argtypes = [p, i32, i32, p, i32, i32, i32, p]

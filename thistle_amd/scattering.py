@@ -74,6 +74,41 @@ def total_from_twordm(ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total, cutof
     return res
 
 
+def total_adjoint(ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total, dL_dS, geom=None,
+                  cutoffz=1e-15, cutoffmd=1e-20, cutoffcentre=0.001, *, want_geom=True,
+                  want_total=True, maxl=None):
+    """S(q), dL/dR and dL/d(2-RDM values) for a given dL/dS(q).
+
+    No reference counterpart (the reference has no derivatives).  ``geom``
+    (natom, 3) in bohr must contain every shell centre; the MO coefficients
+    are held fixed.  Returns ``(S, dL_dR or None, dL_dtotal or None)``.
+    """
+    lib = L.load()
+    b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
+               maxl=maxl)
+    o = L.make_opts(L.FP64)
+    mat = np.require(np.asarray(mat), dtype=np.int32, requirements=["F"])
+    total = np.require(np.asarray(total, dtype=np.float64), requirements=["C"])
+    w = np.require(np.asarray(dL_dS, dtype=np.float64), requirements=["C"])
+    if w.size != np.asarray(q).size:
+        raise ValueError("dL_dS must have one entry per q point")
+    S = np.zeros(w.size)
+    dR = gptr = None
+    natom = 0
+    if want_geom:
+        if geom is None:
+            raise ValueError("geom is required for the coordinate gradient")
+        gptr = np.require(np.asarray(geom, dtype=np.float64).reshape(-1, 3), requirements=["F"])
+        natom = gptr.shape[0]
+        dR = np.zeros((natom, 3), order="F")
+    dT = np.zeros(total.size) if want_total else None
+    rc = lib.thistle_total_scattering_adjoint(C.byref(b), int(total.size), L._ptr(mat),
+                                              L._ptr(total), natom, L._ptr(gptr), L._ptr(w),
+                                              C.byref(o), L._ptr(S), L._ptr(dR), L._ptr(dT))
+    L.check(rc, "total_scattering_adjoint")
+    return S, (np.ascontiguousarray(dR) if dR is not None else None), dT
+
+
 def elastic_from_onerdm(ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm, cutoffz=1e-15,
                         cutoffmd=1e-20, cutoffcentre=0.001, *, precision=L.FP64, waves=0,
                         maxl=None):
