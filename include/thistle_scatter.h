@@ -193,6 +193,26 @@ int32_t thistle_plan_timings(ThistlePlan *plan, double *ms5, int64_t *nquart,
                              int64_t *units);
 void thistle_plan_destroy(ThistlePlan *plan);
 
+/* Batched frames (total j0).  No counterpart in the reference, which runs
+ * one geometry per call.  nframes independent problems, frame f given by
+ * bases[f] and its 2-RDM rows mats[f] (nmat[f] x 4, column-major) with
+ * values totals[f].  The first execution tunes the series kernel's waves
+ * per compute unit on eager runs and captures every frame's stages into
+ * one hipGraph, which later executions replay.  S_out receives the frames'
+ * S(q) concatenated in frame order (sum of bases[f].nq doubles). */
+typedef struct ThistleBatch ThistleBatch;
+int32_t thistle_batch_create(int32_t nframes, const ThistleBasis *bases,
+                             const int64_t *nmat, const int32_t *const *mats,
+                             const double *const *totals,
+                             const ThistleOpts *opts, ThistleBatch **batch);
+int32_t thistle_batch_execute(ThistleBatch *batch, double *S_out);
+/* Device time of the last execution (ms), quartet x q-point units of the
+ * batch, the series kernel's wave count per launch and whether a graph is
+ * replayed (1) or the stages are enqueued eagerly (0). */
+int32_t thistle_batch_info(ThistleBatch *batch, double *ms_last,
+                           int64_t *units, int32_t *waves, int32_t *graph);
+void thistle_batch_destroy(ThistleBatch *batch);
+
 /* Number of HIP devices visible to the library (negative on error), and a
  * device-wide synchronize of the given ordinal. */
 int32_t thistle_device_count(void);

@@ -76,7 +76,7 @@ const ThistleOpts kDefaultOpts = {THISTLE_FP64, 1e-30, nullptr, 0, 0};
 struct ThistlePlan {
     int type = 1, precision = 0;
     int ngto = 0, ng = 0, maxl = 0, nq = 0, nmo = 0, norb = 0;
-    int npair = 0, npmo = 0, waves = 0, nrows = 0;
+    int npair = 0, npmo = 0, waves = 0, waves_alloc = 0, nrows = 0;
     int64_t npp = 0, nquart = 0, nmat = 0, units = 0;
     double cutoffz = 0, cutoffmd = 0, cutoffcentre = 0;
     hipStream_t stream = nullptr;
@@ -375,6 +375,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
         waves = prop.multiProcessorCount * 4;
     }
     pl->waves = (int)std::max<int64_t>(1, std::min<int64_t>(waves, Q));
+    pl->waves_alloc = pl->waves;
     int nqc = std::min(b->nq, 512);
     if ((rc = dalloc(pl->part, sizeof(double) * (size_t)pl->waves * nqc))) return rc;
     if ((rc = dalloc(pl->partc, sizeof(double) * (size_t)pl->waves * nqc))) return rc;
@@ -388,16 +389,16 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     return THISTLE_OK;
 }
 
-int run_tables_and_enum(ThistlePlan* pl) {
+int run_tables_and_enum(ThistlePlan* pl, bool timing) {
     HCHECK(launch_pair_info(pl->G.as<GroupInfo>(), pl->P.as<PairInfo>(), pl->npair, pl->stream));
     HCHECK(launch_pair_E(pl->G.as<GroupInfo>(), pl->P.as<PairInfo>(), pl->npair, pl->maxl,
                          pl->E.as<double>(), pl->stream));
     HCHECK(launch_e12(pl->P.as<PairInfo>(), pl->npair, pl->q.as<double>(), pl->nq,
                       pl->e12.as<double>(), nullptr, pl->stream));
-    HCHECK(hipEventRecord(pl->ev[1], pl->stream));
+    if (timing) HCHECK(hipEventRecord(pl->ev[1], pl->stream));
     HCHECK(launch_enumerate(pl->rows.as<EnumRow>(), pl->nrows, pl->P.as<PairInfo>(), pl->ng,
                             pl->cutoffcentre, pl->qlist.as<uint64_t>(), pl->stream));
-    HCHECK(hipEventRecord(pl->ev[2], pl->stream));
+    if (timing) HCHECK(hipEventRecord(pl->ev[2], pl->stream));
     return THISTLE_OK;
 }
 
@@ -409,7 +410,8 @@ int run_transform_total(ThistlePlan* pl) {
     HCHECK(launch_unique(pl->mat.as<int32_t>(), pl->nmat, 4, pl->total.as<double>(), w,
                          pl->rmat.as<int32_t>(), pl->rtot.as<double>(), nullptr, nullptr,
                          pl->count.as<int64_t>() + 1, pl->stream));
-    HCHECK(hipMemsetAsync(pl->V.p, 0, pl->V.bytes, pl->stream));
+    HCHECK(launch_fill(pl->V.as<double>(), (int64_t)(pl->V.bytes / sizeof(double)), 0.0,
+                       pl->stream));
     // only the unique prefix of rmat/rtot is valid; its length stays on the
     // device and guards the scatter
     HCHECK(launch_gs(pl->rmat.as<int32_t>(), pl->rtot.as<double>(), pl->nmat,
@@ -464,7 +466,7 @@ SeriesArgs series_args(const ThistlePlan* pl) {
     return a;
 }
 
-int run_series(ThistlePlan* pl, double* result_host) {
+int run_series(ThistlePlan* pl, bool timing) {
     SeriesArgs a = series_args(pl);
     // h(beta) of every quartet (q independent), heaviest classes first
     for (int LL = 16; LL >= 0; --LL) {
@@ -477,8 +479,9 @@ int run_series(ThistlePlan* pl, double* result_host) {
     for (int q0 = 0; q0 < pl->nq; q0 += 512) {
         a.q0 = q0;
         a.nqc = std::min(512, pl->nq - q0);
-        HCHECK(hipMemsetAsync(pl->part.p, 0, pl->part.bytes, pl->stream));
-        HCHECK(hipMemsetAsync(pl->partc.p, 0, pl->partc.bytes, pl->stream));
+        const int64_t rows = (int64_t)pl->waves * a.nqc;
+        HCHECK(launch_fill(pl->part.as<double>(), rows, 0.0, pl->stream));
+        HCHECK(launch_fill(pl->partc.as<double>(), rows, 0.0, pl->stream));
         for (int LL = 16; LL >= 0; --LL) {
             int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
             if (n <= 0) continue;
@@ -489,16 +492,37 @@ int run_series(ThistlePlan* pl, double* result_host) {
         HCHECK(launch_reduce(a.part, a.partc, pl->waves, a.nqc, pl->S.as<double>() + q0,
                              pl->stream));
     }
-    HCHECK(hipEventRecord(pl->ev[4], pl->stream));
-    HCHECK(launch_compact(pl->flags.as<uint8_t>(), pl->qlist.as<uint64_t>(), pl->nquart,
-                          pl->scan.as<int64_t>(), pl->qcomp.as<uint64_t>(), pl->count.as<int64_t>(),
-                          pl->stream));
-    HCHECK(hipEventRecord(pl->ev[5], pl->stream));
-    if (result_host)
-        HCHECK(hipMemcpyAsync(result_host, pl->S.p, sizeof(double) * pl->nq, hipMemcpyDeviceToHost,
-                              pl->stream));
-    HCHECK(hipStreamSynchronize(pl->stream));
+    if (timing) {
+        // the screened quartet list (thistle_quartet_list_screened) is a
+        // by-product of single-plan executions
+        HCHECK(hipEventRecord(pl->ev[4], pl->stream));
+        HCHECK(launch_compact(pl->flags.as<uint8_t>(), pl->qlist.as<uint64_t>(), pl->nquart,
+                              pl->scan.as<int64_t>(), pl->qcomp.as<uint64_t>(),
+                              pl->count.as<int64_t>(), pl->stream));
+        HCHECK(hipEventRecord(pl->ev[5], pl->stream));
+    }
     pl->units = pl->nquart * (int64_t)pl->nq;
+    return THISTLE_OK;
+}
+
+// Every device stage of one execution, enqueued on the plan's stream with no
+// host synchronisation, memset or (unless timing) event: safe under stream
+// capture.  S_dst (device, optional) receives a copy of S(q).
+int enqueue_execute(ThistlePlan* pl, bool timing, double* S_dst) {
+    if (timing) HCHECK(hipEventRecord(pl->ev[0], pl->stream));
+    int rc = run_tables_and_enum(pl, timing);
+    if (rc) return rc;
+    if (pl->type == 1) {
+        rc = run_transform_total(pl);
+        if (rc) return rc;
+    } else {
+        HCHECK(launch_zel(pl->mmod.as<double>(), pl->nmo, pl->norb, pl->onerdm.as<double>(),
+                          pl->ngto, pl->Z.as<double>(), pl->stream));
+    }
+    if (timing) HCHECK(hipEventRecord(pl->ev[3], pl->stream));
+    rc = run_series(pl, timing);
+    if (rc) return rc;
+    if (S_dst) HCHECK(launch_copy(pl->S.as<double>(), S_dst, pl->nq, pl->stream));
     return THISTLE_OK;
 }
 
@@ -699,7 +723,185 @@ int run_adjoint(ThistlePlan* pl, const double* w_host, double* gg_host, double* 
 
 }  // namespace
 
+// Many independent frames executed together: per-frame plans spread over a
+// few streams (fork/join through events), captured once into one hipGraph
+// and replayed.  The first execution tunes the series kernel's waves per CU.
+struct ThistleBatch {
+    static constexpr int NS = 4;
+    int device = 0, cus = 0, waves_per_cu = 0;
+    hipStream_t streams[NS] = {};
+    hipEvent_t fork = nullptr, join[NS] = {}, t0 = nullptr, t1 = nullptr;
+    std::vector<std::unique_ptr<ThistlePlan>> plans;
+    std::vector<int64_t> soff;
+    DevBuf S;
+    int64_t nqtot = 0, units = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool tuned = false, graph_ok = false;
+    float ms = 0.f;
+    ~ThistleBatch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        plans.clear();  // plans use the batch streams
+        for (auto& e : join)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        for (auto& st : streams)
+            if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+void batch_set_waves(ThistleBatch* b, int per_cu) {
+    b->waves_per_cu = per_cu;
+    for (auto& p : b->plans)
+        p->waves = (int)std::max<int64_t>(
+            1, std::min<int64_t>({(int64_t)per_cu * b->cus, p->waves_alloc, p->nquart}));
+}
+
+int batch_enqueue(ThistleBatch* b) {
+    HCHECK(hipEventRecord(b->fork, b->streams[0]));
+    for (int s = 1; s < ThistleBatch::NS; ++s) HCHECK(hipStreamWaitEvent(b->streams[s], b->fork, 0));
+    for (size_t f = 0; f < b->plans.size(); ++f) {
+        int rc = enqueue_execute(b->plans[f].get(), false, b->S.as<double>() + b->soff[f]);
+        if (rc) return rc;
+    }
+    for (int s = 1; s < ThistleBatch::NS; ++s) {
+        HCHECK(hipEventRecord(b->join[s], b->streams[s]));
+        HCHECK(hipStreamWaitEvent(b->streams[0], b->join[s], 0));
+    }
+    return THISTLE_OK;
+}
+
+int batch_timed(ThistleBatch* b, float& ms) {
+    HCHECK(hipEventRecord(b->t0, b->streams[0]));
+    int rc = batch_enqueue(b);
+    if (rc) return rc;
+    HCHECK(hipEventRecord(b->t1, b->streams[0]));
+    HCHECK(hipStreamSynchronize(b->streams[0]));
+    HCHECK(hipEventElapsedTime(&ms, b->t0, b->t1));
+    return THISTLE_OK;
+}
+
+// autotune on eager executions, then capture the chosen configuration
+int batch_prepare(ThistleBatch* b) {
+    const int cand[3] = {2, 4, 8};
+    float best = 0.f;
+    int pick = cand[0];
+    for (int c : cand) {
+        batch_set_waves(b, c);
+        float ms = 0.f;
+        int rc = batch_timed(b, ms);  // the first candidate also warms up rocBLAS
+        if (rc) return rc;
+        rc = batch_timed(b, ms);
+        if (rc) return rc;
+        if (c == cand[0] || ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    batch_set_waves(b, pick);
+    b->tuned = true;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(b->streams[0], hipStreamCaptureModeGlobal) != hipSuccess)
+        return THISTLE_OK;  // eager fallback
+    int rc = batch_enqueue(b);
+    hipError_t ec = hipStreamEndCapture(b->streams[0], &g);
+    if (rc || ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return THISTLE_OK;
+    }
+    hipGraphExec_t ex = nullptr;
+    if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return THISTLE_OK;
+    }
+    b->graph = g;
+    b->exec = ex;
+    b->graph_ok = true;
+    return THISTLE_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int32_t thistle_batch_create(int32_t nframes, const ThistleBasis* bases, const int64_t* nmat,
+                             const int32_t* const* mats, const double* const* totals,
+                             const ThistleOpts* opts, ThistleBatch** out) {
+    if (!out) return THISTLE_EINVAL;
+    *out = nullptr;
+    if (nframes <= 0 || !bases || !nmat || !mats || !totals) return THISTLE_EINVAL;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    std::unique_ptr<ThistleBatch> b(new ThistleBatch());
+    b->device = o->device;
+    HCHECK(hipSetDevice(o->device));
+    hipDeviceProp_t prop;
+    HCHECK(hipGetDeviceProperties(&prop, o->device));
+    b->cus = prop.multiProcessorCount;
+    for (auto& st : b->streams) HCHECK(hipStreamCreate(&st));
+    HCHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (auto& e : b->join) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HCHECK(hipEventCreate(&b->t0));
+    HCHECK(hipEventCreate(&b->t1));
+    for (int f = 0; f < nframes; ++f) {
+        ThistleOpts of = *o;
+        of.stream = b->streams[f % ThistleBatch::NS];
+        of.waves = 8 * b->cus;
+        ThistlePlan* raw = nullptr;
+        int rc = thistle_plan_create(1, &bases[f], nmat[f], mats[f], totals[f], 0, nullptr, &of,
+                                     &raw);
+        if (rc) return rc;
+        b->plans.emplace_back(raw);
+        b->soff.push_back(b->nqtot);
+        b->nqtot += bases[f].nq;
+        b->units += raw->nquart * (int64_t)raw->nq;
+    }
+    int rc = dalloc(b->S, sizeof(double) * b->nqtot);
+    if (rc) return rc;
+    *out = b.release();
+    return THISTLE_OK;
+}
+
+int32_t thistle_batch_execute(ThistleBatch* b, double* S_out) {
+    if (!b) return THISTLE_EINVAL;
+    HCHECK(hipSetDevice(b->device));
+    if (!b->tuned) {
+        int rc = batch_prepare(b);
+        if (rc) return rc;
+    }
+    HCHECK(hipEventRecord(b->t0, b->streams[0]));
+    if (b->exec) {
+        HCHECK(hipGraphLaunch(b->exec, b->streams[0]));
+    } else {
+        int rc = batch_enqueue(b);
+        if (rc) return rc;
+    }
+    HCHECK(hipEventRecord(b->t1, b->streams[0]));
+    if (S_out)
+        HCHECK(hipMemcpyAsync(S_out, b->S.p, sizeof(double) * b->nqtot, hipMemcpyDeviceToHost,
+                              b->streams[0]));
+    HCHECK(hipStreamSynchronize(b->streams[0]));
+    HCHECK(hipEventElapsedTime(&b->ms, b->t0, b->t1));
+    return THISTLE_OK;
+}
+
+int32_t thistle_batch_info(ThistleBatch* b, double* ms_last, int64_t* units, int32_t* waves,
+                           int32_t* graph) {
+    if (!b) return THISTLE_EINVAL;
+    if (ms_last) *ms_last = b->ms;
+    if (units) *units = b->units;
+    if (waves) *waves = b->waves_per_cu * b->cus;
+    if (graph) *graph = b->graph_ok ? 1 : 0;
+    return THISTLE_OK;
+}
+
+void thistle_batch_destroy(ThistleBatch* b) { delete b; }
 
 int32_t thistle_total_scattering_adjoint(const ThistleBasis* basis, int64_t nmat,
                                          const int32_t* mat, const double* total, int32_t natom,
@@ -772,19 +974,12 @@ int32_t thistle_plan_create(int32_t type, const ThistleBasis* basis, int64_t nma
 int32_t thistle_plan_execute(ThistlePlan* pl, double* result_out) {
     if (!pl) return THISTLE_EINVAL;
     HCHECK(hipSetDevice(pl->device));
-    HCHECK(hipEventRecord(pl->ev[0], pl->stream));
-    int rc = run_tables_and_enum(pl);
+    int rc = enqueue_execute(pl, true, nullptr);
     if (rc) return rc;
-    if (pl->type == 1) {
-        rc = run_transform_total(pl);
-        if (rc) return rc;
-    } else {
-        HCHECK(launch_zel(pl->mmod.as<double>(), pl->nmo, pl->norb, pl->onerdm.as<double>(),
-                          pl->ngto, pl->Z.as<double>(), pl->stream));
-    }
-    HCHECK(hipEventRecord(pl->ev[3], pl->stream));
-    rc = run_series(pl, result_out);
-    if (rc) return rc;
+    if (result_out)
+        HCHECK(hipMemcpyAsync(result_out, pl->S.p, sizeof(double) * pl->nq, hipMemcpyDeviceToHost,
+                              pl->stream));
+    HCHECK(hipStreamSynchronize(pl->stream));
     float t[5];
     for (int k = 0; k < 5; ++k) {
         HCHECK(hipEventElapsedTime(&t[k], pl->ev[k], pl->ev[k + 1]));
@@ -891,7 +1086,7 @@ int64_t thistle_quartet_list(const ThistleBasis* basis, int32_t* out, int64_t ca
     std::unique_ptr<ThistlePlan> pl(new ThistlePlan());
     int rc = setup_common(pl.get(), basis, o);
     if (rc) return -rc;
-    rc = run_tables_and_enum(pl.get());
+    rc = run_tables_and_enum(pl.get(), false);
     if (rc) return -rc;
     if (out && cap >= pl->nquart) {
         DevBuf tmp;
@@ -994,7 +1189,7 @@ int32_t thistle_pair_tables(const ThistleBasis* basis, double* E, double* P, dou
     std::unique_ptr<ThistlePlan> pl(new ThistlePlan());
     int rc = setup_common(pl.get(), basis, o);
     if (rc) return rc;
-    rc = run_tables_and_enum(pl.get());
+    rc = run_tables_and_enum(pl.get(), false);
     if (rc) return rc;
     int L1 = pl->maxl + 1, T1 = 2 * pl->maxl + 1;
     if (E)

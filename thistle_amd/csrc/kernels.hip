@@ -17,6 +17,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 #define WAVE 64
 
 // pi = acos(-1.0000) in single precision, widened (src/Variables.f90:994)
@@ -413,6 +415,17 @@ __global__ void k_zel(const double* __restrict__ mmod, int nmo, int norb,
     Z[idx] = temp / 2.0;
 }
 
+// clears and copies as kernels, so that captured graphs hold no memset nodes
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_copy(const double* __restrict__ in, double* __restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = in[i];
+}
+
 __global__ void k_to_float(const double* __restrict__ in, float* __restrict__ out,
                            int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -545,6 +558,20 @@ hipError_t launch_zel(const double* mmod, int nmo, int norb, const double* onerd
     int64_t n = (int64_t)ngto * ngto;
     hipLaunchKernelGGL(k_zel, dim3(nblk(n, 256)), dim3(256), 0, s, mmod, nmo, norb, onerdm,
                        ngto, Z);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const double* in, double* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, in, out, n);
     return hipGetLastError();
 }
 

@@ -133,6 +133,8 @@ hipError_t launch_gs_matrix(const double* V, int norb, const int2* mopairs,
                             int npmo, double* Gs, hipStream_t s);
 hipError_t launch_zel(const double* mmod, int nmo, int norb,
                       const double* onerdm, int ngto, double* Z, hipStream_t s);
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+hipError_t launch_copy(const double* in, double* out, int64_t n, hipStream_t s);
 hipError_t launch_to_float(const double* in, float* out, int64_t n,
                            hipStream_t s);
 

@@ -27,7 +27,9 @@ EXPORTS = [
     "thistle_onerdm", "thistle_quartet_list", "thistle_quartet_list_screened",
     "thistle_unique_total", "thistle_pair_tables", "thistle_plan_create",
     "thistle_plan_execute", "thistle_plan_result", "thistle_plan_timings",
-    "thistle_plan_destroy", "thistle_device_count", "thistle_device_synchronize",
+    "thistle_plan_destroy", "thistle_batch_create", "thistle_batch_execute",
+    "thistle_batch_info", "thistle_batch_destroy", "thistle_device_count",
+    "thistle_device_synchronize",
     "thistle_version",
 ]
 
@@ -104,6 +106,14 @@ def _declare(L):
     L.thistle_plan_timings.restype = i32
     L.thistle_plan_destroy.argtypes = [p]
     L.thistle_plan_destroy.restype = None
+    L.thistle_batch_create.argtypes = [i32, p, p, p, p, p, C.POINTER(C.c_void_p)]
+    L.thistle_batch_create.restype = i32
+    L.thistle_batch_execute.argtypes = [p, p]
+    L.thistle_batch_execute.restype = i32
+    L.thistle_batch_info.argtypes = [p, p, p, p, p]
+    L.thistle_batch_info.restype = i32
+    L.thistle_batch_destroy.argtypes = [p]
+    L.thistle_batch_destroy.restype = None
     L.thistle_device_count.argtypes = []
     L.thistle_device_count.restype = i32
     L.thistle_device_synchronize.argtypes = [i32]

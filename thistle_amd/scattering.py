@@ -284,3 +284,71 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many total-j0 frames executed together (thistle_batch_*).
+
+    ``frames`` is a sequence of (ga, l, m, n, xx, yy, zz, mmod, q, group, mat,
+    total).  ``execute()`` returns the list of per-frame S(q); the first call
+    tunes the launch and captures the batch into one hipGraph.
+    """
+
+    def __init__(self, frames, precision=L.FP64, cutoffz=1e-15, cutoffmd=1e-20,
+                 cutoffcentre=0.001):
+        lib = L.load()
+        nf = len(frames)
+        bases = (L.Basis * nf)()
+        self._keep = []
+        mats = (C.c_void_p * nf)()
+        tots = (C.c_void_p * nf)()
+        nmat = np.zeros(nf, dtype=np.int64)
+        self.nq = []
+        for f, fr in enumerate(frames):
+            ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total = fr
+            b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre)
+            bases[f] = b
+            self._keep.append(b._keep)
+            mt = np.require(np.asarray(mat), dtype=np.int32, requirements=["F"])
+            tt = np.require(np.asarray(total, dtype=np.float64), requirements=["C"])
+            self._keep += [mt, tt]
+            mats[f] = mt.ctypes.data
+            tots[f] = tt.ctypes.data
+            nmat[f] = tt.size
+            self.nq.append(int(np.asarray(q).size))
+        self._keep += [bases, mats, tots, nmat]
+        o = L.make_opts(precision)
+        self._h = C.c_void_p()
+        L.check(lib.thistle_batch_create(nf, C.cast(bases, C.c_void_p), L._ptr(nmat),
+                                         C.cast(mats, C.c_void_p), C.cast(tots, C.c_void_p),
+                                         C.byref(o), C.byref(self._h)), "batch_create")
+        self._lib = lib
+        self._off = np.concatenate([[0], np.cumsum(self.nq)])
+
+    def execute(self, fetch=True):
+        out = np.zeros(int(self._off[-1])) if fetch else None
+        L.check(self._lib.thistle_batch_execute(self._h, L._ptr(out)), "batch_execute")
+        if not fetch:
+            return None
+        return [out[self._off[f]:self._off[f + 1]].copy() for f in range(len(self.nq))]
+
+    def info(self):
+        ms = C.c_double()
+        units = C.c_int64()
+        waves = C.c_int32()
+        graph = C.c_int32()
+        L.check(self._lib.thistle_batch_info(self._h, C.byref(ms), C.byref(units), C.byref(waves),
+                                             C.byref(graph)), "batch_info")
+        return {"ms": ms.value, "units": units.value, "waves": waves.value,
+                "graph": bool(graph.value)}
+
+    def close(self):
+        if self._h:
+            self._lib.thistle_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

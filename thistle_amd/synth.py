@@ -17,6 +17,8 @@ Workloads (sizes are those of the named benchmark cases):
         (227 functions), 20 core + CAS(10,10), 512 q in [0, 12]
   mol10 10 atoms, min separation 2.0 bohr, 3s1p1d per atom (120 functions),
         CAS(6,6), 256 q in [0, 10]
+  batch 512 frames of 2-12 atoms (2s1p1d per atom), CAS(2,2)-(6,6),
+        64-512 q in [0, 10], coordinates jittered by 0.1 bohr
 """
 from __future__ import annotations
 
@@ -214,6 +216,28 @@ def make_mol10(seed=4, nq=256):
         shells += _shells_for(a, {"s": 3, "p": 1, "d": 1}, rng, 0.08, 40.0)
     Z = rng.choice([1, 6, 7, 8], size=10)
     return _finish("mol10", rng, Z, coords, shells, 0, 6, 6, qgrid(10.0, nq))
+
+
+def make_batch(nframes=512, seed=6, natom_range=(2, 12), nq_range=(64, 512), qmax=10.0,
+               sigma=0.1):
+    """Frames of uneven size for the batched launch: per frame 2-12 atoms
+    (a seeded base geometry per size, jittered by N(0, sigma^2) bohr), per atom
+    2 s + 1 p + 1 d uncontracted cartesian shells, CAS(2,2), CAS(4,4) or
+    CAS(6,6) with a random CI vector, and 64-512 q points in [0, qmax]."""
+    rng = np.random.default_rng(seed)
+    bases = {n: random_molecule(rng, n) for n in range(natom_range[0], natom_range[1] + 1)}
+    frames = []
+    for f in range(nframes):
+        natom = int(rng.integers(natom_range[0], natom_range[1] + 1))
+        coords = bases[natom] + rng.normal(scale=sigma, size=(natom, 3))
+        shells = []
+        for a in range(natom):
+            shells += _shells_for(a, {"s": 2, "p": 1, "d": 1}, rng, 0.1, 30.0)
+        nact = int(rng.choice([2, 4, 6]))
+        nq = int(rng.integers(nq_range[0], nq_range[1] + 1))
+        frames.append(_finish(f"frame{f}", rng, rng.choice([1, 6, 7, 8], size=natom), coords,
+                              shells, 0, nact, nact, qgrid(qmax, nq)))
+    return frames
 
 
 def make_small(seed=5, natom=2, spec=None, nact=2, nelec=2, ncore=0, nq=16,
