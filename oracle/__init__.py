@@ -51,6 +51,9 @@ def _declare(L):
     L.orc_total_j0.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                _i, _p, _p, _i, _p, _i, _d, _d, _d, _i, _i, _p]
     L.orc_total_j0.restype = _i
+    L.orc_total_j2.argtypes = L.orc_total_j0.argtypes
+    L.orc_total_j2.restype = _i
+    L.orc_set_p0_j2.argtypes = [_p, _i, _p, _i]
     L.orc_total_j0_sample.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
                                       _p, _i, _p, _p, _i, _p, _i, _d, _d, _d, _i,
                                       C.c_int64, _p, _p, _p]
@@ -191,6 +194,37 @@ def total_j0(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q,
     if rc:
         raise RuntimeError(f"oracle total_j0 status {rc}")
     return out
+
+
+def total_j2(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q,
+             cutoffz=1e-15, cutoffmd=1e-20, cutoffcentre=0.001, zmode=1,
+             nthreads=0, maxl=None):
+    """tot_integration_j2 (type 7): total j2 S(q)."""
+    l, m, n, group = _I(l), _I(m), _I(n), _I(group)
+    ga, xx, yy, zz, q = _D(ga), _D(xx), _D(yy), _D(zz), _D(q)
+    mmod = _D(mmod, "F")
+    mat = _I(mat, "F")
+    total = _D(total)
+    ngto = len(l)
+    ng = int(group.max())
+    maxl = int(l.max()) if maxl is None else maxl
+    out = np.zeros(len(q))
+    rc = lib().orc_total_j2(ngto, ng, maxl, _ptr(l), _ptr(m), _ptr(n), _ptr(group),
+                            _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(mmod),
+                            mmod.shape[0], _ptr(mat), _ptr(total), len(total),
+                            _ptr(q), len(q), cutoffz, cutoffmd, cutoffcentre,
+                            zmode, nthreads, _ptr(out))
+    if rc:
+        raise RuntimeError(f"oracle total_j2 status {rc}")
+    return out
+
+
+def set_p0_j2(lmax4, q):
+    q = _D(q)
+    D = lmax4 + 1
+    P0 = np.zeros((D, D, D, len(q)))
+    lib().orc_set_p0_j2(_ptr(P0), lmax4, _ptr(q), len(q))
+    return P0
 
 
 def total_j0_sample(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q, stride,

@@ -65,6 +65,34 @@ void orc_set_p0(double *P0, int lmax4, const double *q, int nq)
             for (int i = 0; i <= lmax4; i += 2) p_lmn(P0, D, i, j, k, q, nq);
 }
 
+/* P_LMN_j2, src/Integrals.f90:3137-3170 */
+static void p_lmn_j2(double *P0, int D, int l, int m, int n, const double *q,
+                     int nq)
+{
+    double prod = 1.0;
+    int r = l + m + n;
+    for (int i = 3; i <= l - 1; i += 2) prod = prod * i;
+    for (int i = 3; i <= m - 1; i += 2) prod = prod * i;
+    for (int i = 3; i <= n - 1; i += 2) prod = prod * i;
+    for (int i = 3; i <= r + 3; i += 2) prod = prod / i;
+    double sgn = ((r / 2) & 1) ? -1.0 : 1.0;
+    double *dst = P0 + ((size_t)((l * D + m) * D + n)) * nq;
+    for (int iq = 0; iq < nq; ++iq)
+        dst[iq] = (l + m - 2 * n) / 4.0 * prod * sgn * ipow(q[iq], r);
+}
+
+/* set_P0_j2, src/Integrals.f90:3034-3064 (times sqrt(5/pi), double pi) */
+void orc_set_p0_j2(double *P0, int lmax4, const double *q, int nq)
+{
+    int D = lmax4 + 1;
+    memset(P0, 0, sizeof(double) * (size_t)D * D * D * nq);
+    for (int k = 0; k <= lmax4; k += 2)
+        for (int j = 0; j <= lmax4; j += 2)
+            for (int i = 0; i <= lmax4; i += 2) p_lmn_j2(P0, D, k, j, i, q, nq);
+    const double f = sqrt(5.0 / acos(-1.0));
+    for (size_t t = 0; t < (size_t)D * D * D * nq; ++t) P0[t] = P0[t] * f;
+}
+
 /* ------------------------------------------------------------ MD tables */
 /* One top-level slice E^{l1,l2}_t of fill_md_table (src/Variables.f90:31-944):
  * first raise the "A" index with PA to l1, then the "B" index with PB to l2,
@@ -633,6 +661,57 @@ static void bessel_deriv(double *BD, int LL, int MM, int NN,
     }
 }
 
+/* rrdj2a, src/Integrals.f90:6902-6927 (a[l][c] = a(l+1,c+1); the power p
+ * of x sits in column c = p + 2) */
+static void rrdj2a(int lmax, double x, double a[DIMH][DIMH])
+{
+    memset(a, 0, sizeof(double) * DIMH * DIMH);
+    a[0][2] = 1.0;
+    a[1][3] = -x;
+    for (int l = 2; l <= lmax; ++l) a[l][2] = (double)(-(l - 1)) * a[l - 2][2];
+    for (int l = 2; l <= lmax; ++l)
+        for (int p = 1; p <= l; ++p)
+            a[l][p + 2] = (double)(-(l - 1)) * a[l - 2][p + 2] - x * a[l - 1][p + 1];
+}
+
+/* rrdj2b, src/Integrals.f90:6929-6952 */
+static void rrdj2b(int lmax, double a[DIMH][DIMH], double b[DIMH][DIMH])
+{
+    memset(b, 0, sizeof(double) * DIMH * DIMH);
+    for (int l = 1; l <= lmax; ++l)
+        for (int p = 0; p <= l; ++p) {
+            double carg = (l + p) / 2.0;
+            b[l][p] = 2.0 * ceil(carg) * a[l][p + 2];
+        }
+}
+
+/* BesselDeriv2j, src/Integrals.f90:6370-6435 */
+static void bessel_deriv2j(double *BD, double hz, double h, int LL, int MM, int NN,
+                           double a[DIMH][DIMH], double b[DIMH][DIMH],
+                           double c[DIMH][DIMH], double ap[DIMH][DIMH],
+                           double bp[DIMH][DIMH], double cp[DIMH][DIMH])
+{
+    const double pi = acos(-1.0);
+    const double eta = 3.0 * hz * hz - h * h;
+    for (int i = 0; i < DIMH; ++i) BD[i] = 0.0;
+    for (int ii = 0; ii <= LL + 2; ++ii) {
+        double C1 = a[LL][ii], C1p = ap[LL][ii];
+        for (int jj = 0; jj <= MM + 2; ++jj) {
+            double Ct2 = b[MM][jj], Ct2p = bp[MM][jj];
+            for (int kk = 0; kk <= NN + 2; ++kk) {
+                double Ct3 = c[NN][kk], Ct3p = cp[NN][kk];
+                double C3 = C1 * Ct2 * Ct3 * eta + C1p * Ct2 * Ct3 + C1 * Ct2p * Ct3 +
+                            C1 * Ct2 * Ct3p;
+                int hOrder = (int)ceil((LL + MM + NN + ii + jj + kk) / 2.0 - 1.0);
+                /* hOrder = -1 only when every index is 0, where C3 = 0 */
+                if (hOrder >= 0) BD[hOrder] = BD[hOrder] + C3;
+            }
+        }
+    }
+    const double f = 0.25 * pow(5.0 / pi, 0.5);
+    for (int i = 0; i < DIMH; ++i) BD[i] = f * BD[i];
+}
+
 /* Shared per-quartet state. */
 typedef struct {
     const int *l, *m, *n, *gs, *gc;
@@ -647,22 +726,34 @@ static inline const double *dd_ptr(const quartet_ctx *C, const double *dd,
     return dd + ((((size_t)(gi * C->ng + gj) * L1 + la) * L1 + lb) * T1);
 }
 
-/* tot_integral_k_ijkr (elastic = 0, src/Integrals.f90:5068-5226) and
- * integral_k_ijkr (elastic = 1, :5730-5901).  ztot is supplied as a block.
- * Returns 0, or 2 when LLmax+1 > 20 (the reference's `stop`). */
+/* tot_integral_k_ijkr (mode 0, src/Integrals.f90:5068-5226),
+ * integral_k_ijkr (mode 1, :5730-5901) and tot_integral_k_ijkr_j2 (mode 2,
+ * :5230-5420).  ztot is supplied as a block.  Returns 0, or 2 when the
+ * coefficient arrays would overflow dim = 20 (the reference's `stop`). */
 static int k_ijkr(const quartet_ctx *C, int gi, int gj, int gk, int gr,
                   double hx, double hy, double hz, double h,
                   const double *ztot, double cutoffz, double cutoffmd,
-                  const double *q, int nq, int elastic, double *h_saved_out,
+                  const double *q, int nq, int mode, double *h_saved_out,
                   double *F)
 {
+    const int elastic = mode == 1, j2 = mode == 2;
     const int *l = C->l, *m = C->m, *n = C->n;
     int si = C->gs[gi], sj = C->gs[gj], sk = C->gs[gk], sr = C->gs[gr];
     int LLmax = l[si] + m[si] + n[si] + l[sj] + m[sj] + n[sj] + l[sk] + m[sk] +
                 n[sk] + l[sr] + m[sr] + n[sr];
-    if (LLmax + 1 > DIMH) return 2;
+    if (LLmax + 1 + (j2 ? 2 : 0) > DIMH) return 2;
     double a[DIMH][DIMH], b[DIMH][DIMH], c[DIMH][DIMH];
-    if (elastic) {
+    double ap[DIMH][DIMH], bp[DIMH][DIMH], cp[DIMH][DIMH];
+    if (j2) {
+        rrdj2a(LLmax, hx, a);
+        rrdj2a(LLmax, hy, b);
+        rrdj2a(LLmax, hz, c);
+        rrdj2b(LLmax, a, ap);
+        rrdj2b(LLmax, b, bp);
+        rrdj2b(LLmax, c, cp);
+        for (int i = 0; i < DIMH; ++i)
+            for (int j = 0; j < DIMH; ++j) cp[i][j] = -2.0 * cp[i][j];
+    } else if (elastic) {
         hermite_like(LLmax, hx, a);
         hermite_like(LLmax, hy, b);
         hermite_like(LLmax, hz, c);
@@ -679,7 +770,10 @@ static int k_ijkr(const quartet_ctx *C, int gi, int gj, int gk, int gr,
         for (int j = 0; j <= LLmax - k; ++j)
             for (int i = 0; i <= LLmax - k - j; ++i) {
                 for (int t = 0; t < DIMH; ++t) BD[t] = 0.0;
-                bessel_deriv(BD, k, j, i, a, b, c, LLmax);
+                if (j2)
+                    bessel_deriv2j(BD, hz, h, k, j, i, a, b, c, ap, bp, cp);
+                else
+                    bessel_deriv(BD, k, j, i, a, b, c, LLmax);
                 memcpy(hp + (((size_t)k * D + j) * D + i) * DIMH, BD,
                        sizeof(double) * DIMH);
             }
@@ -707,21 +801,21 @@ static int k_ijkr(const quartet_ctx *C, int gi, int gj, int gk, int gr,
                     const double *Dz2 = dd_ptr(C, C->ddz, gk, gr, n[k], n[r]);
                     for (int ll = 0; ll <= l[i] + l[j]; ++ll) {
                         double MDL = Dx1[ll];
-                        if (elastic && fabs(MDL) < (double)TINY_SP) continue;
+                        if ((elastic || j2) && fabs(MDL) < (double)TINY_SP) continue;
                         double prod1 = MDL * zt;
                         for (int mm = 0; mm <= m[i] + m[j]; ++mm) {
                             double MDM = Dy1[mm];
-                            if (elastic && fabs(MDM) < (double)TINY_SP) continue;
+                            if ((elastic || j2) && fabs(MDM) < (double)TINY_SP) continue;
                             double prod2 = MDM * prod1;
                             for (int nn = 0; nn <= n[i] + n[j]; ++nn) {
                                 double H1 = ((ll + mm + nn) & 1) ? -1.0 : 1.0;
                                 double MDN = Dz1[nn];
-                                if (elastic && fabs(MDN) < (double)TINY_SP)
+                                if ((elastic || j2) && fabs(MDN) < (double)TINY_SP)
                                     continue;
                                 double prod3 = MDN * H1 * prod2;
                                 for (int llp = 0; llp <= l[k] + l[r]; ++llp) {
                                     double MDLp = Dx2[llp];
-                                    if (elastic && fabs(MDLp) < (double)TINY_SP)
+                                    if ((elastic || j2) && fabs(MDLp) < (double)TINY_SP)
                                         continue;
                                     double prod4 = MDLp * prod3;
                                     for (int mmp = 0; mmp <= m[k] + m[r]; ++mmp) {
@@ -741,7 +835,7 @@ static int k_ijkr(const quartet_ctx *C, int gi, int gj, int gk, int gr,
                                                               D +
                                                           (nn + nnp)) *
                                                              DIMH;
-                                                for (int t = 0; t <= LLmax; ++t)
+                                                for (int t = 0; t <= LLmax + 2 * j2; ++t)
                                                     hs[t] = hs[t] + v[t] * prod6;
                                             }
                                         }
@@ -763,6 +857,12 @@ static int k_ijkr(const quartet_ctx *C, int gi, int gj, int gk, int gr,
             for (int iq = 0; iq < nq; ++iq) F[iq] = 0.0;
             return 0;
         }
+    }
+    if (j2) {
+        /* bessels0rr(LLmax+2) then F / mu**2 */
+        orc_bessels0rr(F, LLmax + 2, q, nq, h, hs);
+        for (int iq = 0; iq < nq; ++iq) F[iq] = F[iq] / pow(q[iq], 2.0);
+        return 0;
     }
     orc_bessels0rr(F, LLmax, q, nq, h, hs);
     return 0;
@@ -960,12 +1060,15 @@ static int total_j0_impl(int ngto, int ng, int maxl, const int *l, const int *m,
                          const double *total, int nmat, const double *q, int nq,
                          double cutoffz, double cutoffmd, double cutoffcentre,
                          int zmode, int nthreads, double *tsi, int64_t stride,
-                         double *kernel_seconds, int64_t *nsampled)
+                         double *kernel_seconds, int64_t *nsampled, int j2)
 {
     setup_t S;
     int rc = setup_build(&S, ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz,
                          q, nq);
     if (rc) return rc;
+    /* tot_integration_j2 (src/Integrals.f90:954-1422): the same nests with
+     * the j2 kernel and the P0_j2 table */
+    if (j2) orc_set_p0_j2(S.P0, S.D0 - 1, q, nq);
     int64_t Q = orc_quartet_list(ng, S.px, S.py, S.pz, cutoffcentre, NULL);
     int32_t *ql = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)Q);
     orc_quartet_list(ng, S.px, S.py, S.pz, cutoffcentre, ql);
@@ -1108,7 +1211,7 @@ static int total_j0_impl(int ngto, int ng, int maxl, const int *l, const int *m,
                 double h = mult == 8 ? pow(hx * hx + hy * hy + hz * hz, 0.5)
                                      : sqrt(hx * hx + hy * hy + hz * hz);
                 int r2 = k_ijkr(&C, gi, gj, gk, gr, hx, hy, hz, h, zb, cutoffz,
-                                cutoffmd, q, nq, 0, NULL, F);
+                                cutoffmd, q, nq, j2 ? 2 : 0, NULL, F);
                 if (r2) {
 #pragma omp atomic write
                     err = r2;
@@ -1158,7 +1261,20 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
 {
     return total_j0_impl(ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, mmod,
                          nmo, mat, total, nmat, q, nq, cutoffz, cutoffmd,
-                         cutoffcentre, zmode, nthreads, tsi, 1, NULL, NULL);
+                         cutoffcentre, zmode, nthreads, tsi, 1, NULL, NULL, 0);
+}
+
+int orc_total_j2(int ngto, int ng, int maxl, const int *l, const int *m,
+                 const int *n, const int *group, const double *ga,
+                 const double *xx, const double *yy, const double *zz,
+                 const double *mmod, int nmo, const int *mat,
+                 const double *total, int nmat, const double *q, int nq,
+                 double cutoffz, double cutoffmd, double cutoffcentre,
+                 int zmode, int nthreads, double *tsi)
+{
+    return total_j0_impl(ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, mmod,
+                         nmo, mat, total, nmat, q, nq, cutoffz, cutoffmd,
+                         cutoffcentre, zmode, nthreads, tsi, 1, NULL, NULL, 1);
 }
 
 int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
@@ -1174,7 +1290,7 @@ int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
     return total_j0_impl(ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, mmod,
                          nmo, mat, total, nmat, q, nq, cutoffz, cutoffmd,
                          cutoffcentre, 1, nthreads, tsi, stride,
-                         kernel_seconds, nsampled);
+                         kernel_seconds, nsampled, 0);
 }
 
 int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,

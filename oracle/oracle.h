@@ -93,6 +93,18 @@ int orc_total_j0(int ngto, int ng, int maxl, const int *l, const int *m,
  * _pzero, bessels0rr and the e12 accumulation) with the 2-RDM blocks
  * supplied; tsi holds the sample's partial S(q).  Used for the CPU
  * baseline. */
+/* tot_integration_j2 (src/Integrals.f90:954-1422), type 7 of
+ * total_scattering_calculation; same arguments as orc_total_j0. */
+int orc_total_j2(int ngto, int ng, int maxl, const int *l, const int *m,
+                 const int *n, const int *group, const double *ga,
+                 const double *xx, const double *yy, const double *zz,
+                 const double *mmod, int nmo, const int *mat,
+                 const double *total, int nmat, const double *q, int nq,
+                 double cutoffz, double cutoffmd, double cutoffcentre,
+                 int zmode, int nthreads, double *tsi);
+/* set_P0_j2 (src/Integrals.f90:3034-3064), same layout as orc_set_p0 */
+void orc_set_p0_j2(double *P0, int lmax4, const double *q, int nq);
+
 int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
                         const int *m, const int *n, const int *group,
                         const double *ga, const double *xx, const double *yy,

@@ -3,7 +3,8 @@
 ! Links the reference's own modules (integrals, twordms, onerdm,
 ! uniquemodule, compiled unmodified from their sources) and calls
 !   set_P0, maxcoincidence, createtwordm, tot_integration,
-!   onerdm_creat, elastic_integration, unique_total
+!   set_P0_j2, tot_integration_j2, onerdm_creat, elastic_integration,
+!   unique_total
 ! on one input record.  The setup tables of variables_total /
 ! variables_elastic (Ddx/Ddy/Ddz, px/py/pz, e12) are read from the input
 ! record because src/Variables.f90 does not compile (line 1012); the z1/z2
@@ -28,7 +29,8 @@ program ref_driver
     integer(ik), allocatable :: confs(:,:), ep3(:,:), ndiff2(:,:), mat(:,:), rmat(:,:)
     real(dp), allocatable :: ddx(:,:,:,:,:), ddy(:,:,:,:,:), ddz(:,:,:,:,:), e12(:,:,:)
     real(dp), allocatable :: p0(:,:,:,:), total(:), rtot(:), z1(:,:,:), z2(:,:,:)
-    real(dp), allocatable :: zt(:,:,:), res1(:), res2(:), orm(:,:), zel(:,:)
+    real(dp), allocatable :: zt(:,:,:), res1(:), res2(:), res3(:), orm(:,:), zel(:,:)
+    real(dp), allocatable :: p0j2(:,:,:,:)
     character(len=512) :: fin, fout
 
     call get_command_argument(1, fin)
@@ -72,8 +74,9 @@ program ref_driver
     call createtwordm(confs, civs, ndiff2, ep3, mat, total, state1, state2)
     nmat = size(total)
 
-    allocate(res1(nq), res2(nq))
+    allocate(res1(nq), res2(nq), res3(nq))
     res1 = 0.0_dp
+    res3 = 0.0_dp
     if (dolist /= 0) then
         ! z11/z22 of variables_total (src/Variables.f90:1041-1061); the pair
         ! sort of :1000-1008 does not change these values.
@@ -89,6 +92,12 @@ program ref_driver
         zt = 0.0_dp
         call tot_integration(ng, px, py, pz, l, m, n, p0, ddx, ddy, ddz, z1, z2, zt, zt, &
                              gs, gc, group, cutoffz, cutoffmd, cutoffcentre, q, e12, res1)
+        ! total j2 (type 7): the P0_j2 table and tot_integration_j2
+        allocate(p0j2(nq, 4*maxval(l)+1, 4*maxval(l)+1, 4*maxval(l)+1))
+        p0j2 = 0.0_dp
+        call set_P0_j2(p0j2, 4*maxval(l), q)
+        call tot_integration_j2(ng, nq, px, py, pz, l, m, n, p0j2, ddx, ddy, ddz, z1, z2, &
+                                gs, gc, group, cutoffz, cutoffmd, cutoffcentre, q, e12, res3)
         deallocate(z1, z2)
     end if
 
@@ -123,6 +132,7 @@ program ref_driver
     write(11) zel
     write(11) rmat
     write(11) rtot
+    write(11) res3
     write(11) p0
     close(11)
 end program ref_driver

@@ -126,3 +126,32 @@ def test_screened_list_is_stable_compaction(golden, name):
     assert np.all(np.diff(pos) > 0)
     # random MOs leave no all-zero block: nothing is dropped
     assert scr.shape[0] == full.shape[0]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_total_j2_matches_reference(golden, name):
+    """Type 7 (tot_integration_j2) against the reference; at q[0] = 1e-10 the
+    j2 intensity is ~1e-20 and is held to 1e-10 of max |S|."""
+    g = golden(name)
+    cz, cm, cc = g["cutoffs"]
+    res = sc.total_j2_from_twordm(*_args(g), g["mmod"], g["q"], g["group"],
+                                  g["mat"].astype(np.int32), g["total"], cz, cm, cc)
+    ref = g["S_total_j2"]
+    rel = np.max(np.abs(res - ref)[1:] / np.abs(ref)[1:])
+    print(f"{name}: max rel err total j2 {rel:.3e}", flush=True)
+    assert rel < RTOL_FP64
+    assert abs(res[0] - ref[0]) <= RTOL_FP64 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", ["sp3", "spd2"])
+def test_drop_in_type7(golden, name):
+    g = golden(name)
+    cz, cm, cc = g["cutoffs"]
+    l = g["l"]
+    _, res = sc.total_scattering_calculation(
+        7, np.ones(2), np.zeros((2, 3)), 1, 1, 1, int(l.max()), l.size, int(g["group"].max()),
+        g["ga"], l, g["m"], g["n"], g["xx"], g["yy"], g["zz"], g["mmod"], g["q"], g["q"].size,
+        g["group"], cz, cm, cc, g["confs"].astype(np.int32), g["civs"])
+    ref = g["S_total_j2"]
+    assert np.max(np.abs(res - ref)[1:] / np.abs(ref)[1:]) < RTOL_FP64
+

@@ -51,6 +51,16 @@ def test_total_vs_oracle(case):
     assert rel < RTOL_FP64
 
 
+def test_total_j2_vs_oracle(case):
+    name, inp, q, mat, tot = case
+    res = sc.total_j2_from_twordm(*_args(inp), q, inp.group, mat, tot)
+    ref = oracle.total_j2(inp.l, inp.m, inp.n, inp.group, inp.ga, inp.xx, inp.yy, inp.zz,
+                          inp.mmod, mat, tot, q)
+    rel = np.max(np.abs(res - ref)[1:] / np.abs(ref)[1:])
+    print(f"{name}: total j2 max rel err {rel:.3e}", flush=True)
+    assert rel < RTOL_FP64
+
+
 def test_elastic_vs_oracle(case):
     name, inp, q, mat, tot = case
     g = sc.onerdm(inp.confs, inp.civs)

@@ -146,6 +146,9 @@ def test_status_codes_before_device(lib):
     mixed_l[2] = 2                                     # a d component in a p shell
     assert _status(1, ga, l, m, mixed_l, xyz, group, mmod, q, mat, tot) == 4
     assert _status(1, ga, l, m, n, xyz, group, mmod, q, np.array([[1, 1, 3, 1]]), tot) == 1
+    gl = l.copy()
+    gl[3] = 4                                          # a g shell: j2 order 18 > 16
+    assert _status(7, ga, gl, m, n, xyz, group, mmod, q, mat, tot) == 3
     assert _status(2, ga, l, m, n, xyz, group, mmod, q, None, None, 3, np.eye(3)) == 1
     assert _status(2, ga, l, m, n, xyz, group, mmod, q, None, None, 0, None) == 1
 

@@ -64,6 +64,19 @@ def test_elastic_j0(case):
     assert rel < 1e-12, (name, rel)
 
 
+def test_total_j2(case):
+    """tot_integration_j2 (src/Integrals.f90:954-1422) with set_P0_j2: type 7.
+    At q[0] = 1e-10 the j2 intensity is ~1e-20 and only its size is checked."""
+    name, g = case
+    mat, tot = g["mat"].astype(np.int32), g["total"]
+    S = oracle.total_j2(g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"], g["yy"],
+                        g["zz"], g["mmod"], mat, tot, g["q"], **_cut(g))
+    ref = g["S_total_j2"]
+    rel = np.max(np.abs(S - ref)[1:] / np.abs(ref)[1:])
+    assert rel < 1e-12, (name, rel)
+    assert abs(S[0] - ref[0]) <= 1e-14 * np.max(np.abs(ref))
+
+
 def test_unique_total(case):
     """L3: unique_total (src/unique_module.f90:90-145) on the 2-RDM rows."""
     name, g = case

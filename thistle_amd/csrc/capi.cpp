@@ -454,6 +454,7 @@ SeriesArgs series_args(const ThistlePlan* pl) {
     a.ldz = elastic ? pl->ngto : pl->npp;
     a.elastic = elastic ? 1 : 0;
     a.mixed = pl->precision == THISTLE_MIXED;
+    a.j2 = pl->type == 7 ? 1 : 0;
     a.cutoffz = pl->cutoffz;
     a.part = pl->part.as<double>();
     a.partc = pl->partc.as<double>();
@@ -486,8 +487,9 @@ int run_series(ThistlePlan* pl, bool timing) {
             int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
             if (n <= 0) continue;
             int w = (int)std::min<int64_t>(pl->waves, n);
-            HCHECK(launch_series(a, LL, pl->cls.as<int32_t>() + pl->cls_off[LL], n, w, 0,
-                                 pl->stream));
+            // j2 carries its series two orders further (bessels0rr(LLmax+2))
+            HCHECK(launch_series(a, a.j2 ? LL + 2 : LL, pl->cls.as<int32_t>() + pl->cls_off[LL], n,
+                                 w, 0, pl->stream));
         }
         HCHECK(launch_reduce(a.part, a.partc, pl->waves, a.nqc, pl->S.as<double>() + q0,
                              pl->stream));
@@ -512,7 +514,7 @@ int enqueue_execute(ThistlePlan* pl, bool timing, double* S_dst) {
     if (timing) HCHECK(hipEventRecord(pl->ev[0], pl->stream));
     int rc = run_tables_and_enum(pl, timing);
     if (rc) return rc;
-    if (pl->type == 1) {
+    if (pl->type == 1 || pl->type == 7) {
         rc = run_transform_total(pl);
         if (rc) return rc;
     } else {
@@ -530,10 +532,16 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
                      const double* total, int32_t norb, const double* onerdm,
                      const ThistleOpts* o, ThistlePlan** out) {
     // every host-side argument check comes before the first HIP call
-    if (type != 1 && type != 2) return THISTLE_ETYPE;
+    if (type != 1 && type != 2 && type != 7) return THISTLE_ETYPE;
     if (!b || !b->mmod || b->nmo <= 0) return THISTLE_EINVAL;
+    if (type == 7 && b->l && b->m && b->n) {
+        // the j2 series runs to order LLmax + 2; the kernels stop at 16
+        int lmax = 0;
+        for (int j = 0; j < b->ngto; ++j) lmax = std::max(lmax, b->l[j] + b->m[j] + b->n[j]);
+        if (4 * lmax + 2 > 16) return THISTLE_ELLMAX;
+    }
     int nrb = 0;
-    if (type == 1) {
+    if (type == 1 || type == 7) {
         if (nmat < 0 || (nmat > 0 && (!mat || !total))) return THISTLE_EINVAL;
         for (int64_t k = 0; k < 4 * nmat; ++k) {
             if (mat[k] < 1 || mat[k] > b->nmo || mat[k] > 65535) return THISTLE_EINVAL;
@@ -546,7 +554,7 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
     pl->type = type;
     int rc = setup_common(pl.get(), b, o);
     if (rc) return rc;
-    if (type == 1) {
+    if (type == 1 || type == 7) {
         pl->norb = std::max(nrb, 1);
         pl->nmat = nmat;
         pl->npmo = pl->norb * (pl->norb + 1) / 2;
@@ -1056,13 +1064,13 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
     (void)Zn;
     (void)geom;
     (void)state3;
-    if (type != 1 && type != 2) return THISTLE_ETYPE;
+    if (type != 1 && type != 2 && type != 7) return THISTLE_ETYPE;
     if (!basis || !result_out) return THISTLE_EINVAL;
     const ThistleOpts* o = opts ? opts : &kDefaultOpts;
     if (q_abs_out)
         for (int i = 0; i < basis->nq; ++i) q_abs_out[i] = 0.0;
     const int norb = lconfs / 2;
-    if (type == 1) {
+    if (type == 1 || type == 7) {
         std::vector<double> tw((size_t)norb * norb * norb * norb);
         int rc = thistle_twordm_dense_host(confs, nconf, lconfs, civecs, nstates, state1, state2,
                                            tw.data());
@@ -1072,7 +1080,14 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
         std::vector<int32_t> mat((size_t)4 * std::max<int64_t>(cnt, 1));
         std::vector<double> tot(std::max<int64_t>(cnt, 1));
         thistle_twordm_select_host(tw.data(), norb, cut, mat.data(), tot.data(), cnt);
-        return thistle_total_from_twordm(basis, cnt, mat.data(), tot.data(), o, result_out);
+        if (type == 1)
+            return thistle_total_from_twordm(basis, cnt, mat.data(), tot.data(), o, result_out);
+        ThistlePlan* pl = nullptr;
+        rc = thistle_plan_create(7, basis, cnt, mat.data(), tot.data(), 0, nullptr, o, &pl);
+        if (rc) return rc;
+        rc = thistle_plan_execute(pl, result_out);
+        thistle_plan_destroy(pl);
+        return rc;
     }
     std::vector<double> g((size_t)norb * norb);
     int rc = thistle_onerdm_host(confs, nconf, lconfs, civecs, nstates, state1, state2, g.data());

@@ -36,10 +36,18 @@
 
 
 // ------------------------------------------------------------ contraction
-template <int LL>
+// J2 (total j2, tot_integral_k_ijkr_j2, src/Integrals.f90:5230-5420): besides
+// h(beta) the kernel accumulates g(beta) = sum z sum px py (mz pz), and writes
+//   series: h2(b) = c (eta h(b-2) + 2(b-1) h(b-1) - 6 g(b-1)),  b <= LL+2,
+//           c = sqrt(5/pi)/4, eta = 3 hz^2 - h^2 (BesselDeriv2j with rrdj2a/b
+//           factorised like the j0 case: the rrdj2b weight 2 ceil((l+p)/2)
+//           is 2 m per axis, and -2 on z)
+//   h = 0:  c0(b) = sqrt(5/pi)/2 (b h(b) - 3 g(b)),  b <= LL  (P0_j2, :3034-3170)
+template <int LL, bool J2>
 __global__ void __launch_bounds__(ContractShape<LL>::CT)
     k_contract(SeriesArgs A, const int32_t* __restrict__ idx, int64_t nidx, int tmax) {
     constexpr int D = LL + 1;
+    constexpr int NA = J2 ? 2 : 1;  // accumulated vectors: h, and g for j2
     constexpr int CT = ContractShape<LL>::CT, G = ContractShape<LL>::G, NG = ContractShape<LL>::NG;
     constexpr int NW = CT / WAVE;
     extern __shared__ double sm[];
@@ -47,7 +55,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const int grp = tid / G, lg = tid % G;
     double* PX = sm + (size_t)grp * 3 * tmax * D;  // [3][T][D] of this group's quartet
-    double* red = sm + (size_t)NG * 3 * tmax * D;  // [NW][D] when a quartet spans waves
+    double* red = sm + (size_t)NG * 3 * tmax * D;  // [NA][NW][D] when a quartet spans waves
     const int L1 = A.maxl + 1, T1 = 2 * A.maxl + 1;
 
     for (int64_t base = (int64_t)blockIdx.x * NG; base < nidx; base += (int64_t)gridDim.x * NG) {
@@ -62,8 +70,8 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
         const GroupInfo Gi = A.G[gi], Gj = A.G[gj], Gk = A.G[gk], Gr = A.G[gr];
         const int Li = Gi.L, Lj = Gj.L, Lk = Gk.L, Lr = Gr.L;
         const int T = valid ? (Li + 1) * (Lj + 1) * (Lk + 1) * (Lr + 1) : 0;
-        const double hx = br ? 0.0 : PK.Px - PB.Px, hy = br ? 0.0 : PK.Py - PB.Py,
-                     hz = br ? 0.0 : PK.Pz - PB.Pz;
+        const double rx = PK.Px - PB.Px, ry = PK.Py - PB.Py, rz = PK.Pz - PB.Pz;
+        const double hx = br ? 0.0 : rx, hy = br ? 0.0 : ry, hz = br ? 0.0 : rz;
 
         __syncthreads();  // the previous quartets are done with the tables
         // px(m) = sum_T c(T) a(T, 2m - T) per axis and exponent tuple, with
@@ -114,9 +122,11 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
         const int S = A.nsig;
         const uint64_t* qt =
             A.quadtab + A.comboff[((Gi.pad * S + Gj.pad) * S + Gk.pad) * S + Gr.pad];
-        double h[D];
+        double h[NA][D];
 #pragma unroll
-        for (int b = 0; b < D; ++b) h[b] = 0.0;
+        for (int k = 0; k < NA; ++k)
+#pragma unroll
+            for (int b = 0; b < D; ++b) h[k][b] = 0.0;
         int any = 0;
         for (uint32_t u = lg; u < nq4; u += G) {
             const uint64_t w = qt[u];
@@ -161,43 +171,93 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
                 double s = 0.0;
 #pragma unroll
                 for (int mx = 0; mx <= b; ++mx) s += vx[mx] * yz[b - mx];
-                h[b] = fma(z, s, h[b]);
+                h[0][b] = fma(z, s, h[0][b]);
+            }
+            if constexpr (J2) {
+                double yzm[D];
+#pragma unroll
+                for (int k2 = 0; k2 < D; ++k2) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int my = 0; my < k2; ++my) s += vy[my] * ((double)(k2 - my) * vz[k2 - my]);
+                    yzm[k2] = s;
+                }
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int mx = 0; mx <= b; ++mx) s += vx[mx] * yzm[b - mx];
+                    h[NA - 1][b] = fma(z, s, h[NA - 1][b]);
+                }
             }
         }
+        // the values of h(beta) (j0) or of the j2 series / h = 0 polynomial
+        // coefficients (DO of them), given the reduced sums in hr / gr
+        constexpr int DO = J2 ? D + 2 : D;
+        auto coef = [&](int b, const double* hr, const double* grr) -> double {
+            if constexpr (!J2) {
+                return hr[b];
+            } else {
+                const double r5 = 1.2615662610100802;  // sqrt(5/pi)
+                if (br) return b < D ? 0.5 * r5 * ((double)b * hr[b] - 3.0 * grr[b]) : 0.0;
+                const double eta = 3.0 * rz * rz - (rx * rx + ry * ry + rz * rz);
+                double v = 0.0;
+                if (b >= 2) v = eta * hr[b - 2];
+                if (b >= 1 && b - 1 < D) v += 2.0 * (double)(b - 1) * hr[b - 1] - 6.0 * grr[b - 1];
+                return 0.25 * r5 * v;
+            }
+        };
         if constexpr (G <= WAVE) {
             // butterfly over the group's lanes; every lane ends with the sums
 #pragma unroll
-            for (int b = 0; b < D; ++b) {
-                double v = h[b];
+            for (int k = 0; k < NA; ++k)
 #pragma unroll
-                for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-                h[b] = v;
-            }
+                for (int b = 0; b < D; ++b) {
+                    double v = h[k][b];
+#pragma unroll
+                    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+                    h[k][b] = v;
+                }
             const uint64_t bal = __ballot(any);
             const uint64_t gmask =
                 G == WAVE ? ~0ull : (((1ull << G) - 1ull) << ((lane / G) * G));
             if (valid) {
                 double mine = 0.0;
 #pragma unroll
-                for (int b = 0; b < D; ++b)
-                    if (lg == b) mine = h[b];
-                if (lg < D) A.hs[e * HS + lg] = mine;
+                for (int b = 0; b < DO; ++b)
+                    if (lg == b) mine = coef(b, h[0], h[NA - 1]);
+                if (lg < DO) A.hs[e * HS + lg] = mine;
                 if (lg == 0) A.flags[e] = (bal & gmask) ? 1 : 0;
             }
         } else {
 #pragma unroll
-            for (int b = 0; b < D; ++b) {
-                double v = wsum(h[b]);
-                if (lane == 0) red[wid * D + b] = v;
-            }
+            for (int k = 0; k < NA; ++k)
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    double v = wsum(h[k][b]);
+                    if (lane == 0) red[(k * NW + wid) * D + b] = v;
+                }
             int wa = __any(any);
             if (lane == 0) anyw[wid] = wa;
             __syncthreads();
-            if (valid && tid < D) {
-                double s = red[tid];
+            if (valid && tid < DO) {
+                double hr[D], grr[D];
 #pragma unroll
-                for (int w = 1; w < NW; ++w) s += red[w * D + tid];
-                A.hs[e * HS + tid] = s;
+                for (int b = 0; b < D; ++b) {
+                    double s0 = red[b], s1 = red[((NA - 1) * NW) * D + b];
+#pragma unroll
+                    for (int w = 1; w < NW; ++w) {
+                        s0 += red[w * D + b];
+                        s1 += red[((NA - 1) * NW + w) * D + b];
+                    }
+                    hr[b] = s0;
+                    grr[b] = s1;
+                }
+                double mine = 0.0;
+#pragma unroll
+                for (int b = 0; b < DO; ++b)
+                    if (tid == b) mine = coef(b, hr, grr);
+                A.hs[e * HS + tid] = mine;
             }
             if (valid && tid == 0) {
                 int f = 0;
@@ -337,10 +397,25 @@ __device__ __forceinline__ double f_up32(const float (&hf)[LL + 1], double q, do
 
 #define SQ 4   // q slices (of 64 points) per wave
 
+// h = 0 polynomial of total j2: sum_b c(b) q^(2b) / (2b+3)!!
+template <int LL>
+__device__ __forceinline__ double f_pzero_j2(const double (&hs)[LL + 1], double q) {
+    double q2 = q * q;
+    double dfi = 1.0;
+#pragma unroll
+    for (int b = 0; b <= LL; ++b) dfi /= (double)(2 * b + 3);
+    double F = hs[LL] * dfi;
+#pragma unroll
+    for (int b = LL - 1; b >= 0; --b) {
+        dfi *= (double)(2 * b + 5);
+        F = fma(F, q2, hs[b] * dfi);
+    }
+    return F;
+}
+
 template <int LL, bool MIXED>
-__device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], double q, double h,
-                                         double invh, int br) {
-    if (br) return f_pzero<LL>(hs, q);
+__device__ __forceinline__ double f_series(const double (&hs)[LL + 1], double q, double h,
+                                           double invh) {
     const double x = q * h;
     if (MIXED && x > (double)LL + 6.0 && x < XDPH) {
         float hf[LL + 1];
@@ -358,7 +433,23 @@ __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], double q, d
     return f_miller<LL>(hs, q, x, j0);
 }
 
-template <int LL, bool MIXED>
+// F(q) of one quartet.  J2: the series divided by q^2 (bessels0rr then
+// F/mu**2, src/Integrals.f90:5415-5418), whose q -> 0 limit is c(1)/3 (the
+// reference's 0/0), or the j2 h = 0 polynomial.
+template <int LL, bool MIXED, bool J2>
+__device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], double q, double h,
+                                         double invh, int br) {
+    if constexpr (J2) {
+        if (br) return f_pzero_j2<LL>(hs, q);
+        const double F = f_series<LL, MIXED>(hs, q, h, invh);
+        return q > 0.0 ? F / (q * q) : (LL >= 1 ? hs[LL >= 1 ? 1 : 0] * (1.0 / 3.0) : 0.0);
+    } else {
+        if (br) return f_pzero<LL>(hs, q);
+        return f_series<LL, MIXED>(hs, q, h, invh);
+    }
+}
+
+template <int LL, bool MIXED, bool J2>
 __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __restrict__ idx,
                                                  int64_t nidx, int first) {
     const int lane = threadIdx.x;
@@ -401,7 +492,7 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
         const double* ek = A.e12 + (int64_t)pk * A.nq;
 #pragma unroll
         for (int s = 0; s < SQ; ++s) {
-            double v = mult * f_eval<LL, MIXED>(hs, qv[s], h, invh, br);
+            double v = mult * f_eval<LL, MIXED, J2>(hs, qv[s], h, invh, br);
             v = v * ebv[s];
             v = v * ek[iqs[s]];
             kahan(S[s], Sc[s], v);
@@ -430,7 +521,8 @@ namespace {
 template <int LL>
 size_t contract_lds_for(int Tmax) {
     constexpr int D = LL + 1, NG = ContractShape<LL>::NG, NW = ContractShape<LL>::CT / WAVE;
-    return sizeof(double) * ((size_t)NG * 3 * Tmax * D + (NG == 1 ? (size_t)NW * D : 0));
+    // one quartet per block: h and (for j2) g partial sums per wave
+    return sizeof(double) * ((size_t)NG * 3 * Tmax * D + (NG == 1 ? (size_t)2 * NW * D : 0));
 }
 }  // namespace
 
@@ -456,8 +548,12 @@ hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t*
     case k: {                                                                                  \
         constexpr int NG = ContractShape<k>::NG;                                               \
         int blocks = (int)std::min<int64_t>(max_blocks, (nidx + NG - 1) / NG);                 \
-        hipLaunchKernelGGL((k_contract<k>), dim3(blocks), dim3(ContractShape<k>::CT), lds, s, a, \
-                           idx, nidx, Tmax);                                                   \
+        if (a.j2)                                                                              \
+            hipLaunchKernelGGL((k_contract<k, true>), dim3(blocks), dim3(ContractShape<k>::CT),  \
+                               lds, s, a, idx, nidx, Tmax);                                    \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_contract<k, false>), dim3(blocks), dim3(ContractShape<k>::CT), \
+                               lds, s, a, idx, nidx, Tmax);                                    \
         break;                                                                                 \
     }
         ALL_LL(C_)
@@ -472,9 +568,12 @@ int contract_threads(int LL) { return LL < 10 ? 64 : 256; }
 hipError_t set_contract_lds_limit(size_t bytes) {
 #define C_(k)                                                                            \
     {                                                                                    \
-        hipError_t e = hipFuncSetAttribute((const void*)k_contract<k>,                   \
+        hipError_t e = hipFuncSetAttribute((const void*)k_contract<k, false>,            \
                                            hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                            (int)bytes);                                  \
+        if (e != hipSuccess) return e;                                                   \
+        e = hipFuncSetAttribute((const void*)k_contract<k, true>,                        \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes); \
         if (e != hipSuccess) return e;                                                   \
     }
     ALL_LL(C_)
@@ -489,10 +588,18 @@ hipError_t launch_series(const SeriesArgs& a, int LL, const int32_t* idx, int64_
     switch (LL) {
 #define CASE(k)                                                                            \
     case k:                                                                                \
-        if (a.mixed)                                                                       \
-            hipLaunchKernelGGL((k_series<k, true>), grid, dim3(WAVE), 0, s, a, idx, nidx, first); \
+        if (a.j2 && a.mixed)                                                               \
+            hipLaunchKernelGGL((k_series<k, true, true>), grid, dim3(WAVE), 0, s, a, idx, nidx,  \
+                               first);                                                     \
+        else if (a.j2)                                                                     \
+            hipLaunchKernelGGL((k_series<k, false, true>), grid, dim3(WAVE), 0, s, a, idx, nidx, \
+                               first);                                                     \
+        else if (a.mixed)                                                                  \
+            hipLaunchKernelGGL((k_series<k, true, false>), grid, dim3(WAVE), 0, s, a, idx, nidx, \
+                               first);                                                     \
         else                                                                               \
-            hipLaunchKernelGGL((k_series<k, false>), grid, dim3(WAVE), 0, s, a, idx, nidx, first); \
+            hipLaunchKernelGGL((k_series<k, false, false>), grid, dim3(WAVE), 0, s, a, idx,     \
+                               nidx, first);                                               \
         break;
         CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)

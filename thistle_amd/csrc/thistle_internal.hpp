@@ -67,6 +67,7 @@ struct SeriesArgs {
     const float* Zf;          // fp32 copy (mixed mode), may be null
     int64_t ldz;
     int elastic, mixed;
+    int j2;                   // total j2 (type 7): h2 coefficients, series / q^2
     double cutoffz;
     double* part;             // [waves][nqc] sums
     double* partc;            // [waves][nqc] Kahan compensations

@@ -17,7 +17,7 @@ THISTLE_OK = 0
 STATUS = {0: "ok", 1: "invalid argument", 2: "a GTO has l, m or n > 4",
           3: "quartet LLmax + 1 > 20", 4: "a group is not one contiguous single-shell run",
           5: "HIP / rocBLAS failure", 6: "device allocation failed",
-          7: "calculation type must be 1 (total j0) or 2 (elastic j0)"}
+          7: "calculation type must be 1 (total j0), 2 (elastic j0) or 7 (total j2)"}
 FP64, MIXED = 0, 1
 
 EXPORTS = [

@@ -35,9 +35,9 @@ def total_scattering_calculation(type, Zn, geom, state1, state2, state3, maxl, n
     q = np.asarray(q, dtype=np.float64)
     if int(nq) != q.size or int(ngto) != np.asarray(l).size:
         raise ValueError("nq / ngto do not match the array sizes")
-    if int(type) not in (1, 2):
+    if int(type) not in (1, 2, 7):
         raise NotImplementedError(
-            f"type {type}: only 1 (total j0) and 2 (elastic j0) are on this path")
+            f"type {type}: only 1 (total j0), 2 (elastic j0) and 7 (total j2) are on this path")
     b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
                maxl=maxl, ng=ng)
     o = L.make_opts(precision, rdm_cutoff, waves=waves)
@@ -72,6 +72,18 @@ def total_from_twordm(ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total, cutof
                                        L._ptr(total), C.byref(o), L._ptr(res))
     L.check(rc, "total_from_twordm")
     return res
+
+
+def total_j2_from_twordm(ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total, cutoffz=1e-15,
+                         cutoffmd=1e-20, cutoffcentre=0.001, *, precision=L.FP64, waves=0):
+    """Total j2 S(q) (type 7, tot_integration_j2) from 2-RDM rows (p,s,q,r) + values."""
+    plan = Plan(7, ga, l, m, n, xx, yy, zz, mmod, q, group, mat=mat, total=total,
+                cutoffz=cutoffz, cutoffmd=cutoffmd, cutoffcentre=cutoffcentre,
+                precision=precision, waves=waves)
+    try:
+        return plan.execute()
+    finally:
+        plan.close()
 
 
 def total_adjoint(ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total, dL_dS, geom=None,
