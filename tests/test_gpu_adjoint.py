@@ -17,8 +17,9 @@ pytestmark = pytest.mark.gpu
 
 def _case(name):
     if name == "spd3":
+        # 4 core + CAS(4,4): 8 orbitals, Z in the MO-factorised layout
         w = synth.make_small(seed=41, natom=3, spec={"s": 2, "p": 1, "d": 1}, nact=4, nelec=4,
-                             ncore=1, nq=48, qmax=8.0)
+                             ncore=4, nq=48, qmax=8.0)
     else:  # 10 atoms, L <= 2, CAS(6,6), 256 q
         w = synth.make_mol10()
     with tempfile.TemporaryDirectory() as d:

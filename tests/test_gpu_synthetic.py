@@ -17,11 +17,15 @@ pytestmark = pytest.mark.gpu
 RTOL_FP64 = 1e-10
 
 
-@pytest.fixture(scope="module", params=["spdf3", "sp6"])
+@pytest.fixture(scope="module", params=["spdf3", "sp6", "spd3core5"])
 def case(request):
     if request.param == "spdf3":
         w = synth.make_small(seed=31, natom=3, spec={"s": 2, "p": 1, "d": 1, "f": 1}, nact=4,
                              nelec=4, ncore=1, nq=96, qmax=14.0)
+    elif request.param == "spd3core5":
+        # 9 orbitals, 5 of them core
+        w = synth.make_small(seed=33, natom=3, spec={"s": 2, "p": 2, "d": 1}, nact=4,
+                             nelec=4, ncore=5, nq=64, qmax=10.0)
     else:
         w = synth.make_small(seed=32, natom=6, spec={"s": 2, "p": 2}, nact=4, nelec=4,
                              ncore=2, nq=128, qmax=12.0)
@@ -100,3 +104,17 @@ def test_mixed_precision_bound(case):
     err = np.max(np.abs(smx - s64)) / np.max(np.abs(s64))
     print(f"{name}: mixed normwise err {err:.3e}", flush=True)
     assert err <= 1e-6
+
+
+def test_z_transforms_agree(case, monkeypatch):
+    """Z = T^T Gs T by one symmetric rank-k product and through the MO
+    factorisation of T give the same S(q)."""
+    name, inp, q, mat, tot = case
+    out = []
+    for fact in ("0", "1"):
+        monkeypatch.setenv("THISTLE_ZFACT", fact)
+        out.append(sc.total_from_twordm(*_args(inp), q, inp.group, mat, tot))
+    rel = np.max(np.abs(out[0] - out[1]) / np.abs(out[0]))
+    print(f"{name}: Z transforms max rel diff {rel:.3e}", flush=True)
+    assert rel < RTOL_FP64
+

@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
             const uint64_t w = qt[u];
             const int64_t xb = PB.off + (int)(w & 255u);
             const int64_t xk = PK.off + (int)((w >> 8) & 255u);
-            const double z = xk >= xb ? A.Z[xk + A.ldz * xb] : A.Z[xb + A.ldz * xk];
+            const double z = load_z(A, PB.off, PK.off, w, Gi, Gj, Gk, Gr);
             if (fabs(z) < A.cutoffz) continue;
             const int tt[3] = {(int)((w >> 16) & 1023u), (int)((w >> 26) & 1023u),
                                (int)((w >> 36) & 1023u)};

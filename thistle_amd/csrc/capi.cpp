@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -77,6 +78,7 @@ struct ThistlePlan {
     int type = 1, precision = 0;
     int ngto = 0, ng = 0, maxl = 0, nq = 0, nmo = 0, norb = 0;
     int npair = 0, npmo = 0, waves = 0, waves_alloc = 0, nrows = 0;
+    bool zfact = false;  // total Z through the MO factorisation of T (else one dsyrkx)
     int64_t npp = 0, nquart = 0, nmat = 0, units = 0;
     double cutoffz = 0, cutoffmd = 0, cutoffcentre = 0;
     hipStream_t stream = nullptr;
@@ -87,8 +89,9 @@ struct ThistlePlan {
     // device buffers
     DevBuf G, P, E, e12, q, lxyz, rows, qlist, flags, scan, qcomp, count;
     DevBuf mmod, mat, total, rmat, rtot, hkeys, hmin, hcnt, hslot, uscan;
-    DevBuf V, Gs, T, M, Z, mopairs, xa, xb, onerdm;
+    DevBuf V, Gs, T, M, Z, Cp, W3, jtab, aend, mopairs, xa, xb, onerdm;
     DevBuf part, partc, S, cls;
+    std::vector<int> gstart, gcount;  // per primitive shell (host)
     int64_t cls_off[18] = {0};  // per-LL ranges in `cls` (positions, ref order)
     int cls_tmax[17] = {0};     // largest exponent-tuple count per LL class
     DevBuf hs;                  // h(beta) per quartet
@@ -202,6 +205,10 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if (rc) return rc;
     const int S = (int)sigs.size();
     pl->precision = o->precision;
+    for (const auto& g : groups) {
+        pl->gstart.push_back(g.start);
+        pl->gcount.push_back(g.count);
+    }
     pl->ngto = b->ngto;
     pl->ng = b->ng;
     pl->maxl = maxl;
@@ -423,15 +430,27 @@ int run_transform_total(ThistlePlan* pl) {
                         pl->xa.as<int32_t>(), pl->xb.as<int32_t>(), pl->npp, pl->T.as<double>(),
                         pl->stream));
     const double one = 1.0, zero = 0.0;
+    const rocblas_int npp = (rocblas_int)pl->npp;
+    if (pl->zfact) {
+        // M^T = T^T Gs^T  (npp x npmo), then Z through the MO factorisation:
+        // 2 npp norb ngto (norb + ngto / 2) flops instead of npp^2 npmo
+        BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_transpose, rocblas_operation_transpose,
+                             npp, npmo, npmo, &one, pl->T.as<double>(), npmo,
+                             pl->Gs.as<double>(), npmo, &zero, pl->M.as<double>(), npp));
+        HCHECK(launch_zfact(pl->M.as<double>(), pl->mmod.as<double>(), pl->nmo, norb, pl->ngto,
+                            pl->npp, pl->jtab.as<int32_t>(), pl->aend.as<int32_t>(),
+                            pl->Cp.as<double>(), pl->W3.as<double>(), pl->Z.as<double>(),
+                            pl->stream));
+        return THISTLE_OK;
+    }
     // M = Gs * T   (npmo x npp)
-    BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_none, rocblas_operation_none, npmo,
-                         (rocblas_int)pl->npp, npmo, &one, pl->Gs.as<double>(), npmo,
-                         pl->T.as<double>(), npmo, &zero, pl->M.as<double>(), npmo));
+    BCHECK(rocblas_dgemm(pl->blas, rocblas_operation_none, rocblas_operation_none, npmo, npp,
+                         npmo, &one, pl->Gs.as<double>(), npmo, pl->T.as<double>(), npmo, &zero,
+                         pl->M.as<double>(), npmo));
     // Z = T^T * M  (npp x npp, symmetric): lower triangle only
-    BCHECK(rocblas_dsyrkx(pl->blas, rocblas_fill_lower, rocblas_operation_transpose,
-                          (rocblas_int)pl->npp, npmo, &one, pl->T.as<double>(), npmo,
-                          pl->M.as<double>(), npmo, &zero, pl->Z.as<double>(),
-                          (rocblas_int)pl->npp));
+    BCHECK(rocblas_dsyrkx(pl->blas, rocblas_fill_lower, rocblas_operation_transpose, npp, npmo,
+                          &one, pl->T.as<double>(), npmo, pl->M.as<double>(), npmo, &zero,
+                          pl->Z.as<double>(), npp));
     return THISTLE_OK;
 }
 
@@ -579,6 +598,47 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
         if ((rc = dalloc(pl->T, sizeof(double) * (size_t)pl->npmo * pl->npp))) return rc;
         if ((rc = dalloc(pl->M, sizeof(double) * (size_t)pl->npmo * pl->npp))) return rc;
         if ((rc = dalloc(pl->Z, sizeof(double) * (size_t)pl->npp * pl->npp))) return rc;
+        // Z through the MO factorisation of T while W3(p, :, :) fits in
+        // registers (norb <= 64); THISTLE_ZFACT=0|1 overrides the choice
+        pl->zfact = zfact_width(pl->norb) > 0;
+        if (const char* ev = std::getenv("THISTLE_ZFACT")) {
+            if (ev[0] == '0') pl->zfact = false;
+            if (ev[0] == '1') pl->zfact = zfact_width(pl->norb) > 0;
+        }
+        if (pl->zfact) {
+            const int ngt = b->ngto;
+            if ((rc = dalloc(pl->W3, sizeof(double) * (size_t)pl->norb * ngt * pl->npp)))
+                return rc;
+            if ((rc = dalloc(pl->Cp, sizeof(double) * (size_t)ngt * zfact_width(pl->norb))))
+                return rc;
+            // per GTO b: the GTOs a that pair with it as (a, b) (groups up to
+            // b's own) and the primitive pair index of each
+            std::vector<int32_t> ae(ngt), jt((size_t)ngt * ngt, -1);
+            std::vector<int64_t> poff((size_t)pl->ng * pl->ng, 0);
+            int64_t o = 0;
+            for (int gi = 0; gi < pl->ng; ++gi)
+                for (int gj = gi; gj < pl->ng; ++gj) {
+                    poff[(size_t)gi * pl->ng + gj] = o;
+                    o += (int64_t)pl->gcount[gi] * pl->gcount[gj];
+                }
+            for (int gb = 0; gb < pl->ng; ++gb)
+                for (int bb = pl->gstart[gb]; bb < pl->gstart[gb] + pl->gcount[gb]; ++bb) {
+                    ae[bb] = pl->gstart[gb] + pl->gcount[gb];
+                    for (int ga = 0; ga <= gb; ++ga)
+                        for (int aa = pl->gstart[ga]; aa < pl->gstart[ga] + pl->gcount[ga]; ++aa)
+                            jt[(size_t)bb * ngt + aa] = (int32_t)(
+                                poff[(size_t)ga * pl->ng + gb] +
+                                (int64_t)(aa - pl->gstart[ga]) * pl->gcount[gb] +
+                                (bb - pl->gstart[gb]));
+                }
+            if ((rc = dalloc(pl->aend, sizeof(int32_t) * ngt))) return rc;
+            if ((rc = dalloc(pl->jtab, sizeof(int32_t) * jt.size()))) return rc;
+            HCHECK(hipMemcpyAsync(pl->aend.p, ae.data(), sizeof(int32_t) * ngt,
+                                  hipMemcpyHostToDevice, pl->stream));
+            HCHECK(hipMemcpyAsync(pl->jtab.p, jt.data(), sizeof(int32_t) * jt.size(),
+                                  hipMemcpyHostToDevice, pl->stream));
+            HCHECK(hipStreamSynchronize(pl->stream));
+        }
         if ((rc = dalloc(pl->mopairs, sizeof(int2) * mop.size()))) return rc;
         HCHECK(hipMemcpyAsync(pl->mopairs.p, mop.data(), sizeof(int2) * mop.size(),
                               hipMemcpyHostToDevice, pl->stream));

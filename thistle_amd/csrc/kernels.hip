@@ -416,6 +416,124 @@ __global__ void k_zel(const double* __restrict__ mmod, int nmo, int norb,
 }
 
 // clears and copies as kernels, so that captured graphs hold no memset nodes
+// C(0:norb, a) padded with zeros to NO doubles per GTO a, so that the
+// factorised transform reads each column with uniform (scalar) loads.
+__global__ void k_cpad(const double* __restrict__ C, int nmo, int norb, int ngto, int NO,
+                       double* __restrict__ Cp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ngto * NO) return;
+    const int a = i / NO, p = i - a * NO;
+    Cp[i] = p < norb ? C[p + (int64_t)nmo * a] : 0.0;
+}
+
+// The MO-pair transform factorises because T((p,s),(a,b)) is a sum of
+// products of MO coefficients C_pa C_sb (the diagonal p == s counted twice,
+// as the ordered pairs (p,s) and (s,p) both reach it).  With M^T(i, x) =
+// sum_y T(y, i) Gs(x, y):
+//   W3(p, b, i) = sum_s Mo(p, s, i) C_sb,  Mo(p, s, i) = M^T(i, pair(p, s)) (x2 if p == s)
+//   Z(i, j)     = sum_p C_{p a_j} W3(p, b_j, i)      (j = primitive pair (a_j, b_j))
+// W3 is laid out [p][b][i] (i fastest).  One thread per i, one block row per p.
+template <int NO>
+__global__ void __launch_bounds__(256) k_w3(const double* __restrict__ Mt,
+                                            const double* __restrict__ Cp, int norb, int ngto,
+                                            int64_t npp, double* __restrict__ W3) {
+    const int p = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npp) return;
+    double m[NO];
+#pragma unroll
+    for (int s = 0; s < NO; ++s) {
+        m[s] = 0.0;
+        if (s < norb) {
+            const int x = p < s ? pair_index(p, s, norb) : pair_index(s, p, norb);
+            const double v = Mt[i + npp * x];
+            m[s] = s == p ? 2.0 * v : v;
+        }
+    }
+    double* out = W3 + i + npp * (int64_t)ngto * p;
+    for (int b = 0; b < ngto; ++b) {
+        const double* c = Cp + (int64_t)b * NO;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int s = 0; s < NO; s += 4) {
+            s0 = fma(m[s], c[s], s0);
+            s1 = fma(m[s + 1], c[s + 1], s1);
+            s2 = fma(m[s + 2], c[s + 2], s2);
+            s3 = fma(m[s + 3], c[s + 3], s3);
+        }
+        out[npp * b] = (s0 + s1) + (s2 + s3);
+    }
+}
+
+typedef const __attribute__((address_space(4))) double* cdouble_p;
+
+// Lower triangle Z(i, j), i >= j, column-major with leading dimension npp.
+// Block (i tile, GTO b): W3(:, b, i) stays in registers while the block walks
+// the columns j = jtab[b][a] (a < aend[b]; j increases with a) of the pairs
+// whose second GTO is b.  Writes are contiguous along i.
+template <int NO>
+__global__ void __launch_bounds__(256) k_zform(const double* __restrict__ W3,
+                                               const double* __restrict__ Cp,
+                                               const int32_t* __restrict__ jtab,
+                                               const int32_t* __restrict__ aend_of, int norb,
+                                               int ngto, int64_t npp, double* __restrict__ Z) {
+    const int b = blockIdx.y;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t ilast = i0 + blockDim.x - 1;
+    const int32_t* jb = jtab + (int64_t)b * ngto;
+    if (jb[0] > ilast) return;  // the whole tile lies above the diagonal
+    const int64_t i = i0 + threadIdx.x;
+    const bool live = i < npp;
+    double w[NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+        w[p] = (live && p < norb) ? W3[i + npp * (b + (int64_t)ngto * p)] : 0.0;
+    const int aend = aend_of[b];
+    // C through the constant address space: uniform reads become scalar
+    // loads even where the compiler sinks the sums into the divergent store
+    const cdouble_p Cc = (cdouble_p)Cp;
+    for (int a = 0; a < aend; ++a) {
+        const int64_t j = jb[a];
+        if (j > ilast) break;
+        const cdouble_p c = Cc + (int64_t)a * NO;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int p = 0; p < NO; p += 4) {
+            s0 = fma(w[p], c[p], s0);
+            s1 = fma(w[p + 1], c[p + 1], s1);
+            s2 = fma(w[p + 2], c[p + 2], s2);
+            s3 = fma(w[p + 3], c[p + 3], s3);
+        }
+        if (live && i >= j) Z[i + npp * j] = (s0 + s1) + (s2 + s3);
+    }
+}
+
+int zfact_width(int norb) {
+    return norb <= 8 ? 8 : norb <= 16 ? 16 : norb <= 32 ? 32 : norb <= 48 ? 48 : norb <= 64 ? 64 : 0;
+}
+
+hipError_t launch_zfact(const double* Mt, const double* C, int nmo, int norb, int ngto,
+                        int64_t npp, const int32_t* jtab, const int32_t* aend_of, double* Cp,
+                        double* W3, double* Z, hipStream_t s) {
+    const int NO = zfact_width(norb);
+    if (NO == 0 || npp <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cpad, dim3((ngto * NO + 255) / 256), dim3(256), 0, s, C, nmo, norb,
+                       ngto, NO, Cp);
+    const unsigned nt = (unsigned)((npp + 255) / 256);
+#define ZF(NOV)                                                                              \
+    case NOV:                                                                                \
+        hipLaunchKernelGGL(k_w3<NOV>, dim3(nt, (unsigned)norb), dim3(256), 0, s, Mt, Cp, norb, \
+                           ngto, npp, W3);                                                   \
+        hipLaunchKernelGGL(k_zform<NOV>, dim3(nt, (unsigned)ngto), dim3(256), 0, s, W3, Cp,   \
+                           jtab, aend_of, norb, ngto, npp, Z);                               \
+        break;
+    switch (NO) {
+        ZF(8) ZF(16) ZF(32) ZF(48) ZF(64)
+    }
+#undef ZF
+    return hipGetLastError();
+}
+
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;

@@ -130,20 +130,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
         int any = 0;
         for (uint32_t u = lg; u < nq4; u += G) {
             const uint64_t w = qt[u];
-            double z;
-            if (!A.elastic) {
-                const int64_t xb = PB.off + (int)(w & 255u);
-                const int64_t xk = PK.off + (int)((w >> 8) & 255u);
-                // Z is symmetric and only its lower triangle is stored; ket
-                // pairs mostly follow their bra, so xk >= xb reads contiguous rows
-                z = xk >= xb ? A.Z[xk + A.ldz * xb] : A.Z[xb + A.ldz * xk];
-            } else {
-                const int ia = Gi.start + (int)((w >> 46) & 15u),
-                          ib = Gj.start + (int)((w >> 50) & 15u),
-                          ic = Gk.start + (int)((w >> 54) & 15u),
-                          id = Gr.start + (int)((w >> 58) & 15u);
-                z = A.Z[ic + A.ldz * id] * A.Z[ia + A.ldz * ib];
-            }
+            const double z = load_z(A, PB.off, PK.off, w, Gi, Gj, Gk, Gr);
             if (fabs(z) < A.cutoffz) continue;
             any = 1;
             const int tx = (int)((w >> 16) & 1023u), ty = (int)((w >> 26) & 1023u),

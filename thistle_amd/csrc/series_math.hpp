@@ -148,3 +148,20 @@ struct SeriesRegime {
     static constexpr int nstart = LL + 16 + (int)(0.9 * LL + 2.0);
 };
 
+// ztot of one GTO quadruple from the quadruple-table entry w (pair-block
+// offsets in bits 0-15, component indices in bits 46-61).
+__device__ __forceinline__ double load_z(const SeriesArgs& A, int64_t offb, int64_t offk,
+                                         uint64_t w, const GroupInfo& Gi, const GroupInfo& Gj,
+                                         const GroupInfo& Gk, const GroupInfo& Gr) {
+    if (A.elastic) {
+        const int ia = Gi.start + (int)((w >> 46) & 15u), ib = Gj.start + (int)((w >> 50) & 15u),
+                  ic = Gk.start + (int)((w >> 54) & 15u), id = Gr.start + (int)((w >> 58) & 15u);
+        return A.Z[ic + A.ldz * id] * A.Z[ia + A.ldz * ib];
+    }
+    const int64_t xk = offk + (int)((w >> 8) & 255u);
+    // lower triangle; ket pairs mostly follow their bra, so xk >= xb reads
+    // contiguous rows
+    const int64_t xb = offb + (int)(w & 255u);
+    return xk >= xb ? A.Z[xk + A.ldz * xb] : A.Z[xb + A.ldz * xk];
+}
+

@@ -63,7 +63,8 @@ struct SeriesArgs {
     int nq, q0, nqc;          // full grid size, chunk start, chunk length
     const double* e12;        // [npair][nq] fp64
     const float* e12f;        // [npair][nq] fp32 (mixed mode)
-    const double* Z;          // total: NPP x NPP; elastic: ngto x ngto
+    const double* Z;          // total: NPP x NPP lower triangle;
+                              // elastic: ngto x ngto
     const float* Zf;          // fp32 copy (mixed mode), may be null
     int64_t ldz;
     int elastic, mixed;
@@ -134,6 +135,13 @@ hipError_t launch_gs_matrix(const double* V, int norb, const int2* mopairs,
                             int npmo, double* Gs, hipStream_t s);
 hipError_t launch_zel(const double* mmod, int nmo, int norb,
                       const double* onerdm, int ngto, double* Z, hipStream_t s);
+// Lower-triangle Z = T^T Gs T through the MO factorisation of T (norb <= 64):
+// Mt = (Gs T)^T (npp x npmo), Cp scratch ngto x zfact_width(norb), W3 scratch
+// norb x ngto x npp, jtab[b][a] the primitive pair of GTOs (a, b) for a < aend[b]
+int zfact_width(int norb);
+hipError_t launch_zfact(const double* Mt, const double* C, int nmo, int norb, int ngto,
+                        int64_t npp, const int32_t* jtab, const int32_t* aend_of, double* Cp,
+                        double* W3, double* Z, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(const double* in, double* out, int64_t n, hipStream_t s);
 hipError_t launch_to_float(const double* in, float* out, int64_t n,
