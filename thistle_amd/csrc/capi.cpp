@@ -493,6 +493,7 @@ int run_series(ThistlePlan* pl, bool timing) {
         int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
         if (n <= 0) continue;
         int blocks = (int)std::min<int64_t>(LL < 10 ? 16384 : 2048, n);
+        a.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
         HCHECK(launch_contract(a, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
                                n, blocks, pl->stream));
     }
@@ -506,6 +507,7 @@ int run_series(ThistlePlan* pl, bool timing) {
             int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
             if (n <= 0) continue;
             int w = (int)std::min<int64_t>(pl->waves, n);
+            a.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
             // j2 carries its series two orders further (bessels0rr(LLmax+2))
             HCHECK(launch_series(a, a.j2 ? LL + 2 : LL, pl->cls.as<int32_t>() + pl->cls_off[LL], n,
                                  w, 0, pl->stream));
@@ -721,6 +723,7 @@ int run_adjoint(ThistlePlan* pl, const double* w_host, double* gg_host, double* 
         bool glob;
         int blocks;
         shape(LL, n, glob, blocks);
+        j.s.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
         HCHECK(launch_contract_adj(j, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
                                    n, blocks, glob, lds_cap, pl->stream));
     }

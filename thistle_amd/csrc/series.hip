@@ -58,6 +58,9 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
     double* red = sm + (size_t)NG * 3 * tmax * D;  // [NA][NW][D] when a quartet spans waves
     const int L1 = A.maxl + 1, T1 = 2 * A.maxl + 1;
 
+    // grid-stride over the class list: neighbouring quartets go to
+    // neighbouring workgroups, hence to different XCDs (measured faster on
+    // XY6 than contiguous runs per workgroup or per XCD)
     for (int64_t base = (int64_t)blockIdx.x * NG; base < nidx; base += (int64_t)gridDim.x * NG) {
         const int64_t kq = base + grp;
         const bool valid = kq < nidx;
@@ -213,7 +216,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
 #pragma unroll
                 for (int b = 0; b < DO; ++b)
                     if (lg == b) mine = coef(b, h[0], h[NA - 1]);
-                if (lg < DO) A.hs[e * HS + lg] = mine;
+                if (lg < DO) A.hs[kq * DO + lg] = mine;
                 if (lg == 0) A.flags[e] = (bal & gmask) ? 1 : 0;
             }
         } else {
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
 #pragma unroll
                 for (int b = 0; b < DO; ++b)
                     if (tid == b) mine = coef(b, hr, grr);
-                A.hs[e * HS + tid] = mine;
+                A.hs[kq * DO + tid] = mine;
             }
             if (valid && tid == 0) {
                 int f = 0;
@@ -272,10 +275,13 @@ __device__ __forceinline__ double f_pzero(const double (&hs)[LL + 1], double q) 
     return F;
 }
 
-// dphrec with start order 100 followed literally, then bessels0rr's sum
+// dphrec with start order 100 followed literally, then bessels0rr's sum.
+// Out of line and rarely taken; it reads the quartet's h(beta) row from
+// global memory, so that the caller's h(beta) registers are never spilled
+// to scratch to pass them.
 template <int LL>
-__device__ __noinline__ double f_dphrec(const double (&hs)[LL + 1], double q, double h, double x, double sx,
-                           double cx) {
+__device__ __noinline__ double f_dphrec(const double* __restrict__ hs, double q, double h,
+                                        double x, double sx, double cx) {
 #pragma clang fp contract(off)
     const double XX = x * x;
     double r2 = 1.0, r1 = 1.0;  // R[i+2], R[i+1] with R[99] = R[98] = 1
@@ -338,21 +344,25 @@ __device__ __forceinline__ double f_miller(const double (&hs)[LL + 1], double q,
     return acc * (j0 * rcp64(v));
 }
 
+// Upward recurrence on u_b = j_b(x) (q/h)^b, so that the series needs no
+// power of q/h: since (q/h)/x = 1/h^2,
+//   u_{b+1} = (2b+1)/h^2 u_b - (q/h)^2 u_{b-1},  u_0 = j_0,  u_1 = (j_0 - cos x)/h^2.
+// cb[b] = (2b+1)/h^2 is the quartet's (uniform over q).
 template <int LL>
-__device__ __forceinline__ double f_up64(const double (&hs)[LL + 1], double q, double invh,
-                                         double inv, double j0, double cx) {
+__device__ __forceinline__ double f_up64(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
+                                         double q, double invh, double j0, double cx) {
     double F = j0 * hs[0];
     if (LL >= 1) {
         const double r = q * invh;
-        double jm = j0, jc = (j0 - cx) * inv, rb = r;
-        F = fma(jc * rb, hs[1], F);
+        const double nr2 = -(r * r);
+        double um = j0, uc = (j0 - cx) * cb[0];
+        F = fma(uc, hs[1], F);
 #pragma unroll
         for (int b = 1; b < LL; ++b) {
-            double jn = fma((double)(2 * b + 1) * inv, jc, -jm);
-            jm = jc;
-            jc = jn;
-            rb *= r;
-            F = fma(jc * rb, hs[b + 1], F);
+            const double un = fma(cb[b], uc, nr2 * um);
+            um = uc;
+            uc = un;
+            F = fma(uc, hs[b + 1], F);
         }
     }
     return F;
@@ -400,8 +410,10 @@ __device__ __forceinline__ double f_pzero_j2(const double (&hs)[LL + 1], double 
     return F;
 }
 
+// qinv = 1/q (0 at q = 0), so that 1/x = invh * qinv needs no reciprocal.
 template <int LL, bool MIXED>
-__device__ __forceinline__ double f_series(const double (&hs)[LL + 1], double q, double h,
+__device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
+                                           const double* hsg, double q, double qinv, double h,
                                            double invh) {
     const double x = q * h;
     if (MIXED && x > (double)LL + 6.0 && x < XDPH) {
@@ -412,11 +424,11 @@ __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], double q,
     }
     double sx, cx;
     sincos_cw(x, sx, cx);
-    const double inv = rcp64(x);
+    const double inv = invh * qinv;
     // q = 0 (x = 0): the j0 limit 1 instead of 0/0
     const double j0 = x > 0.0 ? sx * inv : 1.0;
-    if (x >= XDPH) return f_dphrec<LL>(hs, q, h, x, sx, cx);
-    if (LL == 0 || x >= SeriesRegime<LL>::xup) return f_up64<LL>(hs, q, invh, inv, j0, cx);
+    if (x >= XDPH) return f_dphrec<LL>(hsg, q, h, x, sx, cx);
+    if (LL == 0 || x >= SeriesRegime<LL>::xup) return f_up64<LL>(hs, cb, q, invh, j0, cx);
     return f_miller<LL>(hs, q, x, j0);
 }
 
@@ -424,15 +436,16 @@ __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], double q,
 // F/mu**2, src/Integrals.f90:5415-5418), whose q -> 0 limit is c(1)/3 (the
 // reference's 0/0), or the j2 h = 0 polynomial.
 template <int LL, bool MIXED, bool J2>
-__device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], double q, double h,
+__device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
+                                         const double* hsg, double q, double qinv, double h,
                                          double invh, int br) {
     if constexpr (J2) {
         if (br) return f_pzero_j2<LL>(hs, q);
-        const double F = f_series<LL, MIXED>(hs, q, h, invh);
+        const double F = f_series<LL, MIXED>(hs, cb, hsg, q, qinv, h, invh);
         return q > 0.0 ? F / (q * q) : (LL >= 1 ? hs[LL >= 1 ? 1 : 0] * (1.0 / 3.0) : 0.0);
     } else {
         if (br) return f_pzero<LL>(hs, q);
-        return f_series<LL, MIXED>(hs, q, h, invh);
+        return f_series<LL, MIXED>(hs, cb, hsg, q, qinv, h, invh);
     }
 }
 
@@ -441,19 +454,19 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
                                                  int64_t nidx, int first) {
     const int lane = threadIdx.x;
     int iqs[SQ];
-    double qv[SQ], S[SQ], Sc[SQ];
+    double qv[SQ], qiv[SQ], S[SQ], Sc[SQ];
 #pragma unroll
     for (int s = 0; s < SQ; ++s) {
         const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
         iqs[s] = A.q0 + (iql < A.nqc ? iql : 0);
         qv[s] = A.q[iqs[s]];
+        qiv[s] = qv[s] > 0.0 ? rcp64(qv[s]) : 0.0;
         S[s] = Sc[s] = 0.0;
     }
     // a contiguous chunk of the class list per wave: consecutive quartets
     // of one nest share their bra pair, whose e12 row stays in registers
-    const int64_t per = (nidx + gridDim.x - 1) / gridDim.x;
-    const int64_t k0 = (int64_t)blockIdx.x * per;
-    const int64_t k1 = k0 + per < nidx ? k0 + per : nidx;
+    const int64_t k0 = (int64_t)blockIdx.x * nidx / gridDim.x,
+                  k1 = ((int64_t)blockIdx.x + 1) * nidx / gridDim.x;
     int pbl = -1;
     double ebv[SQ];
     for (int64_t kq = k0; kq < k1; ++kq) {
@@ -467,9 +480,13 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
                      hz = A.P[pk].Pz - A.P[pb].Pz;
         const double h = sqrt(hx * hx + hy * hy + hz * hz);
         const double invh = br ? 0.0 : rcp64(h);
+        const double ih2 = invh * invh;
+        double cb[LL + 1];
+#pragma unroll
+        for (int b = 0; b <= LL; ++b) cb[b] = (double)(2 * b + 1) * ih2;
         double hs[LL + 1];
 #pragma unroll
-        for (int b = 0; b <= LL; ++b) hs[b] = A.hs[e * HS + b];
+        for (int b = 0; b <= LL; ++b) hs[b] = A.hs[kq * (LL + 1) + b];
         if (pb != pbl) {
             const double* eb = A.e12 + (int64_t)pb * A.nq;
 #pragma unroll
@@ -479,7 +496,8 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
         const double* ek = A.e12 + (int64_t)pk * A.nq;
 #pragma unroll
         for (int s = 0; s < SQ; ++s) {
-            double v = mult * f_eval<LL, MIXED, J2>(hs, qv[s], h, invh, br);
+            double v = mult * f_eval<LL, MIXED, J2>(hs, cb, A.hs + kq * (LL + 1), qv[s], qiv[s], h,
+                                                    invh, br);
             v = v * ebv[s];
             v = v * ek[iqs[s]];
             kahan(S[s], Sc[s], v);

@@ -10,7 +10,6 @@
 
 #define WAVE 64
 
-#define HS 17       // stride of the h(beta) buffer (LL <= 16)
 #define XDPH 74.0   // q h from which dphrec's start-100 recurrence is followed
                     // step by step (it departs from the exact j_beta near 76)
 

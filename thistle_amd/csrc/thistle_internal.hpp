@@ -35,6 +35,11 @@ struct EnumRow {
     int64_t out;            // first output slot
 };
 
+// h(beta) buffer: the rows of LL class c start at (class start) * HS doubles
+// and are packed at the stride of the class's kernels (LL + 1, j2 LL + 3),
+// in class-list order.
+#define HS 17
+
 static inline __host__ __device__ int pair_index(int gi, int gj, int ng) {
     // gi <= gj, row-major upper triangle
     return gi * ng - (gi * (gi - 1)) / 2 + (gj - gi);
