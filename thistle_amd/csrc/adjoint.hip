@@ -341,11 +341,12 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
         double acc[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        const ZView ZV = zview(A, pb, pk, PB.off, PK.off, Gi, Gj, Gk, Gr);
         for (uint32_t u = lg; u < nq4; u += G) {
             const uint64_t w = qt[u];
             const int64_t xb = PB.off + (int)(w & 255u);
             const int64_t xk = PK.off + (int)((w >> 8) & 255u);
-            const double z = load_z(A, PB.off, PK.off, w, Gi, Gj, Gk, Gr);
+            const double z = zget(A, ZV, w);
             if (fabs(z) < A.cutoffz) continue;
             const int tt[3] = {(int)((w >> 16) & 1023u), (int)((w >> 26) & 1023u),
                                (int)((w >> 36) & 1023u)};

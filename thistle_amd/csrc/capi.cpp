@@ -300,7 +300,8 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     // GTO-quadruple tables of the signature combinations in use.  Entry u of
     // a combination (d fastest): bits 0-7 a*cj+b, 8-15 c*cr+d (offsets in the
     // bra / ket pair blocks), 16-25 / 26-35 / 36-45 the x / y / z exponent
-    // tuple index, 46-61 a, b, c, d (4 bits each).
+    // tuple index, 46-61 a, b, c, d (4 bits each).  The ket offset runs
+    // fastest, so that a wave's Z reads fall on few cache lines.
     std::vector<int64_t> comboff((size_t)S * S * S * S, 0);
     std::vector<uint64_t> qtab;
     for (int si = 0; si < S; ++si)
@@ -599,6 +600,9 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
         if ((rc = dalloc(pl->Gs, sizeof(double) * (size_t)pl->npmo * pl->npmo))) return rc;
         if ((rc = dalloc(pl->T, sizeof(double) * (size_t)pl->npmo * pl->npp))) return rc;
         if ((rc = dalloc(pl->M, sizeof(double) * (size_t)pl->npmo * pl->npp))) return rc;
+        // block-local Z indices are 24-bit products (npp^2 doubles could not
+        // be allocated beyond that anyway)
+        if (pl->npp >= ((int64_t)1 << 24)) return THISTLE_ENOMEM;
         if ((rc = dalloc(pl->Z, sizeof(double) * (size_t)pl->npp * pl->npp))) return rc;
         // Z through the MO factorisation of T while W3(p, :, :) fits in
         // registers (norb <= 64); THISTLE_ZFACT=0|1 overrides the choice

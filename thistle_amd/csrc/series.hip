@@ -131,16 +131,14 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
 #pragma unroll
             for (int b = 0; b < D; ++b) h[k][b] = 0.0;
         int any = 0;
-        for (uint32_t u = lg; u < nq4; u += G) {
-            const uint64_t w = qt[u];
-            const double z = load_z(A, PB.off, PK.off, w, Gi, Gj, Gk, Gr);
-            if (fabs(z) < A.cutoffz) continue;
-            any = 1;
+        const ZView ZV = zview(A, pb, pk, PB.off, PK.off, Gi, Gj, Gk, Gr);
+        // one GTO quadruple: h(beta) += z * (px * py * pz)(beta)
+        auto quad = [&](uint64_t w, double z) {
             const int tx = (int)((w >> 16) & 1023u), ty = (int)((w >> 26) & 1023u),
                       tz = (int)((w >> 36) & 1023u);
-            const double* px = PX + (int64_t)tx * D;
-            const double* py = PX + ((int64_t)T + ty) * D;
-            const double* pz = PX + ((int64_t)2 * T + tz) * D;
+            const double* px = PX + tx * D;
+            const double* py = PX + (T + ty) * D;
+            const double* pz = PX + (2 * T + tz) * D;
             double vy[D], vz[D], vx[D];
 #pragma unroll
             for (int m = 0; m < D; ++m) {
@@ -179,6 +177,29 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
                     for (int mx = 0; mx <= b; ++mx) s += vx[mx] * yzm[b - mx];
                     h[NA - 1][b] = fma(z, s, h[NA - 1][b]);
                 }
+            }
+        };
+        // ZB quadruples per lane per step: their Z gathers are issued together
+        // (the lane's quadruples are still accumulated in increasing u)
+        constexpr int ZB = 2;
+        for (uint32_t u0 = lg; u0 < nq4; u0 += ZB * G) {
+            uint64_t wv[ZB];
+            double zv[ZB];
+#pragma unroll
+            for (int k = 0; k < ZB; ++k) {
+                const uint32_t u = u0 + k * G;
+                wv[k] = u < nq4 ? qt[u] : 0ull;
+            }
+#pragma unroll
+            for (int k = 0; k < ZB; ++k) {
+                const uint32_t u = u0 + k * G;
+                zv[k] = u < nq4 ? zget(A, ZV, wv[k]) : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < ZB; ++k) {
+                if (u0 + k * G >= nq4 || fabs(zv[k]) < A.cutoffz) continue;
+                any = 1;
+                quad(wv[k], zv[k]);
             }
         }
         // the values of h(beta) (j0) or of the j2 series / h = 0 polynomial

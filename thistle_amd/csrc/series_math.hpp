@@ -147,20 +147,54 @@ struct SeriesRegime {
     static constexpr int nstart = LL + 16 + (int)(0.9 * LL + 2.0);
 };
 
+// Z of one quartet.  Total: the lower-triangle elements (max(xb,xk),
+// min(xb,xk)) of bra element xb and ket element xk all lie in one block of
+// the npp x npp matrix, so an element is base[row + ldz * col] with 32-bit
+// block-local indices (npp < 2^24).  Elastic: the GTO starts of the four
+// shells in the ngto x ngto one-particle Z.
+struct ZView {
+    const double* base;
+    int ldz;
+    int mode;            // total: 0 ket on rows, 1 bra on rows, 2 same pair
+    int sa, sb, sc, sd;  // elastic
+};
+
+__device__ __forceinline__ ZView zview(const SeriesArgs& A, int pb, int pk, int64_t offb,
+                                       int64_t offk, const GroupInfo& Gi, const GroupInfo& Gj,
+                                       const GroupInfo& Gk, const GroupInfo& Gr) {
+    ZView V;
+    V.ldz = (int)A.ldz;
+    V.sa = Gi.start;
+    V.sb = Gj.start;
+    V.sc = Gk.start;
+    V.sd = Gr.start;
+    if (A.elastic) {
+        V.base = A.Z;
+        V.mode = 0;
+    } else if (pk > pb) {
+        V.base = A.Z + offk + A.ldz * offb;
+        V.mode = 0;
+    } else if (pk < pb) {
+        V.base = A.Z + offb + A.ldz * offk;
+        V.mode = 1;
+    } else {
+        V.base = A.Z + offb + A.ldz * offb;
+        V.mode = 2;
+    }
+    return V;
+}
+
 // ztot of one GTO quadruple from the quadruple-table entry w (pair-block
 // offsets in bits 0-15, component indices in bits 46-61).
-__device__ __forceinline__ double load_z(const SeriesArgs& A, int64_t offb, int64_t offk,
-                                         uint64_t w, const GroupInfo& Gi, const GroupInfo& Gj,
-                                         const GroupInfo& Gk, const GroupInfo& Gr) {
+__device__ __forceinline__ double zget(const SeriesArgs& A, const ZView& V, uint64_t w) {
     if (A.elastic) {
-        const int ia = Gi.start + (int)((w >> 46) & 15u), ib = Gj.start + (int)((w >> 50) & 15u),
-                  ic = Gk.start + (int)((w >> 54) & 15u), id = Gr.start + (int)((w >> 58) & 15u);
-        return A.Z[ic + A.ldz * id] * A.Z[ia + A.ldz * ib];
+        const int ia = V.sa + (int)((w >> 46) & 15u), ib = V.sb + (int)((w >> 50) & 15u),
+                  ic = V.sc + (int)((w >> 54) & 15u), id = V.sd + (int)((w >> 58) & 15u);
+        return V.base[ic + (int64_t)V.ldz * id] * V.base[ia + (int64_t)V.ldz * ib];
     }
-    const int64_t xk = offk + (int)((w >> 8) & 255u);
-    // lower triangle; ket pairs mostly follow their bra, so xk >= xb reads
-    // contiguous rows
-    const int64_t xb = offb + (int)(w & 255u);
-    return xk >= xb ? A.Z[xk + A.ldz * xb] : A.Z[xb + A.ldz * xk];
+    const int bl = (int)(w & 255u), kl = (int)((w >> 8) & 255u);
+    const bool swap = V.mode == 1 || (V.mode == 2 && bl > kl);
+    const int r = swap ? bl : kl, c = swap ? kl : bl;
+    return V.base[(unsigned)r + __umul24((unsigned)V.ldz, (unsigned)c)];
 }
 
