@@ -79,6 +79,8 @@ struct ThistlePlan {
     int ngto = 0, ng = 0, maxl = 0, nq = 0, nmo = 0, norb = 0;
     int npair = 0, npmo = 0, waves = 0, waves_alloc = 0, nrows = 0;
     bool zfact = false;  // total Z through the MO factorisation of T (else one dsyrkx)
+    int qlin = 0, q0off = 0;  // q[k] = qa + k dq for k >= 1; q[0] apart
+    double qa = 0.0, dq = 0.0;
     int64_t npp = 0, nquart = 0, nmat = 0, units = 0;
     double cutoffz = 0, cutoffmd = 0, cutoffcentre = 0;
     hipStream_t stream = nullptr;
@@ -369,6 +371,22 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
                           pl->stream));
     HCHECK(hipStreamSynchronize(pl->stream));
     pl->h_q.assign(b->q, b->q + b->nq);
+    // a uniform grid from q[1] on (q[0] may be set apart, as 1e-10 is by the
+    // reference's driver) lets the series kernel rotate sin/cos across q
+    if (b->nq >= 3) {
+        const double* q = b->q;
+        const int n = b->nq;
+        const double dq = (q[n - 1] - q[1]) / (n - 2), qa = q[1] - dq;
+        double qmax = 0.0;
+        for (int k = 0; k < n; ++k) qmax = std::max(qmax, std::fabs(q[k]));
+        const double tol = 8.0 * 2.220446049250313e-16 * qmax;
+        bool lin = dq > 0.0 && std::isfinite(dq);
+        for (int k = 1; lin && k < n; ++k) lin = std::fabs(q[k] - (qa + k * dq)) <= tol;
+        pl->qlin = lin ? 1 : 0;
+        pl->q0off = lin && std::fabs(q[0] - qa) > tol ? 1 : 0;
+        pl->qa = qa;
+        pl->dq = dq;
+    }
     HCHECK(hipMemcpyAsync(pl->G.p, groups.data(), sizeof(GroupInfo) * ng, hipMemcpyHostToDevice, pl->stream));
     HCHECK(hipMemcpyAsync(pl->P.p, pairs.data(), sizeof(PairInfo) * pl->npair, hipMemcpyHostToDevice, pl->stream));
     HCHECK(hipMemcpyAsync(pl->q.p, b->q, sizeof(double) * b->nq, hipMemcpyHostToDevice, pl->stream));
@@ -484,6 +502,10 @@ SeriesArgs series_args(const ThistlePlan* pl) {
     a.comboff = pl->comboff.as<int64_t>();
     a.nsig = pl->nsig;
     a.nqc = std::min(512, pl->nq);
+    a.qlin = pl->qlin;
+    a.q0off = pl->q0off;
+    a.qa = pl->qa;
+    a.dq = pl->dq;
     return a;
 }
 
@@ -517,12 +539,9 @@ int run_series(ThistlePlan* pl, bool timing) {
                              pl->stream));
     }
     if (timing) {
-        // the screened quartet list (thistle_quartet_list_screened) is a
-        // by-product of single-plan executions
+        // stage 4 is empty since the screened quartet list is compacted on
+        // demand (thistle_quartet_list_screened) from the flags kept here
         HCHECK(hipEventRecord(pl->ev[4], pl->stream));
-        HCHECK(launch_compact(pl->flags.as<uint8_t>(), pl->qlist.as<uint64_t>(), pl->nquart,
-                              pl->scan.as<int64_t>(), pl->qcomp.as<uint64_t>(),
-                              pl->count.as<int64_t>(), pl->stream));
         HCHECK(hipEventRecord(pl->ev[5], pl->stream));
     }
     pl->units = pl->nquart * (int64_t)pl->nq;
@@ -1194,6 +1213,12 @@ int64_t thistle_quartet_list_screened(int32_t type, const ThistleBasis* basis, i
     std::unique_ptr<ThistlePlan> pl(raw);
     rc = thistle_plan_execute(pl.get(), nullptr);
     if (rc) return -rc;
+    // quartets with a nonzero 2-RDM block, in reference order
+    if (launch_compact(pl->flags.as<uint8_t>(), pl->qlist.as<uint64_t>(), pl->nquart,
+                       pl->scan.as<int64_t>(), pl->qcomp.as<uint64_t>(), pl->count.as<int64_t>(),
+                       pl->stream) != hipSuccess ||
+        hipStreamSynchronize(pl->stream) != hipSuccess)
+        return -THISTLE_EDEVICE;
     int64_t cnt = 0;
     if (hipMemcpy(&cnt, pl->count.p, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
         return -THISTLE_EDEVICE;

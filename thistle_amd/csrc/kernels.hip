@@ -215,16 +215,32 @@ __global__ void k_scan_blocks(const uint8_t* __restrict__ in, int64_t n,
     if (threadIdx.x == SCAN_T - 1) bsum[blockIdx.x] = s[threadIdx.x];
 }
 
-__global__ void k_scan_sums(int64_t* __restrict__ bsum, int64_t nb,
-                            int64_t* __restrict__ total) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Exclusive scan of the nb block sums in place, total to *total.  One
+// workgroup of SCAN_SUMS_T threads: a contiguous run of block sums per thread,
+// then a scan of the run totals.
+#define SCAN_SUMS_T 1024
+__global__ void __launch_bounds__(SCAN_SUMS_T) k_scan_sums(int64_t* __restrict__ bsum, int64_t nb,
+                                                          int64_t* __restrict__ total) {
+    __shared__ int64_t s[SCAN_SUMS_T];
+    const int t = threadIdx.x;
+    const int64_t b0 = (int64_t)t * nb / SCAN_SUMS_T, b1 = ((int64_t)t + 1) * nb / SCAN_SUMS_T;
     int64_t run = 0;
-    for (int64_t b = 0; b < nb; ++b) {
-        int64_t t = bsum[b];
-        bsum[b] = run;
-        run += t;
+    for (int64_t b = b0; b < b1; ++b) run += bsum[b];
+    s[t] = run;
+    __syncthreads();
+    for (int off = 1; off < SCAN_SUMS_T; off <<= 1) {
+        const int64_t v = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
     }
-    *total = run;
+    int64_t acc = s[t] - run;  // exclusive prefix of this thread's run
+    for (int64_t b = b0; b < b1; ++b) {
+        const int64_t v = bsum[b];
+        bsum[b] = acc;
+        acc += v;
+    }
+    if (t == SCAN_SUMS_T - 1) *total = s[t];
 }
 
 __global__ void k_scan_add(int64_t* __restrict__ out, int64_t n,
@@ -550,17 +566,38 @@ __global__ void k_to_float(const double* __restrict__ in, float* __restrict__ ou
     if (i < n) out[i] = (float)in[i];
 }
 
-__global__ void k_reduce(const double* __restrict__ part,
-                         const double* __restrict__ partc, int waves, int nqc,
-                         double* __restrict__ out) {
-    int iq = blockIdx.x * blockDim.x + threadIdx.x;
-    if (iq >= nqc) return;
+// S(q) from the per-wave compensated partial sums, in a fixed order: the
+// waves are split into RED_SEG fixed segments, each summed (Kahan) by one
+// thread, and the segment sums are combined in segment order.
+#define RED_SEG 32
+#define RED_Q 8
+__global__ void __launch_bounds__(RED_SEG * RED_Q) k_reduce(const double* __restrict__ part,
+                                                           const double* __restrict__ partc,
+                                                           int waves, int nqc,
+                                                           double* __restrict__ out) {
+    __shared__ double ss[RED_SEG][RED_Q], sc[RED_SEG][RED_Q];
+    const int ql = threadIdx.x % RED_Q, seg = threadIdx.x / RED_Q;
+    const int iq = blockIdx.x * RED_Q + ql;
     double s = 0.0, c = 0.0;
-    for (int w = 0; w < waves; ++w) {
-        kahan_add(s, c, part[(int64_t)w * nqc + iq]);
-        kahan_add(s, c, -partc[(int64_t)w * nqc + iq]);
+    if (iq < nqc) {
+        const int w0 = (int)((int64_t)seg * waves / RED_SEG),
+                  w1 = (int)((int64_t)(seg + 1) * waves / RED_SEG);
+        for (int w = w0; w < w1; ++w) {
+            kahan_add(s, c, part[(int64_t)w * nqc + iq]);
+            kahan_add(s, c, -partc[(int64_t)w * nqc + iq]);
+        }
     }
-    out[iq] = s;
+    ss[seg][ql] = s;
+    sc[seg][ql] = c;
+    __syncthreads();
+    if (seg == 0 && iq < nqc) {
+        double S = 0.0, C = 0.0;
+        for (int g = 0; g < RED_SEG; ++g) {
+            kahan_add(S, C, ss[g][ql]);
+            kahan_add(S, C, -sc[g][ql]);
+        }
+        out[iq] = S;
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -600,7 +637,8 @@ hipError_t launch_unpack_quartets(const uint64_t* in, int64_t n, int32_t* out, h
 
 hipError_t launch_reduce(const double* part, const double* partc, int waves, int nqc,
                          double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce, dim3(nblk(nqc, 128)), dim3(128), 0, s, part, partc, waves, nqc, out);
+    hipLaunchKernelGGL(k_reduce, dim3(nblk(nqc, RED_Q)), dim3(RED_SEG * RED_Q), 0, s, part, partc,
+                       waves, nqc, out);
     return hipGetLastError();
 }
 
@@ -611,7 +649,7 @@ hipError_t launch_exclusive_scan_u8(const uint8_t* in, int64_t n, int64_t* out,
     if (nb == 0) nb = 1;
     int64_t* bsum = out + n;
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, n, out, bsum);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1), 0, s, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_SUMS_T), 0, s, bsum, nb, total);
     if (n > 0)
         hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, n, bsum);
     return hipGetLastError();

@@ -432,10 +432,11 @@ __device__ __forceinline__ double f_pzero_j2(const double (&hs)[LL + 1], double 
 }
 
 // qinv = 1/q (0 at q = 0), so that 1/x = invh * qinv needs no reciprocal.
+// sx, cx = sin, cos of x = q h, computed by the caller.
 template <int LL, bool MIXED>
 __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
                                            const double* hsg, double q, double qinv, double h,
-                                           double invh) {
+                                           double invh, double sx, double cx) {
     const double x = q * h;
     if (MIXED && x > (double)LL + 6.0 && x < XDPH) {
         float hf[LL + 1];
@@ -443,8 +444,6 @@ __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const dou
         for (int b = 0; b <= LL; ++b) hf[b] = (float)hs[b];
         return f_up32<LL>(hf, q, invh, x);
     }
-    double sx, cx;
-    sincos_cw(x, sx, cx);
     const double inv = invh * qinv;
     // q = 0 (x = 0): the j0 limit 1 instead of 0/0
     const double j0 = x > 0.0 ? sx * inv : 1.0;
@@ -459,14 +458,14 @@ __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const dou
 template <int LL, bool MIXED, bool J2>
 __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
                                          const double* hsg, double q, double qinv, double h,
-                                         double invh, int br) {
+                                         double invh, int br, double sx, double cx) {
     if constexpr (J2) {
         if (br) return f_pzero_j2<LL>(hs, q);
-        const double F = f_series<LL, MIXED>(hs, cb, hsg, q, qinv, h, invh);
+        const double F = f_series<LL, MIXED>(hs, cb, hsg, q, qinv, h, invh, sx, cx);
         return q > 0.0 ? F / (q * q) : (LL >= 1 ? hs[LL >= 1 ? 1 : 0] * (1.0 / 3.0) : 0.0);
     } else {
         if (br) return f_pzero<LL>(hs, q);
-        return f_series<LL, MIXED>(hs, cb, hsg, q, qinv, h, invh);
+        return f_series<LL, MIXED>(hs, cb, hsg, q, qinv, h, invh, sx, cx);
     }
 }
 
@@ -515,10 +514,39 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
             pbl = pb;
         }
         const double* ek = A.e12 + (int64_t)pk * A.nq;
+        // sin, cos of q h on the lane's SQ q points.  On a uniform grid the
+        // points are x0 + s D, D = 64 dq h, so one sincos of x0, one of D and
+        // rotations give them all; a q[0] off the grid gets its own.
+        double sxs[SQ], cxs[SQ];
+#pragma unroll
+        for (int s = 0; s < SQ; ++s) sxs[s] = cxs[s] = 0.0;
+        if (!br) {
+            if (A.qlin) {
+                const bool off = A.q0off && iqs[0] == 0;
+                double sb, cbb;
+                sincos_cw(off ? A.qa * h : qv[0] * h, sb, cbb);
+                sxs[0] = sb;
+                cxs[0] = cbb;
+                if (off) sincos_cw(qv[0] * h, sxs[0], cxs[0]);
+                double sd, cd;
+                sincos_cw((double)WAVE * A.dq * h, sd, cd);
+#pragma unroll
+                for (int s = 1; s < SQ; ++s) {
+                    const double ns = fma(sb, cd, cbb * sd), nc = fma(cbb, cd, -(sb * sd));
+                    sb = ns;
+                    cbb = nc;
+                    sxs[s] = sb;
+                    cxs[s] = cbb;
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < SQ; ++s) sincos_cw(qv[s] * h, sxs[s], cxs[s]);
+            }
+        }
 #pragma unroll
         for (int s = 0; s < SQ; ++s) {
             double v = mult * f_eval<LL, MIXED, J2>(hs, cb, A.hs + kq * (LL + 1), qv[s], qiv[s], h,
-                                                    invh, br);
+                                                    invh, br, sxs[s], cxs[s]);
             v = v * ebv[s];
             v = v * ek[iqs[s]];
             kahan(S[s], Sc[s], v);

@@ -66,6 +66,9 @@ struct SeriesArgs {
     const char4* lxyz;
     const double* q;
     int nq, q0, nqc;          // full grid size, chunk start, chunk length
+    int qlin;                 // q[k] = qa + k dq for k >= 1 (dq > 0)
+    int q0off;                // ... and q[0] is not qa (e.g. 1e-10 set by hand)
+    double qa, dq;
     const double* e12;        // [npair][nq] fp64
     const float* e12f;        // [npair][nq] fp32 (mixed mode)
     const double* Z;          // total: NPP x NPP lower triangle;
