@@ -104,6 +104,23 @@ def build_workload(name):
     return w, inp, mat, tot
 
 
+def measured_traffic(workload, precision):
+    """HBM bytes per execution of the quartet stage, from the committed
+    FETCH_SIZE / WRITE_SIZE profile of this workload (copy-calibrated); None
+    when no such profile exists for it."""
+    if workload != "xy6" or precision != "fp64":
+        return None
+    path = os.path.join(ROOT, "profiles", "xy6_quartet_stage_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("quartet-stage HBM bytes per execution:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def group_L(inp):
     L = []
     for g in range(inp.ng):
@@ -311,7 +328,11 @@ def main():
                      "kernel": "quartet stage: k_contract<LL> + k_series<LL> over all LL "
                                "classes of one execution (HIP events on the plan stream)",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": measured_traffic(args.workload, chosen),
+                     "traffic_unit": "HBM bytes per execution of the stage (rocprofv3 "
+                                     "FETCH_SIZE + WRITE_SIZE, copy-calibrated, "
+                                     "profiles/xy6_quartet_stage_traffic.md)",
                      "algorithmic_flops": W, "flops_hermite": W_herm, "flops_series": W_ser,
                      "kernel_ms": qk_ms},
     }
