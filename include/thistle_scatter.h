@@ -133,6 +133,19 @@ int32_t thistle_twordm_dense(const int32_t *confs, int32_t nconf,
 int64_t thistle_twordm_select(const double *twordm, int32_t norb,
                               double cutoff, int32_t *mat, double *total,
                               int64_t cap);
+/* maxcoincidence + createtwordm + its selection loop on the GPU
+ * (src/twordms.f90:8-556): rows (p,s,q,r) and values twordm(p,q,r,s) with
+ * |value| >= cutoff, in the reference's p,q,r,s order and with its
+ * floating-point values (each entry sums its contributions in the
+ * reference's (c1, c2, statement) order).  mat is (cap,4) column-major.
+ * Returns the row count, or -status; rows are written only when
+ * cap >= count (pass cap = 0 to count).  Determinant lists that are not in
+ * the punch reader's canonical form (1 on even slots, 2 on odd) are built by
+ * the host accumulation of thistle_twordm_dense. */
+int64_t thistle_createtwordm(const int32_t *confs, int32_t nconf, int32_t lconfs,
+                             const double *civecs, int32_t nstates, int32_t state1,
+                             int32_t state2, double cutoff, const ThistleOpts *opts,
+                             int32_t *mat, double *total, int64_t cap);
 /* onerdm_creat / createonerdm (src/one_rdm.f90:27-222). */
 int32_t thistle_onerdm(const int32_t *confs, int32_t nconf, int32_t lconfs,
                        const double *civecs, int32_t nstates, int32_t state1,

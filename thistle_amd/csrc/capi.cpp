@@ -1133,6 +1133,164 @@ int64_t thistle_twordm_select(const double* twordm, int32_t norb, double cutoff,
                                       cap > 0 ? total : nullptr, cap);
 }
 
+namespace {
+
+// the device accumulation of createtwordm (rdm.hip); count of selected rows,
+// rows written to mat / total when cap >= count
+int64_t createtwordm_device(const int32_t* confs, int nconf, int lconfs, const double* c1v_h,
+                            const double* c2v_h, double cutoff, int device, int32_t* mat,
+                            double* total, int64_t cap) {
+#define RCHECK(x)                                                               \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) {                                                 \
+            std::fprintf(stderr, "thistle: %s failed: %s (%s:%d)\n", #x,        \
+                         hipGetErrorString(e_), __FILE__, __LINE__);            \
+            return -THISTLE_EDEVICE;                                            \
+        }                                                                       \
+    } while (0)
+#define RALLOC(b, n)                                        \
+    do {                                                    \
+        if (dalloc(b, n)) return -THISTLE_ENOMEM;           \
+    } while (0)
+    RCHECK(hipSetDevice(device));
+    hipStream_t st;
+    RCHECK(hipStreamCreate(&st));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    const int N = lconfs / 2, nw = (lconfs + 63) / 64;
+    const int64_t n4 = (int64_t)N * N * N * N;
+    DevBuf dconfs, masks, civ, cnt, off, c1s, c2s, nds, ccnt, coff, keys, vals, keys2, vals2,
+        temp, start, tw, flags, pos, rows, outv;
+    RALLOC(dconfs, sizeof(int32_t) * (size_t)nconf * lconfs);
+    RALLOC(masks, sizeof(uint64_t) * (size_t)nconf * nw);
+    RALLOC(civ, sizeof(double) * 2 * (size_t)nconf);
+    RCHECK(hipMemcpyAsync(dconfs.p, confs, dconfs.bytes, hipMemcpyHostToDevice, st));
+    RCHECK(hipMemcpyAsync(civ.p, c1v_h, sizeof(double) * nconf, hipMemcpyHostToDevice, st));
+    RCHECK(hipMemcpyAsync(civ.as<double>() + nconf, c2v_h, sizeof(double) * nconf,
+                          hipMemcpyHostToDevice, st));
+    RCHECK(rdm_pack(dconfs.as<int32_t>(), nconf, lconfs, nw, masks.as<uint64_t>(), st));
+    // 1. connected pairs in (c1, c2) order
+    RALLOC(cnt, sizeof(int64_t) * ((size_t)nconf + 1));
+    RALLOC(off, sizeof(int64_t) * ((size_t)nconf + 1));
+    RCHECK(launch_fill(reinterpret_cast<double*>(cnt.p), nconf + 1, 0.0, st));
+    RCHECK(rdm_pairs(masks.as<uint64_t>(), nconf, nw, 4, cnt.as<int64_t>(), nullptr, nullptr,
+                     nullptr, nullptr, st));
+    size_t tb = 0;
+    RCHECK(rdm_scan_i64(nullptr, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
+    RALLOC(temp, tb);
+    RCHECK(rdm_scan_i64(temp.p, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
+    int64_t np = 0;
+    RCHECK(hipMemcpyAsync(&np, off.as<int64_t>() + nconf, sizeof(int64_t), hipMemcpyDeviceToHost,
+                          st));
+    RCHECK(hipStreamSynchronize(st));
+    RALLOC(c1s, sizeof(int32_t) * (size_t)std::max<int64_t>(np, 1));
+    RALLOC(c2s, sizeof(int32_t) * (size_t)std::max<int64_t>(np, 1));
+    RALLOC(nds, (size_t)std::max<int64_t>(np, 1));
+    RCHECK(rdm_pairs(masks.as<uint64_t>(), nconf, nw, 4, nullptr, off.as<int64_t>(),
+                     c1s.as<int32_t>(), c2s.as<int32_t>(), nds.as<int8_t>(), st));
+    // 2.-3. contributions at their positions in the reference's order
+    RALLOC(ccnt, sizeof(int64_t) * (size_t)(np + 1));
+    RALLOC(coff, sizeof(int64_t) * (size_t)(np + 1));
+    RCHECK(launch_fill(reinterpret_cast<double*>(ccnt.p), np + 1, 0.0, st));
+    RCHECK(rdm_ccount(masks.as<uint64_t>(), nw, c1s.as<int32_t>(), c2s.as<int32_t>(),
+                      nds.as<int8_t>(), np, ccnt.as<int64_t>(), st));
+    tb = 0;
+    RCHECK(rdm_scan_i64(nullptr, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), np + 1, st));
+    if (tb > temp.bytes) RALLOC(temp, tb);
+    RCHECK(rdm_scan_i64(temp.p, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), np + 1, st));
+    int64_t nc = 0;
+    RCHECK(hipMemcpyAsync(&nc, coff.as<int64_t>() + np, sizeof(int64_t), hipMemcpyDeviceToHost,
+                          st));
+    RCHECK(hipStreamSynchronize(st));
+    const size_t ncs = (size_t)std::max<int64_t>(nc, 1);
+    RALLOC(keys, sizeof(uint32_t) * ncs);
+    RALLOC(vals, sizeof(double) * ncs);
+    RALLOC(keys2, sizeof(uint32_t) * ncs);
+    RALLOC(vals2, sizeof(double) * ncs);
+    RCHECK(rdm_cwrite(masks.as<uint64_t>(), nw, c1s.as<int32_t>(), c2s.as<int32_t>(),
+                      nds.as<int8_t>(), np, civ.as<double>(), civ.as<double>() + nconf,
+                      coff.as<int64_t>(), N, keys.as<uint32_t>(), vals.as<double>(), st));
+    // 4. stable sort by element
+    int keybits = 1;
+    while (((int64_t)1 << keybits) < n4) ++keybits;
+    tb = 0;
+    RCHECK(rdm_sort(nullptr, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
+                    vals2.as<double>(), nc, keybits, st));
+    if (tb > temp.bytes) RALLOC(temp, tb);
+    if (nc > 0)
+        RCHECK(rdm_sort(temp.p, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
+                        vals2.as<double>(), nc, keybits, st));
+    // 5. ordered sums per element
+    RALLOC(start, sizeof(int64_t) * (size_t)(n4 + 1));
+    RALLOC(tw, sizeof(double) * (size_t)n4);
+    RCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, n4, start.as<int64_t>(),
+                    tw.as<double>(), st));
+    // 6. selection in p, q, r, s order
+    const int64_t nb = (n4 + 2047) / 2048 + 1;
+    RALLOC(flags, (size_t)n4);
+    RALLOC(pos, sizeof(int64_t) * (size_t)(n4 + nb + 1));
+    RCHECK(rdm_flags(tw.as<double>(), N, cutoff, flags.as<uint8_t>(), st));
+    RCHECK(launch_exclusive_scan_u8(flags.as<uint8_t>(), n4, pos.as<int64_t>(),
+                                    pos.as<int64_t>() + n4 + nb, st));
+    int64_t nsel = 0;
+    RCHECK(hipMemcpyAsync(&nsel, pos.as<int64_t>() + n4 + nb, sizeof(int64_t),
+                          hipMemcpyDeviceToHost, st));
+    RCHECK(hipStreamSynchronize(st));
+    if (mat && total && cap >= nsel && nsel > 0) {
+        RALLOC(rows, sizeof(int32_t) * 4 * (size_t)nsel);
+        RALLOC(outv, sizeof(double) * (size_t)nsel);
+        RCHECK(rdm_emit(tw.as<double>(), N, flags.as<uint8_t>(), pos.as<int64_t>(),
+                        rows.as<int32_t>(), outv.as<double>(), st));
+        std::vector<int32_t> r4((size_t)4 * nsel);
+        RCHECK(hipMemcpyAsync(r4.data(), rows.p, sizeof(int32_t) * r4.size(),
+                              hipMemcpyDeviceToHost, st));
+        RCHECK(hipMemcpyAsync(total, outv.p, sizeof(double) * nsel, hipMemcpyDeviceToHost, st));
+        RCHECK(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < nsel; ++i)
+            for (int k = 0; k < 4; ++k) mat[i + k * cap] = r4[(size_t)4 * i + k];
+    }
+    return nsel;
+#undef RALLOC
+#undef RCHECK
+}
+
+}  // namespace
+
+int64_t thistle_createtwordm(const int32_t* confs, int32_t nconf, int32_t lconfs,
+                             const double* civecs, int32_t nstates, int32_t state1,
+                             int32_t state2, double cutoff, const ThistleOpts* opts,
+                             int32_t* mat, double* total, int64_t cap) {
+    if (!confs || !civecs) return -THISTLE_EINVAL;
+    if (nconf <= 0 || lconfs <= 0 || (lconfs % 2) != 0 || lconfs > 256) return -THISTLE_EINVAL;
+    if (state1 < 1 || state2 < 1 || state1 > nstates || state2 > nstates) return -THISTLE_EINVAL;
+    if (cap > 0 && (!mat || !total)) return -THISTLE_EINVAL;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    bool canonical = true;
+    for (int j = 0; j < lconfs && canonical; ++j)
+        for (int c = 0; c < nconf; ++c) {
+            const int v = confs[(size_t)c + (size_t)nconf * j];
+            if (v != 0 && v != ((j % 2 == 0) ? 1 : 2)) {
+                canonical = false;
+                break;
+            }
+        }
+    const double* c1v = civecs + (size_t)nconf * (state1 - 1);
+    const double* c2v = civecs + (size_t)nconf * (state2 - 1);
+    if (canonical)
+        return createtwordm_device(confs, nconf, lconfs, c1v, c2v, cutoff, o->device, mat, total,
+                                   cap);
+    const int norb = lconfs / 2;
+    std::vector<double> tw((size_t)norb * norb * norb * norb);
+    int rc = thistle_twordm_dense_host(confs, nconf, lconfs, civecs, nstates, state1, state2,
+                                       tw.data());
+    if (rc) return -rc;
+    return thistle_twordm_select_host(tw.data(), norb, cutoff, cap > 0 ? mat : nullptr,
+                                      cap > 0 ? total : nullptr, cap);
+}
+
 int32_t thistle_onerdm(const int32_t* confs, int32_t nconf, int32_t lconfs, const double* civecs,
                        int32_t nstates, int32_t state1, int32_t state2, double* onerdm) {
     if (!confs || !civecs || !onerdm) return THISTLE_EINVAL;
@@ -1157,15 +1315,29 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
         for (int i = 0; i < basis->nq; ++i) q_abs_out[i] = 0.0;
     const int norb = lconfs / 2;
     if (type == 1 || type == 7) {
-        std::vector<double> tw((size_t)norb * norb * norb * norb);
-        int rc = thistle_twordm_dense_host(confs, nconf, lconfs, civecs, nstates, state1, state2,
-                                           tw.data());
-        if (rc) return rc;
-        double cut = o->rdm_cutoff > 0 ? o->rdm_cutoff : 1e-30;
-        int64_t cnt = thistle_twordm_select_host(tw.data(), norb, cut, nullptr, nullptr, 0);
+        // createtwordm on the device; at most norb^4 rows, so one pass when
+        // that bound is small
+        const double cut = o->rdm_cutoff > 0 ? o->rdm_cutoff : 1e-30;
+        const int64_t n4 = (int64_t)norb * norb * norb * norb;
+        int64_t cap = n4 <= ((int64_t)1 << 22) ? n4 : 0;
+        std::vector<int32_t> m4((size_t)4 * std::max<int64_t>(cap, 1));
+        std::vector<double> tot(std::max<int64_t>(cap, 1));
+        int64_t cnt = thistle_createtwordm(confs, nconf, lconfs, civecs, nstates, state1, state2,
+                                           cut, o, m4.data(), tot.data(), cap);
+        if (cnt < 0) return (int32_t)-cnt;
+        if (cnt > cap) {
+            cap = cnt;
+            m4.assign((size_t)4 * cap, 0);
+            tot.assign(cap, 0.0);
+            int64_t c2 = thistle_createtwordm(confs, nconf, lconfs, civecs, nstates, state1,
+                                              state2, cut, o, m4.data(), tot.data(), cap);
+            if (c2 < 0) return (int32_t)-c2;
+        }
+        // (cap, 4) -> (cnt, 4) column-major
         std::vector<int32_t> mat((size_t)4 * std::max<int64_t>(cnt, 1));
-        std::vector<double> tot(std::max<int64_t>(cnt, 1));
-        thistle_twordm_select_host(tw.data(), norb, cut, mat.data(), tot.data(), cnt);
+        for (int k = 0; k < 4; ++k)
+            for (int64_t i = 0; i < cnt; ++i) mat[(size_t)i + (size_t)k * cnt] = m4[(size_t)i + (size_t)k * cap];
+        int rc = 0;
         if (type == 1)
             return thistle_total_from_twordm(basis, cnt, mat.data(), tot.data(), o, result_out);
         ThistlePlan* pl = nullptr;

@@ -45,6 +45,26 @@ static inline __host__ __device__ int pair_index(int gi, int gj, int ng) {
     return gi * ng - (gi * (gi - 1)) / 2 + (gj - gi);
 }
 
+// createtwordm on the device (rdm.hip); see the header comment there
+hipError_t rdm_pack(const int32_t* confs, int nconf, int lconfs, int nw, uint64_t* masks,
+                    hipStream_t s);
+hipError_t rdm_pairs(const uint64_t* masks, int nconf, int nw, int maxdiff, int64_t* cnt,
+                     const int64_t* off, int32_t* c1s, int32_t* c2s, int8_t* nds, hipStream_t s);
+hipError_t rdm_ccount(const uint64_t* masks, int nw, const int32_t* c1s, const int32_t* c2s,
+                      const int8_t* nds, int64_t np, int64_t* cnt, hipStream_t s);
+hipError_t rdm_cwrite(const uint64_t* masks, int nw, const int32_t* c1s, const int32_t* c2s,
+                      const int8_t* nds, int64_t np, const double* c1v, const double* c2v,
+                      const int64_t* coff, int N, uint32_t* keys, double* vals, hipStream_t s);
+hipError_t rdm_scan_i64(void* temp, size_t& bytes, const int64_t* in, int64_t* out, int64_t n,
+                        hipStream_t s);
+hipError_t rdm_sort(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* kout,
+                    const double* vin, double* vout, int64_t n, int keybits, hipStream_t s);
+hipError_t rdm_esum(const uint32_t* keys, const double* vals, int64_t n, int64_t nel,
+                    int64_t* start, double* tw, hipStream_t s);
+hipError_t rdm_flags(const double* tw, int N, double cutoff, uint8_t* flags, hipStream_t s);
+hipError_t rdm_emit(const double* tw, int N, const uint8_t* flags, const int64_t* pos,
+                    int32_t* rows, double* out, hipStream_t s);
+
 // Host helpers (rdm_host.cpp)
 int thistle_twordm_dense_host(const int32_t* confs, int nconf, int lconfs,
                               const double* civs, int nstates, int state1,

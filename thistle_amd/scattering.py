@@ -136,9 +136,12 @@ def elastic_from_onerdm(ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm, cutoffz
     return res
 
 
-# --------------------------------------------------- 2-RDM / 1-RDM (host)
-def createtwordm(confs, civs, state1=1, state2=1, cutoff=1e-30):
-    """maxcoincidence + createtwordm: rows (p,s,q,r) and values (src/twordms.f90)."""
+# --------------------------------------------------- 2-RDM / 1-RDM
+def createtwordm(confs, civs, state1=1, state2=1, cutoff=1e-30, device=None):
+    """maxcoincidence + createtwordm: rows (p,s,q,r) and values (src/twordms.f90).
+
+    device=None builds it on the GPU when one is present (thistle_createtwordm)
+    and on the host otherwise; both give the reference's rows and values."""
     lib = L.load()
     confs = np.require(np.asarray(confs), dtype=np.int32, requirements=["F"])
     civs = np.require(np.asarray(civs, dtype=np.float64), requirements=["F"])
@@ -146,6 +149,28 @@ def createtwordm(confs, civs, state1=1, state2=1, cutoff=1e-30):
         civs = civs.reshape(-1, 1, order="F")
     nconf, lconfs = confs.shape
     norb = lconfs // 2
+    if device is None:
+        device = L.device_count() > 0
+    if device:
+        # at most norb^4 rows: one pass when that bound is small
+        cap = norb ** 4 if norb ** 4 <= (1 << 22) else 0
+        mat = np.zeros((max(cap, 1), 4), dtype=np.int32, order="F")
+        tot = np.zeros(max(cap, 1))
+        cnt = lib.thistle_createtwordm(L._ptr(confs), nconf, lconfs, L._ptr(civs), civs.shape[1],
+                                       state1, state2, cutoff, None, L._ptr(mat), L._ptr(tot),
+                                       cap)
+        if cnt < 0:
+            L.check(-cnt, "createtwordm")
+        if cnt > cap:
+            mat = np.zeros((cnt, 4), dtype=np.int32, order="F")
+            tot = np.zeros(cnt)
+            cnt2 = lib.thistle_createtwordm(L._ptr(confs), nconf, lconfs, L._ptr(civs),
+                                            civs.shape[1], state1, state2, cutoff, None,
+                                            L._ptr(mat), L._ptr(tot), cnt)
+            if cnt2 < 0:
+                L.check(-cnt2, "createtwordm")
+            cap = cnt
+        return np.ascontiguousarray(mat[:cnt]), tot[:cnt].copy()
     tw = np.zeros((norb,) * 4, order="F")
     L.check(lib.thistle_twordm_dense(L._ptr(confs), nconf, lconfs, L._ptr(civs), civs.shape[1],
                                      state1, state2, L._ptr(tw)), "twordm_dense")
