@@ -47,7 +47,8 @@ enum {
     THISTLE_EDEVICE = 5,     /* HIP or rocBLAS failure                       */
     THISTLE_ENOMEM = 6,      /* device allocation failed                     */
     THISTLE_ETYPE = 7        /* calculation type other than 1 (total j0),
-                                2 (elastic j0) or 7 (total j2)               */
+                                2 (elastic j0), 7 (total j2) or 8 (elastic
+                                j2)                                          */
 };
 
 enum { THISTLE_FP64 = 0, THISTLE_MIXED = 1 };
@@ -78,11 +79,11 @@ typedef struct ThistleOpts {
 } ThistleOpts;
 
 /* Drop-in for main_calculation_mod::total_scattering_calculation, types 1
- * (total j0), 2 (elastic j0) and 7 (total j2): src/main_calculation.f90:
- * 13-15,97-152,386-404.  Zn/geom/state3 are accepted for signature parity
- * and unused (as in the reference for these types).  q_abs_out receives
- * zeros (:97-98).  Type 7 supports shells up to f (its series runs to
- * LLmax + 2 <= 16), else THISTLE_ELLMAX. */
+ * (total j0), 2 (elastic j0), 7 (total j2) and 8 (elastic j2):
+ * src/main_calculation.f90:13-15,97-152,386-404,780-816.  Zn/geom/state3 are
+ * accepted for signature parity and unused (as in the reference for these
+ * types).  q_abs_out receives zeros (:97-98).  Types 7 and 8 support shells
+ * up to f (their series runs to LLmax + 2 <= 16), else THISTLE_ELLMAX. */
 int32_t thistle_total_scattering_calculation(
     int32_t type, int32_t natom, const double *Zn, const double *geom,
     int32_t state1, int32_t state2, int32_t state3, const ThistleBasis *basis,
@@ -190,7 +191,7 @@ int32_t thistle_pair_tables(const ThistleBasis *basis, double *E, double *P,
 typedef struct ThistlePlan ThistlePlan;
 
 /* type 1 (total j0) and 7 (total j2): mat/total rows; type 2 (elastic
- * j0): onerdm (norb x norb). */
+ * j0) and 8 (elastic j2): onerdm (norb x norb). */
 int32_t thistle_plan_create(int32_t type, const ThistleBasis *basis,
                             int64_t nmat, const int32_t *mat,
                             const double *total, int32_t norb,

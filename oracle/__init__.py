@@ -48,6 +48,8 @@ def _declare(L):
     L.orc_elastic_z.argtypes = [_p, _i, _p, _i, _i, _p]
     L.orc_quartet_list.argtypes = [_i, _p, _p, _p, _d, _p]
     L.orc_quartet_list.restype = C.c_int64
+    L.orc_quartet_list_elastic_j2.argtypes = [_i, _p, _p, _p, _d, _p]
+    L.orc_quartet_list_elastic_j2.restype = C.c_int64
     L.orc_total_j0.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                _i, _p, _p, _i, _p, _i, _d, _d, _d, _i, _i, _p]
     L.orc_total_j0.restype = _i
@@ -61,6 +63,8 @@ def _declare(L):
     L.orc_elastic_j0.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
                                  _p, _p, _i, _d, _d, _d, _i, _p]
     L.orc_elastic_j0.restype = _i
+    L.orc_elastic_j2.argtypes = L.orc_elastic_j0.argtypes
+    L.orc_elastic_j2.restype = _i
     L.orc_quartet_series.argtypes = [_p, _p, _p, _p, _p, _i, _i, _p, _p, _p,
                                      _i, _i, _i, _i, _d, _d, _d, _d, _p, _d,
                                      _d, _p, _i, _i, _p, _p]
@@ -255,7 +259,7 @@ def total_j0_sample(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q, stride,
 
 
 def elastic_j0(l, m, n, group, ga, xx, yy, zz, Z, q, cutoffz=1e-15,
-               cutoffmd=1e-20, cutoffcentre=0.001, nthreads=0, maxl=None):
+               cutoffmd=1e-20, cutoffcentre=0.001, nthreads=0, maxl=None, _fn="orc_elastic_j0"):
     l, m, n, group = _I(l), _I(m), _I(n), _I(group)
     ga, xx, yy, zz, q = _D(ga), _D(xx), _D(yy), _D(zz), _D(q)
     Z = _D(Z, "F")
@@ -263,12 +267,29 @@ def elastic_j0(l, m, n, group, ga, xx, yy, zz, Z, q, cutoffz=1e-15,
     ng = int(group.max())
     maxl = int(l.max()) if maxl is None else maxl
     out = np.zeros(len(q))
-    rc = lib().orc_elastic_j0(ngto, ng, maxl, _ptr(l), _ptr(m), _ptr(n), _ptr(group),
-                              _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(Z),
-                              _ptr(q), len(q), cutoffz, cutoffmd, cutoffcentre,
-                              nthreads, _ptr(out))
+    rc = getattr(lib(), _fn)(ngto, ng, maxl, _ptr(l), _ptr(m), _ptr(n), _ptr(group),
+                             _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(Z),
+                             _ptr(q), len(q), cutoffz, cutoffmd, cutoffcentre,
+                             nthreads, _ptr(out))
     if rc:
-        raise RuntimeError(f"oracle elastic_j0 status {rc}")
+        raise RuntimeError(f"oracle {_fn} status {rc}")
+    return out
+
+
+def elastic_j2(l, m, n, group, ga, xx, yy, zz, Z, q, cutoffz=1e-15,
+               cutoffmd=1e-20, cutoffcentre=0.001, nthreads=0, maxl=None):
+    """elastic_integration_j2 (type 8) with variables_elastic's Z."""
+    return elastic_j0(l, m, n, group, ga, xx, yy, zz, Z, q, cutoffz, cutoffmd, cutoffcentre,
+                      nthreads, maxl, _fn="orc_elastic_j2")
+
+
+def quartet_list_elastic_j2(px, py, pz, cutoffcentre):
+    """The six nests of elastic_integration_j2; rows as quartet_list."""
+    px, py, pz = _D(px), _D(py), _D(pz)
+    ng = px.shape[0]
+    Q = lib().orc_quartet_list_elastic_j2(ng, _ptr(px), _ptr(py), _ptr(pz), cutoffcentre, None)
+    out = np.zeros((Q, 6), dtype=np.int32)
+    lib().orc_quartet_list_elastic_j2(ng, _ptr(px), _ptr(py), _ptr(pz), cutoffcentre, _ptr(out))
     return out
 
 

@@ -558,6 +558,80 @@ int64_t orc_quartet_list(int ng, const double *px, const double *py,
     return c;
 }
 
+/* The six nests of elastic_integration_j2 (src/Integrals.f90:1941-2364):
+ * the (i,j,i,r) family of elastic_integration splits into r > j (mult 8)
+ * and r = j (mult 4); the rest is the same as orc_quartet_list. */
+int64_t orc_quartet_list_elastic_j2(int ng, const double *px, const double *py,
+                                    const double *pz, double cutoffcentre,
+                                    int32_t *out)
+{
+    int64_t c = 0;
+#define P(A, g1, g2) A[(g1) * ng + (g2)]
+#define EMIT(a, b, cc, d, mult, br)                                             \
+    do {                                                                       \
+        if (out) {                                                             \
+            int32_t *o = out + 6 * c;                                          \
+            o[0] = (a) + 1; o[1] = (b) + 1; o[2] = (cc) + 1; o[3] = (d) + 1;   \
+            o[4] = (mult); o[5] = (br);                                        \
+        }                                                                      \
+        ++c;                                                                   \
+    } while (0)
+    /* :2004-2007 */
+    for (int i = 0; i < ng; ++i)
+        for (int j = i + 1; j < ng; ++j)
+            for (int k = i + 1; k < ng; ++k)
+                for (int r = k + 1; r < ng; ++r) {
+                    double hx = P(px, k, r) - P(px, i, j);
+                    double hy = P(py, k, r) - P(py, i, j);
+                    double hz = P(pz, k, r) - P(pz, i, j);
+                    double h = pow(hx * hx + hy * hy + hz * hz, 0.5);
+                    EMIT(i, j, k, r, 8, h < cutoffcentre);
+                }
+    /* :2083-2085 */
+    for (int i = 0; i < ng; ++i)
+        for (int j = i + 1; j < ng; ++j)
+            for (int r = j + 1; r < ng; ++r) {
+                double hx = P(px, i, r) - P(px, i, j);
+                double hy = P(py, i, r) - P(py, i, j);
+                double hz = P(pz, i, r) - P(pz, i, j);
+                double h = sqrt(hx * hx + hy * hy + hz * hz);
+                EMIT(i, j, i, r, 8, h < cutoffcentre);
+            }
+    /* :2145-2146 (h = 0 by construction) */
+    for (int i = 0; i < ng; ++i)
+        for (int j = i + 1; j < ng; ++j) {
+            double hx = P(px, i, j) - P(px, i, j);
+            double hy = P(py, i, j) - P(py, i, j);
+            double hz = P(pz, i, j) - P(pz, i, j);
+            double h = sqrt(hx * hx + hy * hy + hz * hz);
+            EMIT(i, j, i, j, 4, h < cutoffcentre);
+        }
+    /* :2204-2206 */
+    for (int i = 0; i < ng; ++i)
+        for (int k = 0; k < ng; ++k)
+            for (int r = k + 1; r < ng; ++r) {
+                double hx = P(px, k, r) - P(px, i, i);
+                double hy = P(py, k, r) - P(py, i, i);
+                double hz = P(pz, k, r) - P(pz, i, i);
+                double h = sqrt(hx * hx + hy * hy + hz * hz);
+                EMIT(i, i, k, r, 4, h < cutoffcentre);
+            }
+    /* :2268-2269 */
+    for (int i = 0; i < ng; ++i)
+        for (int k = i + 1; k < ng; ++k) {
+            double hx = P(px, k, k) - P(px, i, i);
+            double hy = P(py, k, k) - P(py, i, i);
+            double hz = P(pz, k, k) - P(pz, i, i);
+            double h = sqrt(hx * hx + hy * hy + hz * hz);
+            EMIT(i, i, k, k, 2, h < cutoffcentre);
+        }
+    /* :2325 (always the h = 0 branch) */
+    for (int i = 0; i < ng; ++i) EMIT(i, i, i, i, 1, 1);
+#undef EMIT
+#undef P
+    return c;
+}
+
 /* ------------------------------------------------------- Bessel series */
 /* dphrec, src/Integrals.f90:7147-7184 (rbes(leps1,nq) column-major) */
 void orc_dphrec(const double *x, double *rbes, int leps1, int lmax1, int nq)
@@ -727,16 +801,18 @@ static inline const double *dd_ptr(const quartet_ctx *C, const double *dd,
 }
 
 /* tot_integral_k_ijkr (mode 0, src/Integrals.f90:5068-5226),
- * integral_k_ijkr (mode 1, :5730-5901) and tot_integral_k_ijkr_j2 (mode 2,
- * :5230-5420).  ztot is supplied as a block.  Returns 0, or 2 when the
- * coefficient arrays would overflow dim = 20 (the reference's `stop`). */
+ * integral_k_ijkr (mode 1, :5730-5901), tot_integral_k_ijkr_j2 (mode 2,
+ * :5230-5420) and integral_k_ijkr_j2 (mode 3, :5903-6082: mode 2's
+ * coefficients with mode 1's MD-coefficient skips).  ztot is supplied as a
+ * block.  Returns 0, or 2 when the coefficient arrays would overflow dim = 20
+ * (the reference's `stop`). */
 static int k_ijkr(const quartet_ctx *C, int gi, int gj, int gk, int gr,
                   double hx, double hy, double hz, double h,
                   const double *ztot, double cutoffz, double cutoffmd,
                   const double *q, int nq, int mode, double *h_saved_out,
                   double *F)
 {
-    const int elastic = mode == 1, j2 = mode == 2;
+    const int elastic = mode == 1 || mode == 3, j2 = mode == 2 || mode == 3;
     const int *l = C->l, *m = C->m, *n = C->n;
     int si = C->gs[gi], sj = C->gs[gj], sk = C->gs[gk], sr = C->gs[gr];
     int LLmax = l[si] + m[si] + n[si] + l[sj] + m[sj] + n[sj] + l[sk] + m[sk] +
@@ -850,7 +926,7 @@ static int k_ijkr(const quartet_ctx *C, int gi, int gj, int gk, int gr,
     }
     free(hp);
     if (h_saved_out) memcpy(h_saved_out, hs, sizeof hs);
-    if (elastic) {
+    if (elastic && !j2) {
         double s = 0.0;
         for (int t = 0; t < DIMH; ++t) s = s + hs[t];
         if (s == 0.0) {
@@ -1293,20 +1369,26 @@ int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
                          kernel_seconds, nsampled, 0);
 }
 
-int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
-                   const int *n, const int *group, const double *ga,
-                   const double *xx, const double *yy, const double *zz,
-                   const double *Z, const double *q, int nq, double cutoffz,
-                   double cutoffmd, double cutoffcentre, int nthreads,
-                   double *tsi)
+/* elastic_integration (j2 = 0) and elastic_integration_j2 (j2 = 1,
+ * src/Integrals.f90:1941-2364: its six nests, integral_k_ijkr_j2 and the
+ * P0_j2 table set by main_calculation.f90:785). */
+static int elastic_impl(int ngto, int ng, int maxl, const int *l, const int *m,
+                        const int *n, const int *group, const double *ga,
+                        const double *xx, const double *yy, const double *zz,
+                        const double *Z, const double *q, int nq, double cutoffz,
+                        double cutoffmd, double cutoffcentre, int nthreads,
+                        double *tsi, int j2)
 {
     setup_t S;
     int rc = setup_build(&S, ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz,
                          q, nq);
     if (rc) return rc;
-    int64_t Q = orc_quartet_list(ng, S.px, S.py, S.pz, cutoffcentre, NULL);
+    if (j2) orc_set_p0_j2(S.P0, S.D0 - 1, q, nq);
+    int64_t (*qlist)(int, const double *, const double *, const double *, double,
+                     int32_t *) = j2 ? orc_quartet_list_elastic_j2 : orc_quartet_list;
+    int64_t Q = qlist(ng, S.px, S.py, S.pz, cutoffcentre, NULL);
     int32_t *ql = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)Q);
-    orc_quartet_list(ng, S.px, S.py, S.pz, cutoffcentre, ql);
+    qlist(ng, S.px, S.py, S.pz, cutoffcentre, ql);
     quartet_ctx C = {l, m, n, S.gs, S.gc, maxl, ng, S.ddx, S.ddy, S.ddz};
     int nth = 1;
 #ifdef _OPENMP
@@ -1345,11 +1427,12 @@ int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
                 double hx = S.px[gk * ng + gr] - S.px[gi * ng + gj];
                 double hy = S.py[gk * ng + gr] - S.py[gi * ng + gj];
                 double hz = S.pz[gk * ng + gr] - S.pz[gi * ng + gj];
-                /* nest 1 (the only mult-8 nest) uses **0.5, the others sqrt */
-                double h = mult == 8 ? pow(hx * hx + hy * hy + hz * hz, 0.5)
-                                     : sqrt(hx * hx + hy * hy + hz * hz);
+                /* the first nest (gi < gj, gk > gi) uses **0.5, the others
+                 * sqrt */
+                double h = (gi != gj && gk != gi) ? pow(hx * hx + hy * hy + hz * hz, 0.5)
+                                                  : sqrt(hx * hx + hy * hy + hz * hz);
                 int r2 = k_ijkr(&C, gi, gj, gk, gr, hx, hy, hz, h, zb, cutoffz,
-                                cutoffmd, q, nq, 1, NULL, F);
+                                cutoffmd, q, nq, j2 ? 3 : 1, NULL, F);
                 if (r2) {
 #pragma omp atomic write
                     err = r2;
@@ -1373,6 +1456,28 @@ int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
     free(ql);
     setup_free(&S);
     return err;
+}
+
+int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
+                   const int *n, const int *group, const double *ga,
+                   const double *xx, const double *yy, const double *zz,
+                   const double *Z, const double *q, int nq, double cutoffz,
+                   double cutoffmd, double cutoffcentre, int nthreads,
+                   double *tsi)
+{
+    return elastic_impl(ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, Z, q, nq,
+                        cutoffz, cutoffmd, cutoffcentre, nthreads, tsi, 0);
+}
+
+int orc_elastic_j2(int ngto, int ng, int maxl, const int *l, const int *m,
+                   const int *n, const int *group, const double *ga,
+                   const double *xx, const double *yy, const double *zz,
+                   const double *Z, const double *q, int nq, double cutoffz,
+                   double cutoffmd, double cutoffcentre, int nthreads,
+                   double *tsi)
+{
+    return elastic_impl(ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, Z, q, nq,
+                        cutoffz, cutoffmd, cutoffcentre, nthreads, tsi, 1);
 }
 
 /* ------------------------------------------------------------ unique_total */

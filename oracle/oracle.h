@@ -72,6 +72,12 @@ void orc_elastic_z(const double *onerdm, int norb, const double *mmod, int nmo,
  * (1 = h < cutoffcentre -> pzero). */
 int64_t orc_quartet_list(int ng, const double *px, const double *py,
                          const double *pz, double cutoffcentre, int32_t *out);
+/* The six nests of elastic_integration_j2 (src/Integrals.f90:2004-2325):
+ * (i,j,k,r) 8, (i,j,i,r) r>j 8, (i,j,i,j) 4, (i,i,k,r) 4, (i,i,k,k) 2,
+ * (i,i,i,i) 1; same output format. */
+int64_t orc_quartet_list_elastic_j2(int ng, const double *px, const double *py,
+                                    const double *pz, double cutoffcentre,
+                                    int32_t *out);
 
 /* Full drivers.  type 1 = total j0 (tot_integration), type 2 = elastic j0
  * (elastic_integration).
@@ -115,6 +121,14 @@ int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
                         int64_t stride, double *tsi, double *kernel_seconds,
                         int64_t *nsampled);
 int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
+                   const int *n, const int *group, const double *ga,
+                   const double *xx, const double *yy, const double *zz,
+                   const double *Z, const double *q, int nq, double cutoffz,
+                   double cutoffmd, double cutoffcentre, int nthreads,
+                   double *tsi);
+/* elastic_integration_j2 (type 8, src/Integrals.f90:1941-2364) with the
+ * P0_j2 table; arguments as orc_elastic_j0. */
+int orc_elastic_j2(int ngto, int ng, int maxl, const int *l, const int *m,
                    const int *n, const int *group, const double *ga,
                    const double *xx, const double *yy, const double *zz,
                    const double *Z, const double *q, int nq, double cutoffz,

@@ -4,6 +4,7 @@
 ! uniquemodule, compiled unmodified from their sources) and calls
 !   set_P0, maxcoincidence, createtwordm, tot_integration,
 !   set_P0_j2, tot_integration_j2, onerdm_creat, elastic_integration,
+!   elastic_integration_j2,
 !   unique_total
 ! on one input record.  The setup tables of variables_total /
 ! variables_elastic (Ddx/Ddy/Ddz, px/py/pz, e12) are read from the input
@@ -29,7 +30,7 @@ program ref_driver
     integer(ik), allocatable :: confs(:,:), ep3(:,:), ndiff2(:,:), mat(:,:), rmat(:,:)
     real(dp), allocatable :: ddx(:,:,:,:,:), ddy(:,:,:,:,:), ddz(:,:,:,:,:), e12(:,:,:)
     real(dp), allocatable :: p0(:,:,:,:), total(:), rtot(:), z1(:,:,:), z2(:,:,:)
-    real(dp), allocatable :: zt(:,:,:), res1(:), res2(:), res3(:), orm(:,:), zel(:,:)
+    real(dp), allocatable :: zt(:,:,:), res1(:), res2(:), res3(:), res4(:), orm(:,:), zel(:,:)
     real(dp), allocatable :: p0j2(:,:,:,:)
     character(len=512) :: fin, fout
 
@@ -74,9 +75,14 @@ program ref_driver
     call createtwordm(confs, civs, ndiff2, ep3, mat, total, state1, state2)
     nmat = size(total)
 
-    allocate(res1(nq), res2(nq), res3(nq))
+    allocate(res1(nq), res2(nq), res3(nq), res4(nq))
     res1 = 0.0_dp
     res3 = 0.0_dp
+    res4 = 0.0_dp
+    ! the P0_j2 table of types 7 and 8 (src/main_calculation.f90:389,785)
+    allocate(p0j2(nq, 4*maxval(l)+1, 4*maxval(l)+1, 4*maxval(l)+1))
+    p0j2 = 0.0_dp
+    call set_P0_j2(p0j2, 4*maxval(l), q)
     if (dolist /= 0) then
         ! z11/z22 of variables_total (src/Variables.f90:1041-1061); the pair
         ! sort of :1000-1008 does not change these values.
@@ -92,10 +98,7 @@ program ref_driver
         zt = 0.0_dp
         call tot_integration(ng, px, py, pz, l, m, n, p0, ddx, ddy, ddz, z1, z2, zt, zt, &
                              gs, gc, group, cutoffz, cutoffmd, cutoffcentre, q, e12, res1)
-        ! total j2 (type 7): the P0_j2 table and tot_integration_j2
-        allocate(p0j2(nq, 4*maxval(l)+1, 4*maxval(l)+1, 4*maxval(l)+1))
-        p0j2 = 0.0_dp
-        call set_P0_j2(p0j2, 4*maxval(l), q)
+        ! total j2 (type 7)
         call tot_integration_j2(ng, nq, px, py, pz, l, m, n, p0j2, ddx, ddy, ddz, z1, z2, &
                                 gs, gc, group, cutoffz, cutoffmd, cutoffcentre, q, e12, res3)
         deallocate(z1, z2)
@@ -118,6 +121,9 @@ program ref_driver
     end do
     call elastic_integration(ng, px, py, pz, l, m, n, p0, ddx, ddy, ddz, zel, gs, gc, group, &
                              cutoffz, cutoffmd, cutoffcentre, q, e12, res2)
+    ! elastic j2 (type 8, src/main_calculation.f90:780-816)
+    call elastic_integration_j2(ng, px, py, pz, l, m, n, p0j2, ddx, ddy, ddz, zel, gs, gc, group, &
+                                cutoffz, cutoffmd, cutoffcentre, q, e12, res4)
 
     call unique_total(mat, total, rmat, rtot)
     cnt = size(rtot)
@@ -133,6 +139,7 @@ program ref_driver
     write(11) rmat
     write(11) rtot
     write(11) res3
+    write(11) res4
     write(11) p0
     close(11)
 end program ref_driver

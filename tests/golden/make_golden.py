@@ -6,7 +6,8 @@ inputs, reads them back through thistle_amd.readers, computes the setup
 tables with the oracle (the reference's src/Variables.f90 does not compile),
 then runs oracle/_ref/ref_driver, which calls the reference's set_P0,
 maxcoincidence, createtwordm, tot_integration, set_P0_j2,
-tot_integration_j2, onerdm_creat, elastic_integration and unique_total.  Inputs and outputs are saved as
+tot_integration_j2, onerdm_creat, elastic_integration, elastic_integration_j2
+and unique_total.  Inputs and outputs are saved as
 compressed .npz fixtures beside this script.
 
     python tests/golden/make_golden.py [case ...]
@@ -95,10 +96,12 @@ def read_result(path, nq, ngto, ncol=4, D=None):
         rmat = np.frombuffer(f.read(4 * ncol * cnt), dtype=np.int32).reshape(ncol, cnt).T
         rtot = np.frombuffer(f.read(8 * cnt), dtype=np.float64)
         res3 = np.frombuffer(f.read(8 * nq), dtype=np.float64)
+        res4 = np.frombuffer(f.read(8 * nq), dtype=np.float64)
         p0 = np.frombuffer(f.read(), dtype=np.float64)
     return dict(mat=mat.copy(), total=total.copy(), S_total=res1.copy(),
                 S_elastic=res2.copy(), onerdm=orm.copy(), Z=zel.copy(),
-                rmat=rmat.copy(), rtot=rtot.copy(), S_total_j2=res3.copy(), P0=p0.copy())
+                rmat=rmat.copy(), rtot=rtot.copy(), S_total_j2=res3.copy(),
+                S_elastic_j2=res4.copy(), P0=p0.copy())
 
 
 def run_case(name, cutoffs=(1e-15, 1e-20, 0.001)):
@@ -121,7 +124,8 @@ def save(name, inp, q, ref, cutoffs=(1e-15, 1e-20, 0.001)):
         mmod=inp.mmod, group=inp.group, confs=inp.confs.astype(np.int8),
         civs=inp.civs, q=q, cutoffs=np.array(cutoffs),
         mat=ref["mat"].astype(np.int16), total=ref["total"], S_total=ref["S_total"],
-        S_elastic=ref["S_elastic"], S_total_j2=ref["S_total_j2"], onerdm=ref["onerdm"],
+        S_elastic=ref["S_elastic"], S_total_j2=ref["S_total_j2"],
+        S_elastic_j2=ref["S_elastic_j2"], onerdm=ref["onerdm"],
         rmat=ref["rmat"].astype(np.int16),
         rtot=ref["rtot"], P0_sum=np.array([ref["P0"].sum(), np.abs(ref["P0"]).sum()]))
     return path
@@ -164,10 +168,16 @@ def main(names):
                              inp.zz, inp.mmod, mat, tot, q)
         r2 = ref["S_total_j2"]
         rel2 = np.max(np.abs(S2 - r2)[1:] / np.abs(r2)[1:])
+        Ze = oracle.elastic_z(oracle.onerdm(inp.confs, inp.civs), inp.mmod)
+        S4 = oracle.elastic_j2(inp.l, inp.m, inp.n, inp.group, inp.ga, inp.xx, inp.yy,
+                               inp.zz, Ze, q)
+        r4 = ref["S_elastic_j2"]
+        rel4 = np.max(np.abs(S4 - r4)[1:] / np.abs(r4)[1:])
         print(f"{name}: ngto={inp.ngto} ng={inp.ng} nmat={len(tot)} "
               f"mat-equal={np.array_equal(mat, ref['mat'])} "
               f"total-maxrel={np.max(np.abs(tot - ref['total']) / np.abs(ref['total'])):.2e} "
-              f"S_total-maxrel={rel:.2e} S_j2-maxrel(q>0)={rel2:.2e} size={os.path.getsize(path)}B",
+              f"S_total-maxrel={rel:.2e} S_j2-maxrel(q>0)={rel2:.2e} "
+              f"S_el_j2-maxrel(q>0)={rel4:.2e} size={os.path.getsize(path)}B",
               flush=True)
 
 

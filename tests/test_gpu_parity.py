@@ -155,3 +155,46 @@ def test_drop_in_type7(golden, name):
     ref = g["S_total_j2"]
     assert np.max(np.abs(res - ref)[1:] / np.abs(ref)[1:]) < RTOL_FP64
 
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_elastic_j2_matches_reference(golden, name):
+    """Type 8 (elastic_integration_j2) against the reference; q[0] as type 7."""
+    g = golden(name)
+    cz, cm, cc = g["cutoffs"]
+    res = sc.elastic_j2_from_onerdm(*_args(g), g["mmod"], g["q"], g["group"], g["onerdm"],
+                                    cz, cm, cc)
+    ref = g["S_elastic_j2"]
+    rel = np.max(np.abs(res - ref)[1:] / np.abs(ref)[1:])
+    print(f"{name}: max rel err elastic j2 {rel:.3e}", flush=True)
+    assert rel < RTOL_FP64
+    assert abs(res[0] - ref[0]) <= RTOL_FP64 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", ["sp3", "spd2"])
+def test_drop_in_type8(golden, name):
+    g = golden(name)
+    cz, cm, cc = g["cutoffs"]
+    l = g["l"]
+    _, res = sc.total_scattering_calculation(
+        8, np.ones(2), np.zeros((2, 3)), 1, 1, 1, int(l.max()), l.size, int(g["group"].max()),
+        g["ga"], l, g["m"], g["n"], g["xx"], g["yy"], g["zz"], g["mmod"], g["q"], g["q"].size,
+        g["group"], cz, cm, cc, g["confs"].astype(np.int32), g["civs"])
+    ref = g["S_elastic_j2"]
+    assert np.max(np.abs(res - ref)[1:] / np.abs(ref)[1:]) < RTOL_FP64
+
+
+@pytest.mark.parametrize("name", ["sp3", "xyx_cas44"])
+def test_elastic_j2_list_bit_exact(golden, name):
+    """The six nests of elastic_integration_j2 enumerated on the GPU: with
+    random MOs no block is all zero, so the screened list is the whole list,
+    equal to the oracle's row for row (indices, multiplicity, branch)."""
+    g = golden(name)
+    cz, cm, cc = g["cutoffs"]
+    dd, P, e12 = oracle.pair_tables(g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"],
+                                    g["yy"], g["zz"], g["q"][:2])
+    ref = oracle.quartet_list_elastic_j2(P[0], P[1], P[2], cc)
+    scr = sc.quartet_list_screened(8, g["ga"], g["l"], g["m"], g["n"], g["xx"], g["yy"],
+                                   g["zz"], g["mmod"], g["q"], g["group"], onerdm=g["onerdm"],
+                                   cutoffz=cz, cutoffcentre=cc)
+    assert scr.shape == ref.shape and np.array_equal(scr, ref)

@@ -76,6 +76,17 @@ def test_elastic_vs_oracle(case):
     assert rel < RTOL_FP64
 
 
+def test_elastic_j2_vs_oracle(case):
+    name, inp, q, mat, tot = case
+    g = sc.onerdm(inp.confs, inp.civs)
+    res = sc.elastic_j2_from_onerdm(*_args(inp), q, inp.group, g)
+    ref = oracle.elastic_j2(inp.l, inp.m, inp.n, inp.group, inp.ga, inp.xx, inp.yy, inp.zz,
+                            oracle.elastic_z(g, inp.mmod), q)
+    rel = np.max(np.abs(res - ref)[1:] / np.abs(ref)[1:])
+    print(f"{name}: elastic j2 max rel err {rel:.3e}", flush=True)
+    assert rel < RTOL_FP64
+
+
 def test_plan_reuse_is_deterministic(case):
     name, inp, q, mat, tot = case
     plan = sc.Plan(1, *_args(inp), q, inp.group, mat=mat, total=tot)

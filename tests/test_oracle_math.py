@@ -216,6 +216,37 @@ def test_quartet_list_invariants(ng):
         assert mult == {True: 1}.get(nest5, mult) and mult in (1, 2, 4, 8)
 
 
+@pytest.mark.parametrize("ng", [1, 2, 3, 5, 8, 13])
+def test_quartet_list_elastic_j2_invariants(ng):
+    """elastic_integration_j2's six nests (src/Integrals.f90:2004-2325): the
+    (i,j,i,r) family is listed once per unordered {j, r} (mult 8) plus
+    (i,j,i,j) (mult 4); multiplicities still sum to ng^4 and the orbits still
+    cover every ordered group quadruple exactly once."""
+    rng = np.random.default_rng(100 + ng)
+    px, py, pz = (rng.normal(size=(ng, ng)) for _ in range(3))
+    for a in (px, py, pz):
+        a[:] = a + a.T
+    cc = 0.001
+    L = oracle.quartet_list_elastic_j2(px, py, pz, cc)
+    L0 = oracle.quartet_list(px, py, pz, cc)
+    m = np.arange(ng)[::-1]  # ng - 1 - i
+    n_j0_nest2 = int(np.sum(m * m))
+    n_j2_nest2 = int(np.sum(m * (m - 1) // 2 + m))
+    assert len(L) == len(L0) - n_j0_nest2 + n_j2_nest2
+    assert int(L[:, 4].sum()) == ng ** 4
+    cover = Counter()
+    for gi, gj, gk, gr, mult, br in L.tolist():
+        orb = _orbit((gi, gj, gk, gr))
+        for o in orb:
+            cover[o] += mult / len(orb)
+    assert set(cover) == set(itertools.product(range(1, ng + 1), repeat=4))
+    assert all(abs(v - 1.0) < 1e-12 for v in cover.values())
+    # (i,j,i,j) sits at h = 0: always the h = 0 branch
+    for gi, gj, gk, gr, mult, br in L.tolist():
+        if gi == gk and gj == gr:
+            assert br == 1 and mult in (1, 4)
+
+
 # ------------------------------------- a10-a15: whole path, s shells, brute force
 def _s_case(seed, nat=4, nmo=3):
     rng = np.random.default_rng(seed)

@@ -77,6 +77,20 @@ def test_total_j2(case):
     assert abs(S[0] - ref[0]) <= 1e-14 * np.max(np.abs(ref))
 
 
+def test_elastic_j2(case):
+    """elastic_integration_j2 (src/Integrals.f90:1941-2364) with the P0_j2
+    table: type 8.  At q[0] = 1e-10 only the size of the value is checked."""
+    name, g = case
+    gam = oracle.onerdm(g["confs"], g["civs"])
+    Z = oracle.elastic_z(gam, g["mmod"])
+    S = oracle.elastic_j2(g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"], g["yy"],
+                          g["zz"], Z, g["q"], **_cut(g))
+    ref = g["S_elastic_j2"]
+    rel = np.max(np.abs(S - ref)[1:] / np.abs(ref)[1:])
+    assert rel < 1e-12, (name, rel)
+    assert abs(S[0] - ref[0]) <= 1e-14 * np.max(np.abs(ref))
+
+
 def test_unique_total(case):
     """L3: unique_total (src/unique_module.f90:90-145) on the 2-RDM rows."""
     name, g = case

@@ -178,7 +178,9 @@ int check_basis(const ThistleBasis* b, std::vector<GroupInfo>& groups, int& maxl
 }
 
 // Rows of the five nests in reference order (src/Integrals.f90:79-416).
-std::vector<EnumRow> enum_rows(int ng, int64_t& total) {
+// Rows of the reference's quartet nests in reference order; ej2 selects
+// elastic_integration_j2's six nests (src/Integrals.f90:2004-2325).
+std::vector<EnumRow> enum_rows(int ng, int64_t& total, bool ej2 = false) {
     std::vector<EnumRow> rows;
     int64_t off = 0;
     auto add = [&](int nest, int i, int j, int k, int r0, int cnt) {
@@ -189,8 +191,14 @@ std::vector<EnumRow> enum_rows(int ng, int64_t& total) {
     for (int i = 0; i < ng; ++i)
         for (int j = i + 1; j < ng; ++j)
             for (int k = i + 1; k < ng; ++k) add(1, i, j, k, k + 1, ng - 1 - k);
-    for (int i = 0; i < ng; ++i)
-        for (int j = i + 1; j < ng; ++j) add(2, i, j, -1, i + 1, ng - 1 - i);
+    if (ej2) {
+        for (int i = 0; i < ng; ++i)
+            for (int j = i + 1; j < ng; ++j) add(6, i, j, -1, j + 1, ng - 1 - j);
+        for (int i = 0; i < ng; ++i) add(7, i, -1, -1, i + 1, ng - 1 - i);
+    } else {
+        for (int i = 0; i < ng; ++i)
+            for (int j = i + 1; j < ng; ++j) add(2, i, j, -1, i + 1, ng - 1 - i);
+    }
     for (int i = 0; i < ng; ++i)
         for (int k = 0; k < ng; ++k) add(3, i, -1, k, k + 1, ng - 1 - k);
     for (int i = 0; i < ng; ++i) add(4, i, -1, -1, i + 1, ng - 1 - i);
@@ -254,7 +262,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
         lx[j] = make_char4((char)b->l[j], (char)b->m[j], (char)b->n[j],
                            (char)(b->l[j] + b->m[j] + b->n[j]));
     int64_t Q = 0;
-    std::vector<EnumRow> rows = enum_rows(ng, Q);
+    std::vector<EnumRow> rows = enum_rows(ng, Q, pl->type == 8);
     pl->nquart = Q;
     pl->nrows = (int)rows.size();
     // positions of each LL = L_i+L_j+L_k+L_r class, in reference order
@@ -263,14 +271,9 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     {
         std::vector<int64_t> cnt(18, 0);
         auto LLof = [&](const EnumRow& R, int x) {
-            int gi, gj, gk, gr;
-            switch (R.nest) {
-                case 1: gi = R.i; gj = R.j; gk = R.k; gr = x; break;
-                case 2: gi = R.i; gj = R.j; gk = R.i; gr = x; break;
-                case 3: gi = R.i; gj = R.i; gk = R.k; gr = x; break;
-                case 4: gi = R.i; gj = R.i; gk = x; gr = x; break;
-                default: gi = gj = gk = gr = x; break;
-            }
+            int g[4];
+            row_quartet(R, x, g);
+            const int gi = g[0], gj = g[1], gk = g[2], gr = g[3];
             int Ls[4] = {groups[gi].L, groups[gj].L, groups[gk].L, groups[gr].L};
             used[((groups[gi].pad * S + groups[gj].pad) * S + groups[gk].pad) * S +
                  groups[gr].pad] = 1;
@@ -474,7 +477,7 @@ int run_transform_total(ThistlePlan* pl) {
 }
 
 SeriesArgs series_args(const ThistlePlan* pl) {
-    const bool elastic = pl->type == 2;
+    const bool elastic = pl->type == 2 || pl->type == 8;
     SeriesArgs a{};
     a.qlist = pl->qlist.as<uint64_t>();
     a.nquart = pl->nquart;
@@ -492,7 +495,7 @@ SeriesArgs series_args(const ThistlePlan* pl) {
     a.ldz = elastic ? pl->ngto : pl->npp;
     a.elastic = elastic ? 1 : 0;
     a.mixed = pl->precision == THISTLE_MIXED;
-    a.j2 = pl->type == 7 ? 1 : 0;
+    a.j2 = (pl->type == 7 || pl->type == 8) ? 1 : 0;
     a.cutoffz = pl->cutoffz;
     a.part = pl->part.as<double>();
     a.partc = pl->partc.as<double>();
@@ -573,9 +576,9 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
                      const double* total, int32_t norb, const double* onerdm,
                      const ThistleOpts* o, ThistlePlan** out) {
     // every host-side argument check comes before the first HIP call
-    if (type != 1 && type != 2 && type != 7) return THISTLE_ETYPE;
+    if (type != 1 && type != 2 && type != 7 && type != 8) return THISTLE_ETYPE;
     if (!b || !b->mmod || b->nmo <= 0) return THISTLE_EINVAL;
-    if (type == 7 && b->l && b->m && b->n) {
+    if ((type == 7 || type == 8) && b->l && b->m && b->n) {
         // the j2 series runs to order LLmax + 2; the kernels stop at 16
         int lmax = 0;
         for (int j = 0; j < b->ngto; ++j) lmax = std::max(lmax, b->l[j] + b->m[j] + b->n[j]);
@@ -753,17 +756,9 @@ int run_adjoint(ThistlePlan* pl, const double* w_host, double* gg_host, double* 
     if (gg_host) {
         // quartet -> shell entries (e*4 + slot), grouped by shell, in list order
         int64_t Q = 0;
-        std::vector<EnumRow> rows = enum_rows(pl->ng, Q);
+        std::vector<EnumRow> rows = enum_rows(pl->ng, Q, pl->type == 8);
         std::vector<int64_t> cnt(pl->ng + 1, 0);
-        auto quad = [](const EnumRow& R, int x, int g[4]) {
-            switch (R.nest) {
-                case 1: g[0] = R.i; g[1] = R.j; g[2] = R.k; g[3] = x; break;
-                case 2: g[0] = R.i; g[1] = R.j; g[2] = R.i; g[3] = x; break;
-                case 3: g[0] = R.i; g[1] = R.i; g[2] = R.k; g[3] = x; break;
-                case 4: g[0] = R.i; g[1] = R.i; g[2] = x; g[3] = x; break;
-                default: g[0] = g[1] = g[2] = g[3] = x; break;
-            }
-        };
+        auto quad = [](const EnumRow& R, int x, int g[4]) { row_quartet(R, x, g); };
         for (const auto& R : rows)
             for (int u = 0; u < R.count; ++u) {
                 int g[4];
@@ -1308,7 +1303,7 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
     (void)Zn;
     (void)geom;
     (void)state3;
-    if (type != 1 && type != 2 && type != 7) return THISTLE_ETYPE;
+    if (type != 1 && type != 2 && type != 7 && type != 8) return THISTLE_ETYPE;
     if (!basis || !result_out) return THISTLE_EINVAL;
     const ThistleOpts* o = opts ? opts : &kDefaultOpts;
     if (q_abs_out)
@@ -1350,7 +1345,14 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
     std::vector<double> g((size_t)norb * norb);
     int rc = thistle_onerdm_host(confs, nconf, lconfs, civecs, nstates, state1, state2, g.data());
     if (rc) return rc;
-    return thistle_elastic_from_onerdm(basis, norb, g.data(), o, result_out);
+    if (type == 2) return thistle_elastic_from_onerdm(basis, norb, g.data(), o, result_out);
+    // type 8: elastic j2 (src/main_calculation.f90:780-816)
+    ThistlePlan* pl = nullptr;
+    rc = thistle_plan_create(8, basis, 0, nullptr, nullptr, norb, g.data(), o, &pl);
+    if (rc) return rc;
+    rc = thistle_plan_execute(pl, result_out);
+    thistle_plan_destroy(pl);
+    return rc;
 }
 
 int64_t thistle_quartet_list(const ThistleBasis* basis, int32_t* out, int64_t cap,

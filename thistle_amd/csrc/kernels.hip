@@ -146,15 +146,9 @@ __global__ void k_enumerate(const EnumRow* __restrict__ rows,
 #pragma clang fp contract(off)
     EnumRow R = rows[blockIdx.x];
     for (int u = threadIdx.x; u < R.count; u += blockDim.x) {
-        int x = R.r0 + u;
-        int gi, gj, gk, gr, mult;
-        switch (R.nest) {
-            case 1: gi = R.i; gj = R.j; gk = R.k; gr = x; mult = 8; break;
-            case 2: gi = R.i; gj = R.j; gk = R.i; gr = x; mult = 4; break;
-            case 3: gi = R.i; gj = R.i; gk = R.k; gr = x; mult = 4; break;
-            case 4: gi = R.i; gj = R.i; gk = x; gr = x; mult = 2; break;
-            default: gi = x; gj = x; gk = x; gr = x; mult = 1; break;
-        }
+        int g[4];
+        const int mult = row_quartet(R, R.r0 + u, g);
+        const int gi = g[0], gj = g[1], gk = g[2], gr = g[3];
         int br = 1;
         if (R.nest != 5) {
             const PairInfo& b = P[pair_index(gi, gj, ng)];

@@ -40,6 +40,23 @@ struct EnumRow {
 // in class-list order.
 #define HS 17
 
+// The quartet of element x of an enumeration row, and its multiplicity.
+// Nests 1-5: tot_integration / elastic_integration / tot_integration_j2
+// (src/Integrals.f90:79-82,178-180,258-260,340-341,416); elastic_integration_j2
+// (:2004-2325) replaces nest 2 by nest 6, (i,j,i,r) with r > j and mult 8,
+// and nest 7, (i,j,i,j) with mult 4.
+static inline __host__ __device__ int row_quartet(const EnumRow& R, int x, int g[4]) {
+    switch (R.nest) {
+        case 1: g[0] = R.i; g[1] = R.j; g[2] = R.k; g[3] = x; return 8;
+        case 2: g[0] = R.i; g[1] = R.j; g[2] = R.i; g[3] = x; return 4;
+        case 3: g[0] = R.i; g[1] = R.i; g[2] = R.k; g[3] = x; return 4;
+        case 4: g[0] = R.i; g[1] = R.i; g[2] = x; g[3] = x; return 2;
+        case 6: g[0] = R.i; g[1] = R.j; g[2] = R.i; g[3] = x; return 8;
+        case 7: g[0] = R.i; g[1] = x; g[2] = R.i; g[3] = x; return 4;
+        default: g[0] = g[1] = g[2] = g[3] = x; return 1;
+    }
+}
+
 static inline __host__ __device__ int pair_index(int gi, int gj, int ng) {
     // gi <= gj, row-major upper triangle
     return gi * ng - (gi * (gi - 1)) / 2 + (gj - gi);

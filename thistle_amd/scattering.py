@@ -30,14 +30,16 @@ def total_scattering_calculation(type, Zn, geom, state1, state2, state3, maxl, n
                                  ga, l, m, n, xx, yy, zz, mmod, q, nq, group, cutoffz,
                                  cutoffmd, cutoffcentre, confs, civecs, *, precision=L.FP64,
                                  rdm_cutoff=1e-30, waves=0):
-    """Drop-in for main_calculation_mod::total_scattering_calculation (types 1, 2)."""
+    """Drop-in for main_calculation_mod::total_scattering_calculation (types 1, 2,
+    7 and 8)."""
     lib = L.load()
     q = np.asarray(q, dtype=np.float64)
     if int(nq) != q.size or int(ngto) != np.asarray(l).size:
         raise ValueError("nq / ngto do not match the array sizes")
-    if int(type) not in (1, 2, 7):
+    if int(type) not in (1, 2, 7, 8):
         raise NotImplementedError(
-            f"type {type}: only 1 (total j0), 2 (elastic j0) and 7 (total j2) are on this path")
+            f"type {type}: only 1 (total j0), 2 (elastic j0), 7 (total j2) and 8 (elastic j2) "
+            "are on this path")
     b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
                maxl=maxl, ng=ng)
     o = L.make_opts(precision, rdm_cutoff, waves=waves)
@@ -134,6 +136,19 @@ def elastic_from_onerdm(ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm, cutoffz
                                          L._ptr(res))
     L.check(rc, "elastic_from_onerdm")
     return res
+
+
+def elastic_j2_from_onerdm(ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm, cutoffz=1e-15,
+                           cutoffmd=1e-20, cutoffcentre=0.001, *, precision=L.FP64, waves=0,
+                           maxl=None):
+    """Elastic j2 (type 8, elastic_integration_j2) from a 1-RDM."""
+    plan = Plan(8, ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm=onerdm, cutoffz=cutoffz,
+                cutoffmd=cutoffmd, cutoffcentre=cutoffcentre, precision=precision, waves=waves,
+                maxl=maxl)
+    try:
+        return plan.execute()
+    finally:
+        plan.close()
 
 
 # --------------------------------------------------- 2-RDM / 1-RDM
