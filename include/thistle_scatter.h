@@ -47,8 +47,8 @@ enum {
     THISTLE_EDEVICE = 5,     /* HIP or rocBLAS failure                       */
     THISTLE_ENOMEM = 6,      /* device allocation failed                     */
     THISTLE_ETYPE = 7        /* calculation type other than 1 (total j0),
-                                2 (elastic j0), 7 (total j2) or 8 (elastic
-                                j2)                                          */
+                                2 (elastic j0), 5 / 6 (total / elastic
+                                electron), 7 (total j2) or 8 (elastic j2)    */
 };
 
 enum { THISTLE_FP64 = 0, THISTLE_MIXED = 1 };
@@ -79,11 +79,13 @@ typedef struct ThistleOpts {
 } ThistleOpts;
 
 /* Drop-in for main_calculation_mod::total_scattering_calculation, types 1
- * (total j0), 2 (elastic j0), 7 (total j2) and 8 (elastic j2):
- * src/main_calculation.f90:13-15,97-152,386-404,780-816.  Zn/geom/state3 are
- * accepted for signature parity and unused (as in the reference for these
- * types).  q_abs_out receives zeros (:97-98).  Types 7 and 8 support shells
- * up to f (their series runs to LLmax + 2 <= 16), else THISTLE_ELLMAX. */
+ * (total j0), 2 (elastic j0), 5 (total electron), 6 (elastic electron), 7
+ * (total j2) and 8 (elastic j2): src/main_calculation.f90:13-15,97-152,
+ * 270-330,386-404,780-816.  Zn (natom) and geom (natom x 3 column-major,
+ * bohr) are used by types 5 and 6 (Iee - 2 Ine + Inn); state3 is accepted
+ * for signature parity and unused.  q_abs_out receives zeros (:97-98).
+ * Types 7 and 8 support shells up to f (their series runs to
+ * LLmax + 2 <= 16), else THISTLE_ELLMAX. */
 int32_t thistle_total_scattering_calculation(
     int32_t type, int32_t natom, const double *Zn, const double *geom,
     int32_t state1, int32_t state2, int32_t state3, const ThistleBasis *basis,
@@ -113,6 +115,16 @@ int32_t thistle_total_scattering_adjoint(const ThistleBasis *basis,
                                          const ThistleOpts *opts,
                                          double *S_out, double *dL_dR,
                                          double *dL_dtotal);
+
+/* nuclei_electron_integration (src/Integrals.f90:2493-2596): the
+ * electron-nucleus term Ine of electron scattering (types 5/6) from a 1-RDM,
+ * with nuclear charges Zn (natom) at geom (natom x 3 column-major, bohr).
+ * BesselSum's upward recurrence is followed as the reference does; at q = 0
+ * the q -> 0 limit is returned where the reference divides 0 by 0. */
+int32_t thistle_nuclei_electron(const ThistleBasis *basis, int32_t norb,
+                                const double *onerdm, int32_t natom,
+                                const double *Zn, const double *geom,
+                                const ThistleOpts *opts, double *result_out);
 
 /* Elastic j0 from a given 1-RDM (norb x norb, column-major): the body of
  * case 2 after onerdm_creat (src/main_calculation.f90:137-152). */

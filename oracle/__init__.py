@@ -65,6 +65,10 @@ def _declare(L):
     L.orc_elastic_j0.restype = _i
     L.orc_elastic_j2.argtypes = L.orc_elastic_j0.argtypes
     L.orc_elastic_j2.restype = _i
+    L.orc_nuclei_electron.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p,
+                                      _p, _p, _i, _d, _d, _d, _p]
+    L.orc_nuclei_electron.restype = _i
+    L.orc_nuclear_nn.argtypes = [_i, _p, _p, _p, _i, _p]
     L.orc_quartet_series.argtypes = [_p, _p, _p, _p, _p, _i, _i, _p, _p, _p,
                                      _i, _i, _i, _i, _d, _d, _d, _d, _p, _d,
                                      _d, _p, _i, _i, _p, _p]
@@ -281,6 +285,36 @@ def elastic_j2(l, m, n, group, ga, xx, yy, zz, Z, q, cutoffz=1e-15,
     """elastic_integration_j2 (type 8) with variables_elastic's Z."""
     return elastic_j0(l, m, n, group, ga, xx, yy, zz, Z, q, cutoffz, cutoffmd, cutoffcentre,
                       nthreads, maxl, _fn="orc_elastic_j2")
+
+
+def nuclei_electron(l, m, n, group, ga, xx, yy, zz, Z, Zn, geom, q, cutoffz=1e-15,
+                    cutoffmd=1e-20, cutoffcentre=0.001, maxl=None):
+    """nuclei_electron_integration (types 5/6): Sum_na Zn Sum_(i,k) F e12."""
+    l, m, n, group = _I(l), _I(m), _I(n), _I(group)
+    ga, xx, yy, zz, q = _D(ga), _D(xx), _D(yy), _D(zz), _D(q)
+    Z = _D(Z, "F")
+    Zn = _D(np.atleast_1d(Zn))
+    geom = _D(np.asarray(geom, dtype=np.float64).reshape(-1, 3), "F")
+    ngto, ng = len(l), int(group.max())
+    maxl = int(l.max()) if maxl is None else maxl
+    out = np.zeros(len(q))
+    rc = lib().orc_nuclei_electron(ngto, ng, maxl, _ptr(l), _ptr(m), _ptr(n), _ptr(group),
+                                   _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(Z), len(Zn),
+                                   _ptr(Zn), _ptr(geom), _ptr(q), len(q), cutoffz, cutoffmd,
+                                   cutoffcentre, _ptr(out))
+    if rc:
+        raise RuntimeError(f"oracle nuclei_electron status {rc}")
+    return out
+
+
+def nuclear_nn(Zn, geom, q):
+    """Sum_ij Zn_i Zn_j sinc(q R_ij), the nuclear term of types 5/6."""
+    Zn = _D(np.atleast_1d(Zn))
+    geom = _D(np.asarray(geom, dtype=np.float64).reshape(-1, 3), "F")
+    q = _D(q)
+    out = np.zeros(len(q))
+    lib().orc_nuclear_nn(len(Zn), _ptr(Zn), _ptr(geom), _ptr(q), len(q), _ptr(out))
+    return out
 
 
 def quartet_list_elastic_j2(px, py, pz, cutoffcentre):

@@ -1480,6 +1480,152 @@ int orc_elastic_j2(int ngto, int ng, int maxl, const int *l, const int *m,
                         cutoffz, cutoffmd, cutoffcentre, nthreads, tsi, 1);
 }
 
+/* ------------------------------------------- electron scattering, types 5/6 */
+/* BesselSum, src/Integrals.f90:6329-6368: the upward recurrence at every x */
+static void bessel_sum(double *hsum, const double *mu, int nq, double H, int LLmax,
+                       const double *hs)
+{
+    for (int iq = 0; iq < nq; ++iq) {
+        const double Pmu = H * mu[iq];
+        double h0 = sin(Pmu) / Pmu;
+        double s = h0 * hs[0];
+        if (LLmax == 1) {
+            double h1 = (sin(Pmu) / (Pmu * Pmu) - cos(Pmu) / Pmu) * mu[iq] / H;
+            s = s + hs[1] * h1;
+        } else if (LLmax > 1) {
+            const double muOH = mu[iq] / H;
+            double h1 = (sin(Pmu) / (Pmu * Pmu) - cos(Pmu) / Pmu) * muOH;
+            s = s + hs[1] * h1;
+            for (int ra = 2; ra <= LLmax; ++ra) {
+                double hr = ((double)(2 * ra - 1) / Pmu * h1 - h0 * muOH) * muOH;
+                s = s + hr * hs[ra];
+                h0 = h1;
+                h1 = hr;
+            }
+        }
+        hsum[iq] = s;
+    }
+}
+
+/* integral_k_ij (src/Integrals.f90:6192-6326; branch 0) and integral_ij_pzero
+ * (:4865-4951; branch 1) for the group pair (gi, gj) with the elastic Z. */
+static void k_ij(const quartet_ctx *C, int gi, int gj, int branch, double hx, double hy,
+                 double hz, double h, const double *Z, int ngto, double cutoffmd,
+                 const double *P0, int D0, const double *q, int nq, double *F)
+{
+    const int *l = C->l, *m = C->m, *n = C->n;
+    const int si = C->gs[gi], sj = C->gs[gj];
+    const int LLmax = l[si] + m[si] + n[si] + l[sj] + m[sj] + n[sj];
+    const int D = LLmax + 1;
+    double a[DIMH][DIMH], b[DIMH][DIMH], c[DIMH][DIMH];
+    double *hp = NULL;
+    if (!branch) {
+        hermite_like(LLmax, hx, a);
+        hermite_like(LLmax, hy, b);
+        hermite_like(LLmax, hz, c);
+        hp = (double *)calloc((size_t)D * D * D * DIMH, sizeof(double));
+        double BD[DIMH];
+        for (int k = 0; k <= LLmax; ++k)
+            for (int j = 0; j <= LLmax - k; ++j)
+                for (int i = 0; i <= LLmax - k - j; ++i) {
+                    for (int t = 0; t < DIMH; ++t) BD[t] = 0.0;
+                    bessel_deriv(BD, i, j, k, a, b, c, LLmax);
+                    memcpy(hp + (((size_t)i * D + j) * D + k) * DIMH, BD, sizeof BD);
+                }
+    }
+    double hs[DIMH];
+    for (int t = 0; t < DIMH; ++t) hs[t] = 0.0;
+    if (branch)
+        for (int iq = 0; iq < nq; ++iq) F[iq] = 0.0;
+    for (int i = si; i < si + C->gc[gi]; ++i)
+        for (int j = sj; j < sj + C->gc[gj]; ++j) {
+            const double ztot = Z[i + (size_t)ngto * j];
+            const double *Dx1 = dd_ptr(C, C->ddx, gi, gj, l[i], l[j]);
+            const double *Dy1 = dd_ptr(C, C->ddy, gi, gj, m[i], m[j]);
+            const double *Dz1 = dd_ptr(C, C->ddz, gi, gj, n[i], n[j]);
+            for (int ll = 0; ll <= l[i] + l[j]; ++ll) {
+                double MDL = Dx1[ll];
+                if (fabs(MDL) < (double)TINY_SP) continue;
+                double prod1 = MDL * ztot;
+                for (int mm = 0; mm <= m[i] + m[j]; ++mm) {
+                    double MDM = Dy1[mm];
+                    if (fabs(MDM) < (double)TINY_SP) continue;
+                    double prod2 = MDM * prod1;
+                    for (int nn = 0; nn <= n[i] + n[j]; ++nn) {
+                        double H1 = ((ll + mm + nn) & 1) ? -1.0 : 1.0;
+                        double MDN = Dz1[nn];
+                        if (fabs(MDN) < (double)TINY_SP) continue;
+                        double prod3 = MDN * H1 * prod2;
+                        if (fabs(prod3) > cutoffmd) {
+                            if (branch) {
+                                const double *P = P0 + ((size_t)((ll * D0 + mm) * D0 + nn)) * nq;
+                                for (int iq = 0; iq < nq; ++iq) F[iq] = F[iq] + prod3 * P[iq];
+                            } else {
+                                const double *v = hp + (((size_t)ll * D + mm) * D + nn) * DIMH;
+                                for (int t = 0; t < DIMH; ++t) hs[t] = hs[t] + v[t] * prod3;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    if (!branch) {
+        bessel_sum(F, q, nq, h, LLmax, hs);
+        free(hp);
+    }
+}
+
+int orc_nuclei_electron(int ngto, int ng, int maxl, const int *l, const int *m,
+                        const int *n, const int *group, const double *ga,
+                        const double *xx, const double *yy, const double *zz,
+                        const double *Z, int natom, const double *Zn, const double *geom,
+                        const double *q, int nq, double cutoffz, double cutoffmd,
+                        double cutoffcentre, double *tsi)
+{
+    (void)cutoffz;
+    setup_t S;
+    int rc = setup_build(&S, ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, q, nq);
+    if (rc) return rc;
+    quartet_ctx C = {l, m, n, S.gs, S.gc, maxl, ng, S.ddx, S.ddy, S.ddz};
+    double *F = (double *)malloc(sizeof(double) * nq);
+    for (int iq = 0; iq < nq; ++iq) tsi[iq] = 0.0;
+    for (int na = 0; na < natom; ++na)
+        for (int i = 0; i < ng; ++i)
+            for (int k = 0; k < ng; ++k) {
+                double hx = geom[na + (size_t)natom * 0] - S.px[i * ng + k];
+                double hy = geom[na + (size_t)natom * 1] - S.py[i * ng + k];
+                double hz = geom[na + (size_t)natom * 2] - S.pz[i * ng + k];
+                double h = sqrt(hx * hx + hy * hy + hz * hz);
+                k_ij(&C, i, k, h < cutoffcentre, hx, hy, hz, h, Z, ngto, cutoffmd, S.P0, S.D0, q,
+                     nq, F);
+                const double *e = S.e12 + (size_t)(i * ng + k) * nq;
+                for (int iq = 0; iq < nq; ++iq) tsi[iq] = tsi[iq] + Zn[na] * F[iq] * e[iq];
+            }
+    free(F);
+    setup_free(&S);
+    return 0;
+}
+
+/* The nuclear term of types 5/6, src/main_calculation.f90:303-312 (sinc of
+ * src/main_calculation.f90:853-862). */
+void orc_nuclear_nn(int natom, const double *Zn, const double *geom, const double *q, int nq,
+                    double *Inn)
+{
+    for (int iq = 0; iq < nq; ++iq) Inn[iq] = 0.0;
+    for (int i = 0; i < natom; ++i)
+        for (int j = 0; j < natom; ++j)
+            for (int k = 0; k < nq; ++k) {
+                double s2 = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    double d = geom[i + (size_t)natom * c] - geom[j + (size_t)natom * c];
+                    s2 = s2 + d * d;
+                }
+                double a = q[k] * fabs(sqrt(s2));
+                double sinc = fabs(a) < 1.0e-10 ? 1.0 : sin(a) / a;
+                Inn[k] = Inn[k] + Zn[i] * Zn[j] * sinc;
+            }
+}
+
 /* ------------------------------------------------------------ unique_total */
 int orc_unique_total(const int *matrix, int nrow, int ncol, const double *total,
                      int *rmat, double *rtot, int *iuni, int *irec)

@@ -126,6 +126,18 @@ int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
                    const double *Z, const double *q, int nq, double cutoffz,
                    double cutoffmd, double cutoffcentre, int nthreads,
                    double *tsi);
+/* nuclei_electron_integration (src/Integrals.f90:2493-2596): the
+ * electron-nucleus term of types 5/6 with the elastic Z; geom is natom x 3
+ * column-major (bohr). */
+int orc_nuclei_electron(int ngto, int ng, int maxl, const int *l, const int *m,
+                        const int *n, const int *group, const double *ga,
+                        const double *xx, const double *yy, const double *zz,
+                        const double *Z, int natom, const double *Zn, const double *geom,
+                        const double *q, int nq, double cutoffz, double cutoffmd,
+                        double cutoffcentre, double *tsi);
+/* Sum_ij Zn_i Zn_j sinc(q R_ij) (src/main_calculation.f90:303-312). */
+void orc_nuclear_nn(int natom, const double *Zn, const double *geom, const double *q, int nq,
+                    double *Inn);
 /* elastic_integration_j2 (type 8, src/Integrals.f90:1941-2364) with the
  * P0_j2 table; arguments as orc_elastic_j0. */
 int orc_elastic_j2(int ngto, int ng, int maxl, const int *l, const int *m,

@@ -4,7 +4,8 @@
 ! uniquemodule, compiled unmodified from their sources) and calls
 !   set_P0, maxcoincidence, createtwordm, tot_integration,
 !   set_P0_j2, tot_integration_j2, onerdm_creat, elastic_integration,
-!   elastic_integration_j2,
+!   elastic_integration_j2, nuclei_electron_integration and the nuclear sinc
+!   sum of types 5/6 (with calculate_form_factors' sinc),
 !   unique_total
 ! on one input record.  The setup tables of variables_total /
 ! variables_elastic (Ddx/Ddy/Ddz, px/py/pz, e12) are read from the input
@@ -14,6 +15,7 @@
 !
 ! Usage: ref_driver in.bin out.bin   (stream-access, native endian)
 program ref_driver
+    use calculate_form_factors
     use integrals
     use twordms
     use onerdm
@@ -31,6 +33,9 @@ program ref_driver
     real(dp), allocatable :: ddx(:,:,:,:,:), ddy(:,:,:,:,:), ddz(:,:,:,:,:), e12(:,:,:)
     real(dp), allocatable :: p0(:,:,:,:), total(:), rtot(:), z1(:,:,:), z2(:,:,:)
     real(dp), allocatable :: zt(:,:,:), res1(:), res2(:), res3(:), res4(:), orm(:,:), zel(:,:)
+    real(dp), allocatable :: res5(:), res6(:), Zn(:), geom(:,:)
+    integer(ik) :: natom, na, nb, kq
+    real(dp) :: Rij
     real(dp), allocatable :: p0j2(:,:,:,:)
     character(len=512) :: fin, fout
 
@@ -53,6 +58,10 @@ program ref_driver
              ddz(maxl*2+1, maxl+1, maxl+1, ng, ng), e12(nq, ng, ng))
     read(10) ddx, ddy, ddz
     read(10) e12
+    read(10) natom
+    allocate(Zn(natom), geom(natom, 3))
+    read(10) Zn
+    read(10) geom
     close(10)
 
     ! group metadata, src/main_calculation.f90:85-93
@@ -125,6 +134,20 @@ program ref_driver
     call elastic_integration_j2(ng, px, py, pz, l, m, n, p0j2, ddx, ddy, ddz, zel, gs, gc, group, &
                                 cutoffz, cutoffmd, cutoffcentre, q, e12, res4)
 
+    ! electron scattering terms (types 5/6, src/main_calculation.f90:298-312)
+    allocate(res5(nq), res6(nq))
+    call nuclei_electron_integration(Zn, geom, ng, px, py, pz, l, m, n, p0, ddx, ddy, ddz, zel, &
+                                     gs, gc, group, cutoffz, cutoffmd, cutoffcentre, q, e12, res5)
+    res6 = 0.0_dp
+    do na = 1, size(Zn)
+        do nb = 1, size(Zn)
+            do kq = 1, nq
+                Rij = sqrt(sum((geom(na,:) - geom(nb,:))**2))
+                res6(kq) = res6(kq) + Zn(na) * Zn(nb) * sinc(q(kq) * abs(Rij))
+            end do
+        end do
+    end do
+
     call unique_total(mat, total, rmat, rtot)
     cnt = size(rtot)
 
@@ -140,6 +163,8 @@ program ref_driver
     write(11) rtot
     write(11) res3
     write(11) res4
+    write(11) res5
+    write(11) res6
     write(11) p0
     close(11)
 end program ref_driver

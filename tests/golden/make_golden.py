@@ -82,6 +82,10 @@ def write_record(path, inp, q, cutoffs, dolist=1):
             f.write(np.ascontiguousarray(np.transpose(a, (0, 1, 3, 2, 4))).tobytes(order="C"))
         # e12(q, gi, gj): ours is [gi, gj, q]
         f.write(np.ascontiguousarray(np.transpose(e12, (1, 0, 2))).tobytes(order="C"))
+        # nuclei: natom, Zn, geom(natom, 3) column-major (bohr)
+        f.write(i8(len(inp.Zn)))
+        f.write(np.asarray(inp.Zn, dtype=np.float64).tobytes())
+        f.write(np.asfortranarray(np.asarray(inp.geom, dtype=np.float64)).tobytes(order="F"))
 
 
 def read_result(path, nq, ngto, ncol=4, D=None):
@@ -97,11 +101,13 @@ def read_result(path, nq, ngto, ncol=4, D=None):
         rtot = np.frombuffer(f.read(8 * cnt), dtype=np.float64)
         res3 = np.frombuffer(f.read(8 * nq), dtype=np.float64)
         res4 = np.frombuffer(f.read(8 * nq), dtype=np.float64)
+        res5 = np.frombuffer(f.read(8 * nq), dtype=np.float64)
+        res6 = np.frombuffer(f.read(8 * nq), dtype=np.float64)
         p0 = np.frombuffer(f.read(), dtype=np.float64)
     return dict(mat=mat.copy(), total=total.copy(), S_total=res1.copy(),
                 S_elastic=res2.copy(), onerdm=orm.copy(), Z=zel.copy(),
                 rmat=rmat.copy(), rtot=rtot.copy(), S_total_j2=res3.copy(),
-                S_elastic_j2=res4.copy(), P0=p0.copy())
+                S_elastic_j2=res4.copy(), S_ne=res5.copy(), S_nn=res6.copy(), P0=p0.copy())
 
 
 def run_case(name, cutoffs=(1e-15, 1e-20, 0.001)):
@@ -125,7 +131,9 @@ def save(name, inp, q, ref, cutoffs=(1e-15, 1e-20, 0.001)):
         civs=inp.civs, q=q, cutoffs=np.array(cutoffs),
         mat=ref["mat"].astype(np.int16), total=ref["total"], S_total=ref["S_total"],
         S_elastic=ref["S_elastic"], S_total_j2=ref["S_total_j2"],
-        S_elastic_j2=ref["S_elastic_j2"], onerdm=ref["onerdm"],
+        S_elastic_j2=ref["S_elastic_j2"], S_ne=ref["S_ne"], S_nn=ref["S_nn"],
+        Zn=np.asarray(inp.Zn, dtype=np.float64), geom=np.asarray(inp.geom, dtype=np.float64),
+        onerdm=ref["onerdm"],
         rmat=ref["rmat"].astype(np.int16),
         rtot=ref["rtot"], P0_sum=np.array([ref["P0"].sum(), np.abs(ref["P0"]).sum()]))
     return path
@@ -173,11 +181,16 @@ def main(names):
                                inp.zz, Ze, q)
         r4 = ref["S_elastic_j2"]
         rel4 = np.max(np.abs(S4 - r4)[1:] / np.abs(r4)[1:])
+        S5 = oracle.nuclei_electron(inp.l, inp.m, inp.n, inp.group, inp.ga, inp.xx, inp.yy,
+                                    inp.zz, Ze, inp.Zn, inp.geom, q)
+        rel5 = np.max(np.abs(S5 - ref["S_ne"]) / np.max(np.abs(ref["S_ne"])))
+        rel6 = np.max(np.abs(oracle.nuclear_nn(inp.Zn, inp.geom, q) - ref["S_nn"]))
         print(f"{name}: ngto={inp.ngto} ng={inp.ng} nmat={len(tot)} "
               f"mat-equal={np.array_equal(mat, ref['mat'])} "
               f"total-maxrel={np.max(np.abs(tot - ref['total']) / np.abs(ref['total'])):.2e} "
               f"S_total-maxrel={rel:.2e} S_j2-maxrel(q>0)={rel2:.2e} "
-              f"S_el_j2-maxrel(q>0)={rel4:.2e} size={os.path.getsize(path)}B",
+              f"S_el_j2-maxrel(q>0)={rel4:.2e} S_ne-normrel={rel5:.2e} S_nn-maxabs={rel6:.2e} "
+              f"size={os.path.getsize(path)}B",
               flush=True)
 
 

@@ -198,3 +198,36 @@ def test_elastic_j2_list_bit_exact(golden, name):
                                    g["zz"], g["mmod"], g["q"], g["group"], onerdm=g["onerdm"],
                                    cutoffz=cz, cutoffcentre=cc)
     assert scr.shape == ref.shape and np.array_equal(scr, ref)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_nuclei_electron_matches_reference(golden, name):
+    """Ine of electron scattering (nuclei_electron_integration) against the
+    reference, within 1e-10 of max |Ine| (BesselSum's upward recurrence is
+    followed as the reference does, so small q h carries its rounding)."""
+    g = golden(name)
+    cz, cm, cc = g["cutoffs"]
+    res = sc.nuclei_electron(*_args(g), g["mmod"], g["q"], g["group"], g["onerdm"], g["Zn"],
+                             g["geom"], cz, cm, cc)
+    ref = g["S_ne"]
+    err = np.max(np.abs(res - ref)) / np.max(np.abs(ref))
+    rel = np.max(np.abs(res - ref)[1:] / np.abs(ref)[1:])
+    print(f"{name}: Ine normwise err {err:.3e}, per-q (q>0) {rel:.3e}", flush=True)
+    assert err < RTOL_FP64
+
+
+@pytest.mark.parametrize("name", ["h2", "sp3", "spd2"])
+def test_drop_in_types_5_6(golden, name):
+    """Types 5 and 6: Iee - 2 Ine + Inn with the reference's pieces."""
+    g = golden(name)
+    cz, cm, cc = g["cutoffs"]
+    l = g["l"]
+    for typ, iee in ((5, "S_total"), (6, "S_elastic")):
+        _, res = sc.total_scattering_calculation(
+            typ, g["Zn"], g["geom"], 1, 1, 1, int(l.max()), l.size, int(g["group"].max()),
+            g["ga"], l, g["m"], g["n"], g["xx"], g["yy"], g["zz"], g["mmod"], g["q"],
+            g["q"].size, g["group"], cz, cm, cc, g["confs"].astype(np.int32), g["civs"])
+        ref = g[iee] - 2 * g["S_ne"] + g["S_nn"]
+        err = np.max(np.abs(res - ref)) / np.max(np.abs(ref))
+        print(f"{name} type {typ}: normwise err {err:.3e}", flush=True)
+        assert err < RTOL_FP64

@@ -87,6 +87,17 @@ def test_elastic_j2_vs_oracle(case):
     assert rel < RTOL_FP64
 
 
+def test_nuclei_electron_vs_oracle(case):
+    name, inp, q, mat, tot = case
+    g = sc.onerdm(inp.confs, inp.civs)
+    res = sc.nuclei_electron(*_args(inp), q, inp.group, g, inp.Zn, inp.geom)
+    ref = oracle.nuclei_electron(inp.l, inp.m, inp.n, inp.group, inp.ga, inp.xx, inp.yy, inp.zz,
+                                 oracle.elastic_z(g, inp.mmod), inp.Zn, inp.geom, q)
+    err = np.max(np.abs(res - ref)) / np.max(np.abs(ref))
+    print(f"{name}: Ine normwise err {err:.3e}", flush=True)
+    assert err < RTOL_FP64
+
+
 def test_plan_reuse_is_deterministic(case):
     name, inp, q, mat, tot = case
     plan = sc.Plan(1, *_args(inp), q, inp.group, mat=mat, total=tot)

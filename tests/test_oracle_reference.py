@@ -91,6 +91,21 @@ def test_elastic_j2(case):
     assert abs(S[0] - ref[0]) <= 1e-14 * np.max(np.abs(ref))
 
 
+def test_nuclei_electron(case):
+    """nuclei_electron_integration (src/Integrals.f90:2493-2596) with
+    integral_k_ij / integral_ij_pzero / BesselSum, and the nuclear sinc sum
+    of types 5/6 (src/main_calculation.f90:303-312)."""
+    name, g = case
+    gam = oracle.onerdm(g["confs"], g["civs"])
+    Z = oracle.elastic_z(gam, g["mmod"])
+    Ine = oracle.nuclei_electron(g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"], g["yy"],
+                                 g["zz"], Z, g["Zn"], g["geom"], g["q"], **_cut(g))
+    ref = g["S_ne"]
+    assert np.max(np.abs(Ine - ref)) <= 1e-13 * np.max(np.abs(ref)), name
+    Inn = oracle.nuclear_nn(g["Zn"], g["geom"], g["q"])
+    assert np.array_equal(Inn, g["S_nn"]), name
+
+
 def test_unique_total(case):
     """L3: unique_total (src/unique_module.f90:90-145) on the 2-RDM rows."""
     name, g = case

@@ -93,7 +93,7 @@ struct ThistlePlan {
     DevBuf mmod, mat, total, rmat, rtot, hkeys, hmin, hcnt, hslot, uscan;
     DevBuf V, Gs, T, M, Z, Cp, W3, jtab, aend, mopairs, xa, xb, onerdm;
     DevBuf part, partc, S, cls;
-    std::vector<int> gstart, gcount;  // per primitive shell (host)
+    std::vector<int> gstart, gcount, gL;  // per primitive shell (host)
     int64_t cls_off[18] = {0};  // per-LL ranges in `cls` (positions, ref order)
     int cls_tmax[17] = {0};     // largest exponent-tuple count per LL class
     DevBuf hs;                  // h(beta) per quartet
@@ -218,6 +218,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     for (const auto& g : groups) {
         pl->gstart.push_back(g.start);
         pl->gcount.push_back(g.count);
+        pl->gL.push_back(g.L);
     }
     pl->ngto = b->ngto;
     pl->ng = b->ng;
@@ -690,6 +691,83 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
     return THISTLE_OK;
 }
 
+// Electron-nucleus term (nuclei_electron_integration) on an elastic plan whose
+// pair tables, e12 and Z are computed: out_h[q] (host).
+int run_nuclear(ThistlePlan* pl, int natom, const double* Zn, const double* geom,
+                double* out_h) {
+    if (natom <= 0 || !Zn || !geom || !out_h) return THISTLE_EINVAL;
+    std::vector<std::vector<int32_t>> byL(9);
+    for (int gi = 0; gi < pl->ng; ++gi)
+        for (int gj = gi; gj < pl->ng; ++gj) {
+            const int LP = pl->gL[gi] + pl->gL[gj];
+            if (LP > 8) return THISTLE_ELLMAX;
+            byL[LP].push_back(pair_index(gi, gj, pl->ng));
+        }
+    std::vector<int32_t> all;
+    std::vector<int> off(10, 0);
+    for (int LP = 0; LP <= 8; ++LP) {
+        off[LP] = (int)all.size();
+        all.insert(all.end(), byL[LP].begin(), byL[LP].end());
+    }
+    off[9] = (int)all.size();
+    DevBuf pairs, part, zn, geo, out;
+    int rc;
+    if ((rc = dalloc(pairs, sizeof(int32_t) * all.size()))) return rc;
+    if ((rc = dalloc(part, sizeof(double) * all.size() * (size_t)pl->nq))) return rc;
+    if ((rc = dalloc(zn, sizeof(double) * natom))) return rc;
+    if ((rc = dalloc(geo, sizeof(double) * 3 * (size_t)natom))) return rc;
+    if ((rc = dalloc(out, sizeof(double) * pl->nq))) return rc;
+    HCHECK(hipMemcpyAsync(pairs.p, all.data(), sizeof(int32_t) * all.size(),
+                          hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(zn.p, Zn, sizeof(double) * natom, hipMemcpyHostToDevice, pl->stream));
+    HCHECK(hipMemcpyAsync(geo.p, geom, sizeof(double) * 3 * natom, hipMemcpyHostToDevice,
+                          pl->stream));
+    NucArgs a{};
+    a.P = pl->P.as<PairInfo>();
+    a.G = pl->G.as<GroupInfo>();
+    a.E = pl->E.as<double>();
+    a.maxl = pl->maxl;
+    a.ngto = pl->ngto;
+    a.nq = pl->nq;
+    a.lxyz = pl->lxyz.as<char4>();
+    a.Z = pl->Z.as<double>();
+    a.e12 = pl->e12.as<double>();
+    a.q = pl->q.as<double>();
+    a.Zn = zn.as<double>();
+    a.geom = geo.as<double>();
+    a.natom = natom;
+    a.cutoffcentre = pl->cutoffcentre;
+    a.part = part.as<double>();
+    for (int LP = 0; LP <= 8; ++LP)
+        HCHECK(launch_nuc(a, LP, pairs.as<int32_t>() + off[LP], off[LP + 1] - off[LP], off[LP],
+                          pl->stream));
+    HCHECK(launch_rowsum(part.as<double>(), (int)all.size(), pl->nq, out.as<double>(),
+                         pl->stream));
+    HCHECK(hipMemcpyAsync(out_h, out.p, sizeof(double) * pl->nq, hipMemcpyDeviceToHost,
+                          pl->stream));
+    HCHECK(hipStreamSynchronize(pl->stream));
+    return THISTLE_OK;
+}
+
+// Sum_ij Zn_i Zn_j sinc(q |R_i - R_j|) in the reference's loop order
+// (src/main_calculation.f90:303-312, sinc :853-862)
+void nuclear_nn(int natom, const double* Zn, const double* geom, const double* q, int nq,
+                double* Inn) {
+    for (int k = 0; k < nq; ++k) Inn[k] = 0.0;
+    for (int i = 0; i < natom; ++i)
+        for (int j = 0; j < natom; ++j)
+            for (int k = 0; k < nq; ++k) {
+                double s2 = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    const double d = geom[i + (size_t)natom * c] - geom[j + (size_t)natom * c];
+                    s2 = s2 + d * d;
+                }
+                const double a = q[k] * std::fabs(std::sqrt(s2));
+                const double sinc = std::fabs(a) < 1.0e-10 ? 1.0 : std::sin(a) / a;
+                Inn[k] = Inn[k] + Zn[i] * Zn[j] * sinc;
+            }
+}
+
 // Adjoint of one executed type-1 plan: w = dL/dS(q) -> dL/d(shell centre)
 // [ng][3] (gg_host, may be null) and dL/dtotal [nmat] (dtot_host, may be null).
 int run_adjoint(ThistlePlan* pl, const double* w_host, double* gg_host, double* dtot_host) {
@@ -1104,6 +1182,22 @@ int32_t thistle_total_from_twordm(const ThistleBasis* basis, int64_t nmat, const
     return rc;
 }
 
+int32_t thistle_nuclei_electron(const ThistleBasis* basis, int32_t norb, const double* onerdm,
+                                int32_t natom, const double* Zn, const double* geom,
+                                const ThistleOpts* opts, double* result_out) {
+    if (natom <= 0 || !Zn || !geom || !result_out) return THISTLE_EINVAL;
+    ThistlePlan* raw = nullptr;
+    int rc = thistle_plan_create(2, basis, 0, nullptr, nullptr, norb, onerdm, opts, &raw);
+    if (rc) return rc;
+    std::unique_ptr<ThistlePlan> pl(raw);
+    // pair tables, e12 and the elastic Z; no quartet stage
+    rc = run_tables_and_enum(pl.get(), false);
+    if (rc) return rc;
+    HCHECK(launch_zel(pl->mmod.as<double>(), pl->nmo, pl->norb, pl->onerdm.as<double>(),
+                      pl->ngto, pl->Z.as<double>(), pl->stream));
+    return run_nuclear(pl.get(), natom, Zn, geom, result_out);
+}
+
 int32_t thistle_elastic_from_onerdm(const ThistleBasis* basis, int32_t norb, const double* onerdm,
                                     const ThistleOpts* opts, double* result_out) {
     ThistlePlan* pl = nullptr;
@@ -1299,16 +1393,45 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
                                              const int32_t* confs, const double* civecs,
                                              const ThistleOpts* opts, double* q_abs_out,
                                              double* result_out) {
-    (void)natom;
-    (void)Zn;
-    (void)geom;
     (void)state3;
-    if (type != 1 && type != 2 && type != 7 && type != 8) return THISTLE_ETYPE;
+    if (type != 1 && type != 2 && type != 5 && type != 6 && type != 7 && type != 8)
+        return THISTLE_ETYPE;
     if (!basis || !result_out) return THISTLE_EINVAL;
+    if ((type == 5 || type == 6) && (natom <= 0 || !Zn || !geom)) return THISTLE_EINVAL;
     const ThistleOpts* o = opts ? opts : &kDefaultOpts;
     if (q_abs_out)
         for (int i = 0; i < basis->nq; ++i) q_abs_out[i] = 0.0;
     const int norb = lconfs / 2;
+    if (type == 5 || type == 6) {
+        // electron scattering (src/main_calculation.f90:270-330):
+        // Iee - 2 Ine + Inn, Iee total (5) or elastic (6) j0
+        const int nq = basis->nq;
+        std::vector<double> Iee(nq), Ine(nq), Inn(nq);
+        std::vector<double> g((size_t)norb * norb);
+        int rc = thistle_onerdm_host(confs, nconf, lconfs, civecs, nstates, state1, state2,
+                                     g.data());
+        if (rc) return rc;
+        if (type == 5) {
+            rc = thistle_total_scattering_calculation(1, natom, Zn, geom, state1, state2, state3,
+                                                      basis, nconf, lconfs, nstates, confs,
+                                                      civecs, o, nullptr, Iee.data());
+            if (rc) return rc;
+            rc = thistle_nuclei_electron(basis, norb, g.data(), natom, Zn, geom, o, Ine.data());
+            if (rc) return rc;
+        } else {
+            ThistlePlan* raw = nullptr;
+            rc = thistle_plan_create(2, basis, 0, nullptr, nullptr, norb, g.data(), o, &raw);
+            if (rc) return rc;
+            std::unique_ptr<ThistlePlan> pl(raw);
+            rc = thistle_plan_execute(pl.get(), Iee.data());
+            if (rc) return rc;
+            rc = run_nuclear(pl.get(), natom, Zn, geom, Ine.data());
+            if (rc) return rc;
+        }
+        nuclear_nn(natom, Zn, geom, basis->q, nq, Inn.data());
+        for (int k = 0; k < nq; ++k) result_out[k] = Iee[k] - 2 * Ine[k] + Inn[k];
+        return THISTLE_OK;
+    }
     if (type == 1 || type == 7) {
         // createtwordm on the device; at most norb^4 rows, so one pass when
         // that bound is small

@@ -570,6 +570,147 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
     }
 }
 
+// --------------------------------------------- electron-nucleus (types 5/6)
+// integral_k_ij / integral_ij_pzero for the group pair (gi, gj) and each
+// nucleus (src/Integrals.f90:6192-6326,4865-4951), summed as
+// nuclei_electron_integration does (:2540-2590):
+//   h(beta) = sum_{a,b} Z(a,b) (px * py * pz)(beta),
+//   px(m)   = sum_T (-1)^T E^x_ab(T) a(T, 2m-T) hx^(2m-T),  h = R_nucleus - P,
+// the factorisation of BesselDeriv with Hermite_like_coeffs (which equals
+// rrdj0's a(l,p)).  F(q) follows BesselSum (:6329-6368) literally, upward
+// recurrence at every q h as the reference does, or the h = 0 polynomial
+// for h < cutoffcentre.  The reference visits (gi,gj) and (gj,gi) with the
+// same value; a pair gi < gj counts twice.
+template <int LP>
+__global__ void __launch_bounds__(WAVE) k_nuc(NucArgs A, const int32_t* __restrict__ pairs,
+                                              int row0) {
+    constexpr int D = LP + 1;
+    const int lane = threadIdx.x;
+    const int p = pairs[blockIdx.x];
+    const PairInfo PP = A.P[p];
+    const GroupInfo Gi = A.G[PP.gi], Gj = A.G[PP.gj];
+    const double w = PP.gi == PP.gj ? 1.0 : 2.0;
+    const int L1 = A.maxl + 1, T1 = 2 * A.maxl + 1;
+    double* row = A.part + (int64_t)(row0 + blockIdx.x) * A.nq;
+    const double* e12 = A.e12 + (int64_t)p * A.nq;
+    for (int iq = lane; iq < A.nq; iq += WAVE) row[iq] = 0.0;
+    const int nab = Gi.count * Gj.count;
+    for (int na = 0; na < A.natom; ++na) {
+        const double rx = A.geom[na] - PP.Px, ry = A.geom[na + A.natom] - PP.Py,
+                     rz = A.geom[na + 2 * A.natom] - PP.Pz;
+        const double h = sqrt(rx * rx + ry * ry + rz * rz);
+        const bool br = h < A.cutoffcentre;
+        const double hv3[3] = {br ? 0.0 : rx, br ? 0.0 : ry, br ? 0.0 : rz};
+        double hb[D];
+#pragma unroll
+        for (int b = 0; b < D; ++b) hb[b] = 0.0;
+        for (int u = lane; u < nab; u += WAVE) {
+            const int a = u / Gj.count, b = u - a * Gj.count;
+            const int ia = Gi.start + a, ib = Gj.start + b;
+            const double z = A.Z[ia + (int64_t)A.ngto * ib];
+            const char4 ca = A.lxyz[ia], cb = A.lxyz[ib];
+            const int la[3] = {ca.x, ca.y, ca.z}, lb[3] = {cb.x, cb.y, cb.z};
+            double v[3][D];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const double* Er = A.E + (((int64_t)p * 3 + ax) * L1 * L1 + la[ax] * L1 + lb[ax]) * T1;
+                const int tmax = la[ax] + lb[ax];
+                double hp[D];
+                hp[0] = 1.0;
+#pragma unroll
+                for (int k = 1; k < D; ++k) hp[k] = hp[k - 1] * hv3[ax];
+#pragma unroll
+                for (int m = 0; m < D; ++m) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int T = m; T <= 2 * m && T < D; ++T) {
+                        if (T > tmax) continue;
+                        const double c = (T & 1) ? -Er[T] : Er[T];
+                        s += c * (kAcoef.v[T][2 * m - T] * hp[2 * m - T]);
+                    }
+                    v[ax][m] = s;
+                }
+            }
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) {
+                double s = 0.0;
+#pragma unroll
+                for (int mx = 0; mx <= b2; ++mx) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int my = 0; my <= b2 - mx; ++my) t += v[1][my] * v[2][b2 - mx - my];
+                    s += v[0][mx] * t;
+                }
+                hb[b2] = fma(z, s, hb[b2]);
+            }
+        }
+#pragma unroll
+        for (int b2 = 0; b2 < D; ++b2) hb[b2] = wsum(hb[b2]);
+        const double zn = A.Zn[na];
+        for (int iq = lane; iq < A.nq; iq += WAVE) {
+            const double q = A.q[iq];
+            double F;
+            if (br) {
+                F = f_pzero<LP>(hb, q);
+            } else if (q == 0.0) {
+                F = hb[0];  // the q -> 0 limit, where BesselSum divides 0 by 0
+            } else {
+                // BesselSum
+                const double Pmu = h * q;
+                double sx, cx;
+                sincos_cw(Pmu, sx, cx);
+                double h0 = sx / Pmu;
+                F = h0 * hb[0];
+                if (LP == 1) {
+                    const double h1 = (sx / (Pmu * Pmu) - cx / Pmu) * q / h;
+                    F = F + hb[LP >= 1 ? 1 : 0] * h1;
+                } else if (LP > 1) {
+                    const double muOH = q / h;
+                    double h1 = (sx / (Pmu * Pmu) - cx / Pmu) * muOH;
+                    F = F + hb[1] * h1;
+#pragma unroll
+                    for (int ra = 2; ra <= LP; ++ra) {
+                        const double hr = ((double)(2 * ra - 1) / Pmu * h1 - h0 * muOH) * muOH;
+                        F = F + hr * hb[ra];
+                        h0 = h1;
+                        h1 = hr;
+                    }
+                }
+            }
+            row[iq] = row[iq] + w * (zn * F * e12[iq]);
+        }
+    }
+}
+
+__global__ void k_rowsum(const double* __restrict__ part, int rows, int nq,
+                         double* __restrict__ out) {
+    const int iq = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iq >= nq) return;
+    double s = 0.0;
+    for (int r = 0; r < rows; ++r) s += part[(int64_t)r * nq + iq];
+    out[iq] = s;
+}
+
+hipError_t launch_nuc(const NucArgs& a, int LP, const int32_t* pairs, int npairs, int row0,
+                      hipStream_t s) {
+    if (npairs <= 0) return hipSuccess;
+    switch (LP) {
+#define N_(k)                                                                                  \
+    case k:                                                                                    \
+        hipLaunchKernelGGL(k_nuc<k>, dim3(npairs), dim3(WAVE), 0, s, a, pairs, row0);          \
+        break;
+        N_(0) N_(1) N_(2) N_(3) N_(4) N_(5) N_(6) N_(7) N_(8)
+#undef N_
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rowsum(const double* part, int rows, int nq, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_rowsum, dim3((nq + 127) / 128), dim3(128), 0, s, part, rows, nq, out);
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------- launchers
 namespace {
 template <int LL>

@@ -62,6 +62,29 @@ static inline __host__ __device__ int pair_index(int gi, int gj, int ng) {
     return gi * ng - (gi * (gi - 1)) / 2 + (gj - gi);
 }
 
+// Electron-nucleus term of electron scattering (nuclei_electron_integration):
+// per group pair (gi <= gj), every nucleus.  part: [npair_listed][nq].
+struct NucArgs {
+    const PairInfo* P;
+    const GroupInfo* G;
+    const double* E;          // pair MD tables, as SeriesArgs::E
+    int maxl, ngto, nq;
+    const char4* lxyz;
+    const double* Z;          // elastic Z, ngto x ngto
+    const double* e12;        // [npair][nq]
+    const double* q;
+    const double* Zn;         // [natom]
+    const double* geom;       // [natom][3] column-major (natom x 3)
+    int natom;
+    double cutoffcentre;
+    double* part;             // one row of nq per listed pair
+};
+// pairs: pair indices whose L_i + L_j = LP; rows written at row0 + k
+hipError_t launch_nuc(const NucArgs& a, int LP, const int32_t* pairs, int npairs, int row0,
+                      hipStream_t s);
+// out[q] = sum over rows r of part[r][q], rows in order
+hipError_t launch_rowsum(const double* part, int rows, int nq, double* out, hipStream_t s);
+
 // createtwordm on the device (rdm.hip); see the header comment there
 hipError_t rdm_pack(const int32_t* confs, int nconf, int lconfs, int nw, uint64_t* masks,
                     hipStream_t s);

@@ -24,7 +24,7 @@ EXPORTS = [
     "thistle_total_scattering_calculation", "thistle_total_from_twordm",
     "thistle_total_scattering_adjoint",
     "thistle_elastic_from_onerdm", "thistle_twordm_dense", "thistle_twordm_select",
-    "thistle_createtwordm",
+    "thistle_createtwordm", "thistle_nuclei_electron",
     "thistle_onerdm", "thistle_quartet_list", "thistle_quartet_list_screened",
     "thistle_unique_total", "thistle_pair_tables", "thistle_plan_create",
     "thistle_plan_execute", "thistle_plan_result", "thistle_plan_timings",
@@ -89,6 +89,8 @@ def _declare(L):
     L.thistle_twordm_select.restype = i64
     L.thistle_createtwordm.argtypes = [p, i32, i32, p, i32, i32, i32, d, p, p, p, i64]
     L.thistle_createtwordm.restype = i64
+    L.thistle_nuclei_electron.argtypes = [p, i32, p, i32, p, p, p, p]
+    L.thistle_nuclei_electron.restype = i32
     L.thistle_onerdm.argtypes = [p, i32, i32, p, i32, i32, i32, p]
     L.thistle_onerdm.restype = i32
     L.thistle_quartet_list.argtypes = [p, p, i64, p]

@@ -4,8 +4,10 @@
 Fortran procedure (src/main_calculation.f90:13-15, the form with ``state3``
 that ``src/Main.f90:171-173`` calls) and returns ``(q_abs, result)`` like the
 f2py binding used at src/scattering_main_routine.py:240-247.  Types 1 (total
-j0) and 2 (elastic j0) run on the GPU through libthistle_hip.so; the other
-types of the reference are outside this path and raise.
+j0), 2 (elastic j0), 5 / 6 (total / elastic electron scattering), 7 (total
+j2) and 8 (elastic j2) run on the GPU through libthistle_hip.so; the other
+types of the reference (aligned, resolved, j1) are outside this path and
+raise.
 
 Beyond the reference arguments, keyword ``precision`` selects FP64 (default)
 or the mixed-precision kernels, and ``rdm_cutoff`` the |Gamma| selection
@@ -31,15 +33,15 @@ def total_scattering_calculation(type, Zn, geom, state1, state2, state3, maxl, n
                                  cutoffmd, cutoffcentre, confs, civecs, *, precision=L.FP64,
                                  rdm_cutoff=1e-30, waves=0):
     """Drop-in for main_calculation_mod::total_scattering_calculation (types 1, 2,
-    7 and 8)."""
+    5, 6, 7 and 8)."""
     lib = L.load()
     q = np.asarray(q, dtype=np.float64)
     if int(nq) != q.size or int(ngto) != np.asarray(l).size:
         raise ValueError("nq / ngto do not match the array sizes")
-    if int(type) not in (1, 2, 7, 8):
+    if int(type) not in (1, 2, 5, 6, 7, 8):
         raise NotImplementedError(
-            f"type {type}: only 1 (total j0), 2 (elastic j0), 7 (total j2) and 8 (elastic j2) "
-            "are on this path")
+            f"type {type}: only 1 (total j0), 2 (elastic j0), 5/6 (total/elastic electron), "
+            "7 (total j2) and 8 (elastic j2) are on this path")
     b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
                maxl=maxl, ng=ng)
     o = L.make_opts(precision, rdm_cutoff, waves=waves)
@@ -135,6 +137,24 @@ def elastic_from_onerdm(ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm, cutoffz
     rc = lib.thistle_elastic_from_onerdm(C.byref(b), int(g.shape[0]), L._ptr(g), C.byref(o),
                                          L._ptr(res))
     L.check(rc, "elastic_from_onerdm")
+    return res
+
+
+def nuclei_electron(ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm, Zn, geom, cutoffz=1e-15,
+                    cutoffmd=1e-20, cutoffcentre=0.001, maxl=None):
+    """nuclei_electron_integration: the electron-nucleus term Ine of electron
+    scattering (types 5/6) from a 1-RDM; geom (natom, 3) in bohr."""
+    lib = L.load()
+    b = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
+               maxl=maxl)
+    o = L.make_opts()
+    g = np.require(np.asarray(onerdm, dtype=np.float64), requirements=["F"])
+    Zn = np.require(np.atleast_1d(np.asarray(Zn, dtype=np.float64)), requirements=["C"])
+    geom = np.require(np.asarray(geom, dtype=np.float64).reshape(-1, 3), requirements=["F"])
+    res = np.zeros(np.asarray(q).size)
+    rc = lib.thistle_nuclei_electron(C.byref(b), int(g.shape[0]), L._ptr(g), int(Zn.size),
+                                     L._ptr(Zn), L._ptr(geom), C.byref(o), L._ptr(res))
+    L.check(rc, "nuclei_electron")
     return res
 
 
