@@ -140,3 +140,16 @@ def test_z_transforms_agree(case, monkeypatch):
     print(f"{name}: Z transforms max rel diff {rel:.3e}", flush=True)
     assert rel < RTOL_FP64
 
+
+
+def test_stream_overlap_bit_identical(case, monkeypatch):
+    """Contraction and series on two streams (the default) give the same
+    bits as every stage on one stream: the series launches keep one order."""
+    name, inp, q, mat, tot = case
+    out = []
+    for ov in ("0", "1"):
+        monkeypatch.setenv("THISTLE_OVERLAP", ov)
+        out.append(sc.total_from_twordm(*_args(inp), q, inp.group, mat, tot))
+        g = sc.onerdm(inp.confs, inp.civs)
+        out.append(sc.elastic_j2_from_onerdm(*_args(inp), q, inp.group, g))
+    assert np.array_equal(out[0], out[2]) and np.array_equal(out[1], out[3])

@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
                 double Cb = 0.0, Ch = 0.0;
 #pragma unroll
                 for (int b = 0; b < D; ++b) {
-                    const double hb = A.hs[kq * D + b];
+                    const double hb = A.hs[kq * HS_ROW(D) + b];
                     Cb = fma(hb, Gb[b], Cb);
                     Ch = fma(hb, Gb[b + 1], Ch);
                 }

@@ -101,7 +101,18 @@ struct ThistlePlan {
     int nsig = 0;
     hipEvent_t ev[6] = {};
     double ms[5] = {0, 0, 0, 0, 0};
+    // series kernels on a second stream, each behind its class's contraction
+    hipStream_t aux = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, cev[17] = {};
+    void drop_overlap() {
+        for (auto& e : cev)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
+        if (fork_ev) (void)hipEventDestroy(fork_ev), fork_ev = nullptr;
+        if (join_ev) (void)hipEventDestroy(join_ev), join_ev = nullptr;
+        if (aux) (void)hipStreamDestroy(aux), aux = nullptr;
+    }
     ~ThistlePlan() {
+        drop_overlap();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (blas) rocblas_destroy_handle(blas);
@@ -236,6 +247,14 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
         pl->own_stream = true;
     }
     for (auto& e : pl->ev) HCHECK(hipEventCreate(&e));
+    // THISTLE_OVERLAP=0 keeps every stage on the one stream
+    const char* ov = std::getenv("THISTLE_OVERLAP");
+    if (!(ov && ov[0] == '0')) {
+        HCHECK(hipStreamCreateWithFlags(&pl->aux, hipStreamNonBlocking));
+        HCHECK(hipEventCreateWithFlags(&pl->fork_ev, hipEventDisableTiming));
+        HCHECK(hipEventCreateWithFlags(&pl->join_ev, hipEventDisableTiming));
+        for (auto& e : pl->cev) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     int ng = b->ng;
     pl->npair = ng * (ng + 1) / 2;
     std::vector<PairInfo> pairs(pl->npair);
@@ -364,7 +383,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if ((rc = dalloc(pl->xa, sizeof(int32_t) * pl->npp))) return rc;
     if ((rc = dalloc(pl->xb, sizeof(int32_t) * pl->npp))) return rc;
     if ((rc = dalloc(pl->cls, sizeof(int32_t) * Q))) return rc;
-    if ((rc = dalloc(pl->hs, sizeof(double) * 17 * (size_t)Q))) return rc;
+    if ((rc = dalloc(pl->hs, sizeof(double) * HS * (size_t)Q))) return rc;
     if ((rc = dalloc(pl->quadtab, sizeof(uint64_t) * qtab.size()))) return rc;
     if ((rc = dalloc(pl->comboff, sizeof(int64_t) * comboff.size()))) return rc;
     HCHECK(hipMemcpyAsync(pl->quadtab.p, qtab.data(), sizeof(uint64_t) * qtab.size(),
@@ -513,8 +532,19 @@ SeriesArgs series_args(const ThistlePlan* pl) {
     return a;
 }
 
+// The contraction (gather- and latency-bound) and the series (FP64 VALU
+// bound) of different classes run concurrently: contractions on the plan's
+// stream, series on the auxiliary stream, each series launch behind its own
+// class's contraction through an event.  The series launches stay in one
+// stream order, so the partial rows accumulate exactly as on one stream.
 int run_series(ThistlePlan* pl, bool timing) {
     SeriesArgs a = series_args(pl);
+    const bool ov = pl->aux != nullptr;
+    hipStream_t ss = ov ? pl->aux : pl->stream;
+    if (ov) {
+        HCHECK(hipEventRecord(pl->fork_ev, pl->stream));
+        HCHECK(hipStreamWaitEvent(pl->aux, pl->fork_ev, 0));
+    }
     // h(beta) of every quartet (q independent), heaviest classes first
     for (int LL = 16; LL >= 0; --LL) {
         int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
@@ -523,24 +553,28 @@ int run_series(ThistlePlan* pl, bool timing) {
         a.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
         HCHECK(launch_contract(a, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
                                n, blocks, pl->stream));
+        if (ov) HCHECK(hipEventRecord(pl->cev[LL], pl->stream));
     }
     for (int q0 = 0; q0 < pl->nq; q0 += 512) {
         a.q0 = q0;
         a.nqc = std::min(512, pl->nq - q0);
         const int64_t rows = (int64_t)pl->waves * a.nqc;
-        HCHECK(launch_fill(pl->part.as<double>(), rows, 0.0, pl->stream));
-        HCHECK(launch_fill(pl->partc.as<double>(), rows, 0.0, pl->stream));
+        HCHECK(launch_fill(pl->part.as<double>(), rows, 0.0, ss));
+        HCHECK(launch_fill(pl->partc.as<double>(), rows, 0.0, ss));
         for (int LL = 16; LL >= 0; --LL) {
             int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
             if (n <= 0) continue;
+            if (ov && q0 == 0) HCHECK(hipStreamWaitEvent(ss, pl->cev[LL], 0));
             int w = (int)std::min<int64_t>(pl->waves, n);
             a.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
             // j2 carries its series two orders further (bessels0rr(LLmax+2))
-            HCHECK(launch_series(a, a.j2 ? LL + 2 : LL, pl->cls.as<int32_t>() + pl->cls_off[LL], n,
-                                 w, 0, pl->stream));
+            HCHECK(launch_series(a, a.j2 ? LL + 2 : LL, n, w, 0, ss));
         }
-        HCHECK(launch_reduce(a.part, a.partc, pl->waves, a.nqc, pl->S.as<double>() + q0,
-                             pl->stream));
+        HCHECK(launch_reduce(a.part, a.partc, pl->waves, a.nqc, pl->S.as<double>() + q0, ss));
+    }
+    if (ov) {
+        HCHECK(hipEventRecord(pl->join_ev, pl->aux));
+        HCHECK(hipStreamWaitEvent(pl->stream, pl->join_ev, 0));
     }
     if (timing) {
         // stage 4 is empty since the screened quartet list is compacted on
@@ -1024,6 +1058,7 @@ int32_t thistle_batch_create(int32_t nframes, const ThistleBasis* bases, const i
         int rc = thistle_plan_create(1, &bases[f], nmat[f], mats[f], totals[f], 0, nullptr, &of,
                                      &raw);
         if (rc) return rc;
+        raw->drop_overlap();  // the frames already overlap across the batch streams
         b->plans.emplace_back(raw);
         b->soff.push_back(b->nqtot);
         b->nqtot += bases[f].nq;

@@ -205,6 +205,14 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
         // the values of h(beta) (j0) or of the j2 series / h = 0 polynomial
         // coefficients (DO of them), given the reduced sums in hr / gr
         constexpr int DO = J2 ? D + 2 : D;
+        double* rec = A.hs + kq * HS_ROW(DO);
+        // the series kernel's per-quartet scalars, written by one lane
+        auto tail = [&]() {
+            const double hq = sqrt(rx * rx + ry * ry + rz * rz);
+            rec[DO] = hq;
+            rec[DO + 1] = br ? 0.0 : rcp64(hq);
+            rec[DO + 2] = __longlong_as_double((long long)HS_META(pb, pk, QE_MULT(ent), br));
+        };
         auto coef = [&](int b, const double* hr, const double* grr) -> double {
             if constexpr (!J2) {
                 return hr[b];
@@ -237,8 +245,11 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
 #pragma unroll
                 for (int b = 0; b < DO; ++b)
                     if (lg == b) mine = coef(b, h[0], h[NA - 1]);
-                if (lg < DO) A.hs[kq * DO + lg] = mine;
-                if (lg == 0) A.flags[e] = (bal & gmask) ? 1 : 0;
+                if (lg < DO) rec[lg] = mine;
+                if (lg == 0) {
+                    tail();
+                    A.flags[e] = (bal & gmask) ? 1 : 0;
+                }
             }
         } else {
 #pragma unroll
@@ -268,9 +279,10 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
 #pragma unroll
                 for (int b = 0; b < DO; ++b)
                     if (tid == b) mine = coef(b, hr, grr);
-                A.hs[kq * DO + tid] = mine;
+                rec[tid] = mine;
             }
             if (valid && tid == 0) {
+                tail();
                 int f = 0;
 #pragma unroll
                 for (int w = 0; w < NW; ++w) f |= anyw[w];
@@ -470,8 +482,10 @@ __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const doubl
 }
 
 template <int LL, bool MIXED, bool J2>
-__global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __restrict__ idx,
-                                                 int64_t nidx, int first) {
+__global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int first) {
+    constexpr int R = HS_ROW(LL + 1);
+    constexpr int NB = 32;  // quartet records staged in LDS per batch
+    __shared__ double recs[NB * R];
     const int lane = threadIdx.x;
     int iqs[SQ];
     double qv[SQ], qiv[SQ], S[SQ], Sc[SQ];
@@ -483,73 +497,79 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, const int32_t* __
         qiv[s] = qv[s] > 0.0 ? rcp64(qv[s]) : 0.0;
         S[s] = Sc[s] = 0.0;
     }
-    // a contiguous chunk of the class list per wave: consecutive quartets
-    // of one nest share their bra pair, whose e12 row stays in registers
+    // a contiguous chunk of the class's records per wave, staged NB at a time
+    // by coalesced loads; each record then is an LDS broadcast
     const int64_t k0 = (int64_t)blockIdx.x * nidx / gridDim.x,
                   k1 = ((int64_t)blockIdx.x + 1) * nidx / gridDim.x;
-    int pbl = -1;
-    double ebv[SQ];
-    for (int64_t kq = k0; kq < k1; ++kq) {
-        const int64_t e = idx[kq];
-        const uint64_t ent = A.qlist[e];
-        const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
-        const int br = QE_BR(ent);
-        const double mult = (double)QE_MULT(ent);
-        const int pb = pair_index(gi, gj, A.ng), pk = pair_index(gk, gr, A.ng);
-        const double hx = A.P[pk].Px - A.P[pb].Px, hy = A.P[pk].Py - A.P[pb].Py,
-                     hz = A.P[pk].Pz - A.P[pb].Pz;
-        const double h = sqrt(hx * hx + hy * hy + hz * hz);
-        const double invh = br ? 0.0 : rcp64(h);
-        const double ih2 = invh * invh;
-        double cb[LL + 1];
+    for (int64_t kb = k0; kb < k1; kb += NB) {
+        const int nb = (int)(k1 - kb < NB ? k1 - kb : NB);
+        const double* src = A.hs + kb * R;
+        __syncthreads();  // the previous batch is consumed
+        for (int t = lane; t < nb * R; t += WAVE) recs[t] = src[t];
+        __syncthreads();
+        for (int j = 0; j < nb; ++j) {
+            // the record is wave-uniform: kept in scalar registers
+            const double* rc = recs + j * R;
+            double hs[LL + 1];
 #pragma unroll
-        for (int b = 0; b <= LL; ++b) cb[b] = (double)(2 * b + 1) * ih2;
-        double hs[LL + 1];
-#pragma unroll
-        for (int b = 0; b <= LL; ++b) hs[b] = A.hs[kq * (LL + 1) + b];
-        if (pb != pbl) {
+            for (int b = 0; b <= LL; ++b) hs[b] = uniform(rc[b]);
+            const double h = uniform(rc[LL + 1]), invh = uniform(rc[LL + 2]);
+            const uint64_t meta = (uint64_t)__double_as_longlong(uniform(rc[LL + 3]));
+            const int pb = (int)(meta & 0xFFFFFu), pk = (int)((meta >> 20) & 0xFFFFFu);
+            const double mult = (double)((meta >> 40) & 15u);
+            const int br = (int)((meta >> 44) & 1u);
+            // both pair factors of the lane's q points, issued before the
+            // series so that their latency hides behind it
             const double* eb = A.e12 + (int64_t)pb * A.nq;
+            const double* ek = A.e12 + (int64_t)pk * A.nq;
+            double ebv[SQ], ekv[SQ];
 #pragma unroll
-            for (int s = 0; s < SQ; ++s) ebv[s] = eb[iqs[s]];
-            pbl = pb;
-        }
-        const double* ek = A.e12 + (int64_t)pk * A.nq;
-        // sin, cos of q h on the lane's SQ q points.  On a uniform grid the
-        // points are x0 + s D, D = 64 dq h, so one sincos of x0, one of D and
-        // rotations give them all; a q[0] off the grid gets its own.
-        double sxs[SQ], cxs[SQ];
-#pragma unroll
-        for (int s = 0; s < SQ; ++s) sxs[s] = cxs[s] = 0.0;
-        if (!br) {
-            if (A.qlin) {
-                const bool off = A.q0off && iqs[0] == 0;
-                double sb, cbb;
-                sincos_cw(off ? A.qa * h : qv[0] * h, sb, cbb);
-                sxs[0] = sb;
-                cxs[0] = cbb;
-                if (off) sincos_cw(qv[0] * h, sxs[0], cxs[0]);
-                double sd, cd;
-                sincos_cw((double)WAVE * A.dq * h, sd, cd);
-#pragma unroll
-                for (int s = 1; s < SQ; ++s) {
-                    const double ns = fma(sb, cd, cbb * sd), nc = fma(cbb, cd, -(sb * sd));
-                    sb = ns;
-                    cbb = nc;
-                    sxs[s] = sb;
-                    cxs[s] = cbb;
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < SQ; ++s) sincos_cw(qv[s] * h, sxs[s], cxs[s]);
+            for (int s = 0; s < SQ; ++s) {
+                ebv[s] = eb[iqs[s]];
+                ekv[s] = ek[iqs[s]];
             }
-        }
+            const double ih2 = invh * invh;
+            double cb[LL + 1];
 #pragma unroll
-        for (int s = 0; s < SQ; ++s) {
-            double v = mult * f_eval<LL, MIXED, J2>(hs, cb, A.hs + kq * (LL + 1), qv[s], qiv[s], h,
-                                                    invh, br, sxs[s], cxs[s]);
-            v = v * ebv[s];
-            v = v * ek[iqs[s]];
-            kahan(S[s], Sc[s], v);
+            for (int b = 0; b <= LL; ++b) cb[b] = (double)(2 * b + 1) * ih2;
+            // sin, cos of q h on the lane's SQ q points.  On a uniform grid the
+            // points are x0 + s D, D = 64 dq h, so one sincos of x0, one of D and
+            // rotations give them all; a q[0] off the grid gets its own.
+            double sxs[SQ], cxs[SQ];
+#pragma unroll
+            for (int s = 0; s < SQ; ++s) sxs[s] = cxs[s] = 0.0;
+            if (!br) {
+                if (A.qlin) {
+                    const bool off = A.q0off && iqs[0] == 0;
+                    double sb, cbb;
+                    sincos_cw(off ? A.qa * h : qv[0] * h, sb, cbb);
+                    sxs[0] = sb;
+                    cxs[0] = cbb;
+                    if (off) sincos_cw(qv[0] * h, sxs[0], cxs[0]);
+                    double sd, cd;
+                    sincos_cw((double)WAVE * A.dq * h, sd, cd);
+#pragma unroll
+                    for (int s = 1; s < SQ; ++s) {
+                        const double ns = fma(sb, cd, cbb * sd), nc = fma(cbb, cd, -(sb * sd));
+                        sb = ns;
+                        cbb = nc;
+                        sxs[s] = sb;
+                        cxs[s] = cbb;
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < SQ; ++s) sincos_cw(qv[s] * h, sxs[s], cxs[s]);
+                }
+            }
+            const double* hsg = A.hs + (kb + j) * R;
+#pragma unroll
+            for (int s = 0; s < SQ; ++s) {
+                double v = mult * f_eval<LL, MIXED, J2>(hs, cb, hsg, qv[s], qiv[s], h, invh, br,
+                                                        sxs[s], cxs[s]);
+                v = v * ebv[s];
+                v = v * ekv[s];
+                kahan(S[s], Sc[s], v);
+            }
         }
     }
 #pragma unroll
@@ -776,25 +796,25 @@ hipError_t set_contract_lds_limit(size_t bytes) {
     return hipSuccess;
 }
 
-hipError_t launch_series(const SeriesArgs& a, int LL, const int32_t* idx, int64_t nidx,
-                         int waves, int first, hipStream_t s) {
+hipError_t launch_series(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,
+                         hipStream_t s) {
     if (nidx <= 0) return hipSuccess;
     dim3 grid(waves, (a.nqc + SQ * WAVE - 1) / (SQ * WAVE));
     switch (LL) {
-#define CASE(k)                                                                            \
-    case k:                                                                                \
-        if (a.j2 && a.mixed)                                                               \
-            hipLaunchKernelGGL((k_series<k, true, true>), grid, dim3(WAVE), 0, s, a, idx, nidx,  \
-                               first);                                                     \
-        else if (a.j2)                                                                     \
-            hipLaunchKernelGGL((k_series<k, false, true>), grid, dim3(WAVE), 0, s, a, idx, nidx, \
-                               first);                                                     \
-        else if (a.mixed)                                                                  \
-            hipLaunchKernelGGL((k_series<k, true, false>), grid, dim3(WAVE), 0, s, a, idx, nidx, \
-                               first);                                                     \
-        else                                                                               \
-            hipLaunchKernelGGL((k_series<k, false, false>), grid, dim3(WAVE), 0, s, a, idx,     \
-                               nidx, first);                                               \
+#define CASE(k)                                                                              \
+    case k:                                                                                  \
+        if (a.j2 && a.mixed)                                                                 \
+            hipLaunchKernelGGL((k_series<k, true, true>), grid, dim3(WAVE), 0, s, a, nidx,   \
+                               first);                                                       \
+        else if (a.j2)                                                                       \
+            hipLaunchKernelGGL((k_series<k, false, true>), grid, dim3(WAVE), 0, s, a, nidx,  \
+                               first);                                                       \
+        else if (a.mixed)                                                                    \
+            hipLaunchKernelGGL((k_series<k, true, false>), grid, dim3(WAVE), 0, s, a, nidx,  \
+                               first);                                                       \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_series<k, false, false>), grid, dim3(WAVE), 0, s, a, nidx, \
+                               first);                                                       \
         break;
         CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)

@@ -35,10 +35,18 @@ struct EnumRow {
     int64_t out;            // first output slot
 };
 
-// h(beta) buffer: the rows of LL class c start at (class start) * HS doubles
-// and are packed at the stride of the class's kernels (LL + 1, j2 LL + 3),
-// in class-list order.
-#define HS 17
+// Per-quartet records written by the contraction and read by the series
+// kernel, in class-list order: the DO series coefficients (DO = LL + 1, j2
+// LL + 3), then h = |P_ket - P_bra|, 1/h (0 on the h = 0 branch) and the
+// packed pair indices and flags of HS_META.  The records of LL class c start
+// at (class start) * HS doubles.
+#define HS_ROW(DO) ((DO) + 3)
+#define HS HS_ROW(17)
+// record word DO + 2: bra pair (bits 0-19), ket pair (20-39), multiplicity
+// (40-43), h = 0 branch (44)
+#define HS_META(pb, pk, mult, br)                                                \
+    ((uint64_t)(pb) | ((uint64_t)(pk) << 20) | ((uint64_t)(mult) << 40) |        \
+     ((uint64_t)(br) << 44))
 
 // The quartet of element x of an enumeration row, and its multiplicity.
 // Nests 1-5: tot_integration / elastic_integration / tot_integration_j2
@@ -188,9 +196,10 @@ size_t contract_lds_bytes(int LL, int Tmax);
 hipError_t set_contract_lds_limit(size_t bytes);
 hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t* idx,
                            int64_t nidx, int blocks, hipStream_t s);
-// k_series<LL>: S(q) of one LL class; first = 1 overwrites the partial rows
-hipError_t launch_series(const SeriesArgs& a, int LL, const int32_t* idx,
-                         int64_t nidx, int waves, int first, hipStream_t s);
+// k_series<LL>: S(q) of one LL class from its nidx quartet records (a.hs at
+// the class start); first = 1 overwrites the partial rows
+hipError_t launch_series(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,
+                         hipStream_t s);
 hipError_t launch_reduce(const double* part, const double* partc, int waves,
                          int nqc, double* out, hipStream_t s);
 hipError_t launch_compact(const uint8_t* flags, const uint64_t* in, int64_t n,
