@@ -427,7 +427,7 @@ __global__ void k_zel(const double* __restrict__ mmod, int nmo, int norb,
 
 // clears and copies as kernels, so that captured graphs hold no memset nodes
 // C(0:norb, a) padded with zeros to NO doubles per GTO a, so that the
-// factorised transform reads each column with uniform (scalar) loads.
+// factorised transform stages whole columns into LDS with contiguous loads.
 __global__ void k_cpad(const double* __restrict__ C, int nmo, int norb, int ngto, int NO,
                        double* __restrict__ Cp) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -442,36 +442,47 @@ __global__ void k_cpad(const double* __restrict__ C, int nmo, int norb, int ngto
 // sum_y T(y, i) Gs(x, y):
 //   W3(p, b, i) = sum_s Mo(p, s, i) C_sb,  Mo(p, s, i) = M^T(i, pair(p, s)) (x2 if p == s)
 //   Z(i, j)     = sum_p C_{p a_j} W3(p, b_j, i)      (j = primitive pair (a_j, b_j))
-// W3 is laid out [p][b][i] (i fastest).  One thread per i, one block row per p.
+// W3 is laid out [p][b][i] (i fastest).  One thread per i, one block row per
+// p: Mo(p, :, i) stays in registers while the C columns b are staged through
+// LDS ZF_CH at a time (measured faster than scalar loads of each column).
+#define ZF_T 256  // threads per block of the transform kernels
+#define ZF_CH 16  // C columns per LDS stage
 template <int NO>
-__global__ void __launch_bounds__(256) k_w3(const double* __restrict__ Mt,
-                                            const double* __restrict__ Cp, int norb, int ngto,
-                                            int64_t npp, double* __restrict__ W3) {
-    const int p = blockIdx.y;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npp) return;
+__global__ void __launch_bounds__(ZF_T) k_w3(const double* __restrict__ Mt,
+                                             const double* __restrict__ Cp, int norb, int ngto,
+                                             int64_t npp, double* __restrict__ W3) {
+    __shared__ double cs[ZF_CH * NO];
+    const int p = blockIdx.y, tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * ZF_T + tid;
+    const bool live = i < npp;
     double m[NO];
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
         m[s] = 0.0;
-        if (s < norb) {
+        if (s < norb && live) {
             const int x = p < s ? pair_index(p, s, norb) : pair_index(s, p, norb);
             const double v = Mt[i + npp * x];
             m[s] = s == p ? 2.0 * v : v;
         }
     }
     double* out = W3 + i + npp * (int64_t)ngto * p;
-    for (int b = 0; b < ngto; ++b) {
-        const double* c = Cp + (int64_t)b * NO;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int b0 = 0; b0 < ngto; b0 += ZF_CH) {
+        const int nb = ngto - b0 < ZF_CH ? ngto - b0 : ZF_CH;
+        __syncthreads();  // the previous stage is consumed
+        for (int t = tid; t < nb * NO; t += ZF_T) cs[t] = Cp[(int64_t)b0 * NO + t];
+        __syncthreads();
+        for (int bb = 0; bb < nb; ++bb) {
+            const double* c = cs + bb * NO;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-        for (int s = 0; s < NO; s += 4) {
-            s0 = fma(m[s], c[s], s0);
-            s1 = fma(m[s + 1], c[s + 1], s1);
-            s2 = fma(m[s + 2], c[s + 2], s2);
-            s3 = fma(m[s + 3], c[s + 3], s3);
+            for (int s = 0; s < NO; s += 4) {
+                s0 = fma(m[s], c[s], s0);
+                s1 = fma(m[s + 1], c[s + 1], s1);
+                s2 = fma(m[s + 2], c[s + 2], s2);
+                s3 = fma(m[s + 3], c[s + 3], s3);
+            }
+            if (live) out[npp * (int64_t)(b0 + bb)] = (s0 + s1) + (s2 + s3);
         }
-        out[npp * b] = (s0 + s1) + (s2 + s3);
     }
 }
 
@@ -482,14 +493,14 @@ typedef const __attribute__((address_space(4))) double* cdouble_p;
 // the columns j = jtab[b][a] (a < aend[b]; j increases with a) of the pairs
 // whose second GTO is b.  Writes are contiguous along i.
 template <int NO>
-__global__ void __launch_bounds__(256) k_zform(const double* __restrict__ W3,
-                                               const double* __restrict__ Cp,
-                                               const int32_t* __restrict__ jtab,
-                                               const int32_t* __restrict__ aend_of, int norb,
-                                               int ngto, int64_t npp, double* __restrict__ Z) {
+__global__ void __launch_bounds__(ZF_T) k_zform(const double* __restrict__ W3,
+                                                const double* __restrict__ Cp,
+                                                const int32_t* __restrict__ jtab,
+                                                const int32_t* __restrict__ aend_of, int norb,
+                                                int ngto, int64_t npp, double* __restrict__ Z) {
     const int b = blockIdx.y;
-    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
-    const int64_t ilast = i0 + blockDim.x - 1;
+    const int64_t i0 = (int64_t)blockIdx.x * ZF_T;
+    const int64_t ilast = i0 + ZF_T - 1;
     const int32_t* jb = jtab + (int64_t)b * ngto;
     if (jb[0] > ilast) return;  // the whole tile lies above the diagonal
     const int64_t i = i0 + threadIdx.x;
@@ -501,6 +512,7 @@ __global__ void __launch_bounds__(256) k_zform(const double* __restrict__ W3,
     const int aend = aend_of[b];
     // C through the constant address space: uniform reads become scalar
     // loads even where the compiler sinks the sums into the divergent store
+    // (measured faster here than staging C through LDS)
     const cdouble_p Cc = (cdouble_p)Cp;
     for (int a = 0; a < aend; ++a) {
         const int64_t j = jb[a];
@@ -529,12 +541,12 @@ hipError_t launch_zfact(const double* Mt, const double* C, int nmo, int norb, in
     if (NO == 0 || npp <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cpad, dim3((ngto * NO + 255) / 256), dim3(256), 0, s, C, nmo, norb,
                        ngto, NO, Cp);
-    const unsigned nt = (unsigned)((npp + 255) / 256);
+    const unsigned nt = (unsigned)((npp + ZF_T - 1) / ZF_T);
 #define ZF(NOV)                                                                              \
     case NOV:                                                                                \
-        hipLaunchKernelGGL(k_w3<NOV>, dim3(nt, (unsigned)norb), dim3(256), 0, s, Mt, Cp, norb, \
-                           ngto, npp, W3);                                                   \
-        hipLaunchKernelGGL(k_zform<NOV>, dim3(nt, (unsigned)ngto), dim3(256), 0, s, W3, Cp,   \
+        hipLaunchKernelGGL(k_w3<NOV>, dim3(nt, (unsigned)norb), dim3(ZF_T), 0, s, Mt, Cp,      \
+                           norb, ngto, npp, W3);                                             \
+        hipLaunchKernelGGL(k_zform<NOV>, dim3(nt, (unsigned)ngto), dim3(ZF_T), 0, s, W3, Cp,   \
                            jtab, aend_of, norb, ngto, npp, Z);                               \
         break;
     switch (NO) {

@@ -54,8 +54,9 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
     __shared__ int anyw[NW];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const int grp = tid / G, lg = tid % G;
-    double* PX = sm + (size_t)grp * 3 * tmax * D;  // [3][T][D] of this group's quartet
-    double* red = sm + (size_t)NG * 3 * tmax * D;  // [NA][NW][D] when a quartet spans waves
+    constexpr int DP = ContractShape<LL>::DP;
+    double* PX = sm + (size_t)grp * 3 * tmax * DP;  // [3][T][DP] of this group's quartet
+    double* red = sm + (size_t)NG * 3 * tmax * DP;  // [NA][NW][D] when a quartet spans waves
     const int L1 = A.maxl + 1, T1 = 2 * A.maxl + 1;
 
     // grid-stride over the class list: neighbouring quartets go to
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
             hp[0] = 1.0;
 #pragma unroll
             for (int k = 1; k < D; ++k) hp[k] = hp[k - 1] * hax;
-            double* out = PX + ((int64_t)ax * T + t) * D;
+            double* out = PX + ((int64_t)ax * T + t) * DP;
 #pragma unroll
             for (int m = 0; m < D; ++m) {
                 double s = 0.0;
@@ -136,9 +137,9 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
         auto quad = [&](uint64_t w, double z) {
             const int tx = (int)((w >> 16) & 1023u), ty = (int)((w >> 26) & 1023u),
                       tz = (int)((w >> 36) & 1023u);
-            const double* px = PX + tx * D;
-            const double* py = PX + (T + ty) * D;
-            const double* pz = PX + (2 * T + tz) * D;
+            const double* px = PX + tx * DP;
+            const double* py = PX + (T + ty) * DP;
+            const double* pz = PX + (2 * T + tz) * DP;
             double vy[D], vz[D], vx[D];
 #pragma unroll
             for (int m = 0; m < D; ++m) {
@@ -735,9 +736,10 @@ hipError_t launch_rowsum(const double* part, int rows, int nq, double* out, hipS
 namespace {
 template <int LL>
 size_t contract_lds_for(int Tmax) {
-    constexpr int D = LL + 1, NG = ContractShape<LL>::NG, NW = ContractShape<LL>::CT / WAVE;
+    constexpr int D = LL + 1, DP = ContractShape<LL>::DP, NG = ContractShape<LL>::NG,
+                  NW = ContractShape<LL>::CT / WAVE;
     // one quartet per block: h and (for j2) g partial sums per wave
-    return sizeof(double) * ((size_t)NG * 3 * Tmax * D + (NG == 1 ? (size_t)2 * NW * D : 0));
+    return sizeof(double) * ((size_t)NG * 3 * Tmax * DP + (NG == 1 ? (size_t)2 * NW * D : 0));
 }
 }  // namespace
 

@@ -143,6 +143,10 @@ struct ContractShape {
     static constexpr int CT = LL < 10 ? 64 : 256;
     static constexpr int G = LL <= 1 ? 4 : LL == 2 ? 8 : LL == 3 ? 16 : LL == 4 ? 32 : CT;
     static constexpr int NG = CT / G;
+    // row stride of the px/py/pz tables: odd, so that rows gathered by the
+    // lanes of a wave spread over all LDS banks (an even stride of 8 doubles
+    // maps every row onto two bank positions)
+    static constexpr int DP = (LL + 1) | 1;
 };
 
 // Upward recurrence from j0, j1 keeps every j_b (b <= LL) within 1e-14 of
