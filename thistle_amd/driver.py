@@ -68,6 +68,28 @@ def prepare_inputs(mldfile, punfile, closed) -> ScatteringInputs:
         confs=readers.confs_to_int(confs), civs=readers.civs_to_array(civs))
 
 
+def prepare_inputs_molcas(mldfile, logfile, caspt2=False) -> ScatteringInputs:
+    """The same inputs from a Molcas molden file and the CI vector printed in
+    a Molcas RASSCF (or, with ``caspt2``, CASPT2) log: the commented Molcas
+    route of src/scattering_main_routine.py:117-119,136 with the Molcas
+    molden rules (src/molden_reader_nikola_molcas.py) and the CSF to
+    determinant conversion of src/molcas_ci_reader.py:316-338."""
+    from . import molcas
+    civs, confs = molcas.get_civs_and_confs(logfile, caspt2)
+    nmo_max = len(confs[0]) / 2
+    gtos, atoms = readers.read_orbitals(mldfile, N=nmo_max, decontract=True, flavour="molcas")
+    return ScatteringInputs(
+        Zn=atoms.atomic_numbers(), geom=atoms.geometry(),
+        ga=np.asarray(gtos.ga, dtype=np.float64),
+        l=np.asarray(gtos.l, dtype=np.int32), m=np.asarray(gtos.m, dtype=np.int32),
+        n=np.asarray(gtos.n, dtype=np.int32),
+        xx=np.asarray(gtos.x, dtype=np.float64), yy=np.asarray(gtos.y, dtype=np.float64),
+        zz=np.asarray(gtos.z, dtype=np.float64),
+        mmod=np.ascontiguousarray(np.transpose(gtos.mo), dtype=np.float64),
+        group=np.asarray(gtos.group, dtype=np.int32),
+        confs=readers.confs_to_int(confs), civs=readers.civs_to_array(civs))
+
+
 def default_q(qmax=10.0, nq=200):
     q = np.linspace(0.00000000001, qmax, nq)
     if q[0] < 1e-10:

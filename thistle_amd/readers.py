@@ -1,8 +1,13 @@
 """Input readers of the scattering path, behaviour-compatible with the reference.
 
-* Molden (Molpro flavour): ``read_orbitals`` restates
-  ``src/molden_reader_nikola.py:496-508`` (``_read_MO_GTOs`` :470-493) with the
-  same parsing rules, MO ordering, primitive normalisation and decontraction.
+* Molden: ``read_orbitals`` restates ``src/molden_reader_nikola.py:496-508``
+  (``_read_MO_GTOs`` :470-493) with the same parsing rules, MO ordering,
+  primitive normalisation and decontraction.  ``flavour`` selects the rules
+  of the program-specific copies of that reader: "molcas"
+  (``src/molden_reader_nikola_molcas.py``), "pyscf"
+  (``src/molden_reader_nikola_pyscf.py``) and "bagel"
+  (``src/molden_reader_nikola_bagel.py``); the differences are listed in
+  ``FLAVOURS``.
 * CI punch file: ``twordmconst`` restates ``src/twordm_red.py:12-139`` and
   ``confs_to_int`` the string -> integer conversion of
   ``src/scattering_main_routine.py:232-238``.
@@ -107,33 +112,72 @@ class ArrayGTO:
     syms: np.ndarray | None = None
 
 
-def _read_atoms(lines):
-    """[Atoms] block; coordinates are always divided by AU2ANG (:80-115)."""
+# Per-program rules of the molden readers.  atoms: how the [Atoms] block
+# starts and ends, and whether its coordinates are divided by AU2ANG; gto_end:
+# what ends the [GTO] block after a blank line; shells: the shell letters the
+# program's reader expands; mo: the [MO] block rules (see _read_mo).
+FLAVOURS = {
+    # src/molden_reader_nikola.py:91-115,163-177,258-344
+    "molpro": dict(atoms_start=lambda ln: "[Atoms]" in ln, atoms_end=lambda ln: "[GTO]" in ln,
+                   to_bohr=True, gto_end="blank", shells="spdfg", mo="molpro"),
+    # src/molden_reader_nikola_molcas.py:92-113,166-183,274-323
+    "molcas": dict(atoms_start=lambda ln: "[ATOMS]" in ln.upper(), atoms_end=lambda ln: "[" in ln,
+                   to_bohr=False, gto_end="[MO]", shells="spdf", mo="molcas"),
+    # src/molden_reader_nikola_pyscf.py:109-111,168-181,296-344
+    "pyscf": dict(atoms_start=lambda ln: "[Atoms]" in ln, atoms_end=lambda ln: "[GTO]" in ln,
+                  to_bohr=False, gto_end="[6d]", shells="spdfg", mo="pyscf"),
+    # src/molden_reader_nikola_bagel.py:222,279-324
+    "bagel": dict(atoms_start=lambda ln: "[Atoms]" in ln, atoms_end=lambda ln: "[GTO]" in ln,
+                  to_bohr=True, gto_end="blank", shells="spdf", mo="bagel"),
+}
+
+
+def _flavour(flavour):
+    try:
+        return FLAVOURS[flavour]
+    except KeyError:
+        raise ValueError(f"molden flavour must be one of {sorted(FLAVOURS)}, not {flavour!r}")
+
+
+def _read_atoms(lines, flavour="molpro"):
+    """[Atoms] block (src/molden_reader_nikola.py:80-115 and the copies'
+    variants): the Molpro and BAGEL readers divide every coordinate by AU2ANG
+    whatever the block's unit tag, the Molcas and PySCF readers take it as
+    bohr."""
+    fl = _flavour(flavour)
     it = iter(lines)
     for line in it:
-        if "[Atoms]" in line:
+        if fl["atoms_start"](line):
             break
     else:
         raise ValueError("molden: no [Atoms] section")
+    scale = 1.0 / AU2ANG if fl["to_bohr"] else 1.0
     mol = Molecule()
     for line in it:
-        if "[GTO]" in line:
+        if fl["atoms_end"](line):
             break
         cols = line.split()
-        mol.atoms.append(Atom(int(cols[2]), float(cols[3]) / AU2ANG,
-                              float(cols[4]) / AU2ANG, float(cols[5]) / AU2ANG))
+        if fl["to_bohr"]:
+            mol.atoms.append(Atom(int(cols[2]), float(cols[3]) / AU2ANG,
+                                  float(cols[4]) / AU2ANG, float(cols[5]) / AU2ANG))
+        else:
+            mol.atoms.append(Atom(int(cols[2]), float(cols[3]) * scale,
+                                  float(cols[4]) * scale, float(cols[5]) * scale))
     else:
         raise ValueError("molden: no [GTO] section")
     return mol
 
 
-def _read_contractions(lines):
+def _read_contractions(lines, flavour="molpro"):
     """[GTO] block -> list of primitives (src/molden_reader_nikola.py:133-255).
 
     One group per primitive shared by its cartesian components; the
-    contraction index of each component is its MO-table row.  A single blank
-    line starts a new atom, two blank lines end the section.
+    contraction index of each component is its MO-table row.  A blank line
+    followed by an atom line starts a new atom.  What else may follow a blank
+    line: another blank line (Molpro, BAGEL: skipped), the [MO] tag (Molcas:
+    the block ends) or a [6d] tag (PySCF: the block ends).
     """
+    fl = _flavour(flavour)
     i = 0
     while i < len(lines) and "[GTO]" not in lines[i]:
         i += 1
@@ -153,7 +197,11 @@ def _read_contractions(lines):
         if not line or line.isspace():
             nxt = lines[i] if i < len(lines) else ""
             i += 1
-            if not nxt or nxt.isspace():
+            if fl["gto_end"] == "[MO]" and "[MO]" in nxt:
+                break
+            if fl["gto_end"] == "[6d]" and "[6d]" in nxt:
+                break
+            if fl["gto_end"] != "[MO]" and (not nxt or nxt.isspace()):
                 continue
             atm = [int(s) for s in nxt.split() if s.isdigit()][0]
             line = lines[i]
@@ -161,8 +209,10 @@ def _read_contractions(lines):
         spec = line.split()
         kind = spec[0].lower()
         nprim = int(spec[1])
-        if kind not in _ANG:
-            raise ValueError(f"molden: unsupported shell '{spec[0]}'")
+        if kind not in _ANG or kind not in fl["shells"]:
+            # the reference copies skip the components of a letter they do
+            # not expand, which misaligns the MO rows: refuse instead
+            raise ValueError(f"molden ({flavour}): unsupported shell '{spec[0]}'")
         ang = _ANG[kind]
         for _ in range(nprim):
             pl = lines[i].replace("D", "E")
@@ -180,31 +230,47 @@ def _read_contractions(lines):
 _FLOAT_RE = re.compile(r"\d+\.\d+")
 
 
-def _read_mo(lines, mo_cutoff):
-    """[MO] block (src/molden_reader_nikola.py:258-344).
+def _read_mo(lines, mo_cutoff, flavour="molpro"):
+    """[MO] block (src/molden_reader_nikola.py:258-344 and the copies).
 
-    Reads up to ``mo_cutoff`` orbitals, keeps the occupied subset and orders
-    it by the key frac(sym)*1000 + int(sym) with numpy's argsort.
+    Molpro: reads up to ``mo_cutoff`` orbitals, keeps the occupied subset and
+    orders it by the key frac(sym)*1000 + int(sym) (numpy argsort); the
+    permutation of the occupied subset is applied to the leading rows of the
+    MO table, energies and occupations, as the reference does.
+    PySCF (:296-344): the same keys and permutation, applied to the energies
+    and occupations only; every MO is kept in file order.
+    Molcas (:274-323): the key of an orbital is the first character of its
+    symmetry label; every MO is kept in file order, energies and occupations
+    are permuted, and the keys are returned as the labels.
+    BAGEL (:279-324): an orbital starts at its Ene line; nothing is
+    reordered and the labels are 0..len(last MO)-1.
     """
+    mode = _flavour(flavour)["mo"]
     i = 0
     while i < len(lines) and "[MO]" not in lines[i]:
         i += 1
     if i == len(lines):
         raise ValueError("molden: no [MO] section")
     i += 1
+    start = "Ene" if mode == "bagel" else "Sym"
     syms, energy, occ, table = [], [], [], []
     mo_counter = 0
     this = []
     while i < len(lines) and mo_counter <= mo_cutoff and "[FREQ]" not in lines[i]:
         line = lines[i]
-        if "Sym" in line:
+        if start in line:
             mo_counter += 1
             if mo_counter <= mo_cutoff:
-                s = _FLOAT_RE.findall(line)
-                syms.append(s[0] if s else 1.1)
+                if mode == "molcas":
+                    syms.append(line.split()[1][0])
+                else:
+                    s = _FLOAT_RE.findall(line)
+                    syms.append(s[0] if s else 1.1)
             if this:
                 table.append(this)
             this = []
+            if mode == "bagel":
+                energy.append(float(_FLOAT_RE.findall(line)[0]))
         elif "Ene" in line:
             energy.append(float(_FLOAT_RE.findall(line)[0]))
         elif "Spin" in line:
@@ -216,13 +282,21 @@ def _read_mo(lines, mo_cutoff):
         i += 1
     if this:
         table.append(this)
+    mo = np.array(table)
+    if mode == "bagel":
+        return mo.T, np.array(occ), np.array(energy), np.arange(len(this))
     syms_array = np.array([float(s) for s in syms])
+    if mode == "molcas":
+        keys = (syms_array - syms_array.astype(int)) * 1000 + syms_array.astype(int)
+        idx = np.argsort(keys)
+        return mo.T, np.array(occ)[idx], np.array(energy)[idx], keys
     occ_array = np.array([float(o) for o in occ])
     finalind = np.where(occ_array > 0.0)
     keys = syms_array[finalind]
     keys = (keys - keys.astype(int)) * 1000 + keys.astype(int)
     idx = np.argsort(keys)
-    mo = np.array(table)[idx, :]
+    if mode == "molpro":
+        mo = mo[idx, :]
     energy_array = np.array(energy)[idx]
     occ_array = np.array(occ)[idx]
     return mo.T, occ_array, energy_array, syms_array
@@ -257,26 +331,43 @@ def _reverse_contraction(gtos):
     return contracted
 
 
-def read_orbitals(file, N=10000, decontract=False):
-    """Primitive GTOs + MOs from a Molpro-style molden file.
+def read_orbitals(file, N=10000, decontract=False, flavour="molpro"):
+    """Primitive GTOs + MOs from a molden file.
 
-    Mirrors ``read_orbitals`` (src/molden_reader_nikola.py:496-508): returns
-    ``(ArrayGTO, Molecule)`` when given a file name, ``ArrayGTO`` when given an
-    open file.
+    Mirrors ``read_orbitals`` (src/molden_reader_nikola.py:496-508 and the
+    same function of the Molcas, PySCF and BAGEL copies, selected by
+    ``flavour``): returns ``(ArrayGTO, Molecule)`` when given a file name,
+    ``ArrayGTO`` when given an open file.
     """
     if isinstance(file, str):
         with open(file, "r") as f:
             text = f.read()
-        gtos = _read_mo_gtos(text.splitlines(keepends=True), N, decontract)
-        return gtos, _read_atoms(text.splitlines(keepends=True))
+        lines = text.splitlines(keepends=True)
+        return _read_mo_gtos(lines, N, decontract, flavour), _read_atoms(lines, flavour)
     text = file.read()
-    return _read_mo_gtos(text.splitlines(keepends=True), N, decontract)
+    return _read_mo_gtos(text.splitlines(keepends=True), N, decontract, flavour)
 
 
-def _read_mo_gtos(lines, N, decontract):
-    mol = _read_atoms(lines)
-    gtos = _read_contractions(lines)
-    mo_table, occ, energy, syms = _read_mo(lines, N)
+def read_occupied_orbitals(file, decontract=False, flavour="molpro"):
+    """``read_orbitals`` without the orbitals of zero occupation
+    (read_occupied_orbitals, src/molden_reader_nikola_molcas.py:504-518)."""
+    if isinstance(file, str):
+        with open(file, "r") as f:
+            arr = _read_mo_gtos(f.read().splitlines(keepends=True), 10000, decontract, flavour)
+    else:
+        arr = _read_mo_gtos(file.read().splitlines(keepends=True), 10000, decontract, flavour)
+    drop = np.where(arr.occ == 0)[0]
+    arr.occ = np.delete(arr.occ, drop)
+    arr.energy = np.delete(arr.energy, drop)
+    arr.mo = np.delete(arr.mo, drop, 1)
+    arr.syms = np.delete(arr.syms, drop)
+    return arr
+
+
+def _read_mo_gtos(lines, N, decontract, flavour="molpro"):
+    mol = _read_atoms(lines, flavour)
+    gtos = _read_contractions(lines, flavour)
+    mo_table, occ, energy, syms = _read_mo(lines, N, flavour)
     for g in gtos:
         g.mo = mo_table[g.contraction - 1, :]
     for g in gtos:
