@@ -9,7 +9,10 @@ conversion), pinned by physics on the reference's own data files:
   the CSFs of one occupation are orthonormal;
 * the Molcas CI vector converted to determinants keeps its norm, and its
   active 1-RDM is diagonal with the natural occupation numbers that the
-  same Molcas log prints (the log's orbitals are the natural orbitals).
+  same Molcas log prints (the log's orbitals are the natural orbitals);
+* the BAGEL CI reader (src/bagel_ci_reader.f90) on logs written to the
+  columns it reads, with and without the XMS-CASPT2 rotation (the
+  reference holds no BAGEL output file).
 """
 import gzip
 import itertools
@@ -226,3 +229,61 @@ def test_prepare_inputs_molcas(tmp_path):
     err = np.max(np.abs(inp.mmod @ overlap(arr) @ inp.mmod.T - np.eye(29)))
     assert err < 5e-6, err
     assert inp.civs.shape == (4900, 1)
+
+
+# -------------------------------------------------------------- BAGEL CI log
+def _bagel_log(path, nclosed, dets, vecs, rot=None):
+    """A log with the lines and columns src/bagel_ci_reader.f90 reads."""
+    nact = len(dets[0])
+    out = ["  Cartesian basis functions are used",
+           f"{'  Number of electrons':<29}{2 * nclosed + 4:>6}",
+           f"{'    nstate':<18}{len(vecs):>7}",
+           f"{'    nclosed':<18}{nclosed:>7}",
+           f"{'    nact':<18}{nact:>7}"]
+    for r, v in enumerate(vecs):
+        out.append(f"     * ci vector, state {r:>3}")
+        # each root prints its determinants in its own order
+        order = np.argsort(-np.abs(v), kind="stable")
+        out += [f"       {dets[k]}  {v[k]: .12f}" for k in order]
+    if rot is not None:
+        out.append("  * XMS-CASPT2 rotation matrix")
+        out += ["   " + "  ".join(f"{x: .12f}" for x in row) for row in rot]
+    path.write_text("\n".join(out) + "\n")
+
+
+def _cas44_dets():
+    dets = []
+    for a in itertools.combinations(range(4), 2):
+        for b in itertools.combinations(range(4), 2):
+            dets.append("".join("2" if (i in a and i in b) else "a" if i in a else
+                                "b" if i in b else "." for i in range(4)))
+    return dets
+
+
+@pytest.mark.parametrize("caspt2", [False, True])
+def test_bagel_log_route(tmp_path, caspt2):
+    rng = np.random.default_rng(5)
+    dets = _cas44_dets()
+    vecs, _ = np.linalg.qr(rng.standard_normal((36, 2)))
+    vecs = vecs.T
+    th = 0.3
+    rot = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    p = tmp_path / "bagel.out"
+    _bagel_log(p, 3, dets, vecs, rot if caspt2 else None)
+    from thistle_amd import bagel
+    sds, civs, ncl = bagel.get_civecs_in_sds(str(p), caspt2)
+    assert ncl == 3 and len(sds) == 36 and civs.shape == (2, 36)
+    # determinants in order of first appearance (root 0's print order)
+    order = np.argsort(-np.abs(vecs[0]), kind="stable")
+    assert sds == [dets[k] for k in order]
+    expect = vecs[:, order]
+    if caspt2:
+        expect = rot @ expect
+    np.testing.assert_allclose(civs, expect, atol=1e-12)
+    confs, cv = bagel.get_civs_and_confs(str(p), caspt2)
+    assert confs.shape == (36, 14) and np.all(confs[:, :6] == [1, 2, 1, 2, 1, 2])
+    # every row holds the 10 electrons, 5 alpha and 5 beta
+    assert np.all(np.sum(confs == 1, 1) == 5) and np.all(np.sum(confs == 2, 1) == 5)
+    for r in range(2):
+        g = oracle.onerdm(confs, cv, r + 1, r + 1)
+        assert np.trace(g) == pytest.approx(10.0, abs=1e-10)
