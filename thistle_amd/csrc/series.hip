@@ -547,8 +547,8 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
                     sxs[0] = sb;
                     cxs[0] = cbb;
                     if (off) sincos_cw(qv[0] * h, sxs[0], cxs[0]);
-                    double sd, cd;
-                    sincos_cw((double)WAVE * A.dq * h, sd, cd);
+                    // the phase step comes with the record
+                    const double sd = uniform(rc[LL + 4]), cd = uniform(rc[LL + 5]);
 #pragma unroll
                     for (int s = 1; s < SQ; ++s) {
                         const double ns = fma(sb, cd, cbb * sd), nc = fma(cbb, cd, -(sb * sd));
@@ -796,6 +796,28 @@ hipError_t set_contract_lds_limit(size_t bytes) {
     ALL_LL(C_)
 #undef C_
     return hipSuccess;
+}
+
+// sin, cos of 64 dq h of every record of one class, one thread per record
+// (a kernel of its own: inside k_contract the polynomial's registers would
+// lower the contraction's occupancy)
+__global__ void k_phase(double* __restrict__ rec, int64_t n, int R, int DO, double dq) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double* r = rec + k * R;
+    const double h = r[DO];
+    const bool br = (((uint64_t)__double_as_longlong(r[DO + 2]) >> 44) & 1u) != 0;
+    double s = 0.0, c = 0.0;
+    if (!br) sincos_cw((double)WAVE * dq * h, s, c);
+    r[DO + 3] = s;
+    r[DO + 4] = c;
+}
+
+hipError_t launch_phase(const SeriesArgs& a, int DO, int64_t n, hipStream_t s) {
+    if (n <= 0 || !a.qlin) return hipSuccess;
+    hipLaunchKernelGGL(k_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.hs, n,
+                       HS_ROW(DO), DO, a.dq);
+    return hipGetLastError();
 }
 
 hipError_t launch_series(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,

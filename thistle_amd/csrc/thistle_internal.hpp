@@ -37,10 +37,12 @@ struct EnumRow {
 
 // Per-quartet records written by the contraction and read by the series
 // kernel, in class-list order: the DO series coefficients (DO = LL + 1, j2
-// LL + 3), then h = |P_ket - P_bra|, 1/h (0 on the h = 0 branch) and the
-// packed pair indices and flags of HS_META.  The records of LL class c start
-// at (class start) * HS doubles.
-#define HS_ROW(DO) ((DO) + 3)
+// LL + 3), then h = |P_ket - P_bra|, 1/h (0 on the h = 0 branch), the packed
+// pair indices and flags of HS_META, and sin, cos of 64 dq h (the phase step
+// between a lane's q slices on a uniform grid, written by k_phase; unset
+// otherwise).  The records of
+// LL class c start at (class start) * HS doubles.
+#define HS_ROW(DO) ((DO) + 5)
 #define HS HS_ROW(17)
 // record word DO + 2: bra pair (bits 0-19), ket pair (20-39), multiplicity
 // (40-43), h = 0 branch (44)
@@ -196,6 +198,9 @@ size_t contract_lds_bytes(int LL, int Tmax);
 hipError_t set_contract_lds_limit(size_t bytes);
 hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t* idx,
                            int64_t nidx, int blocks, hipStream_t s);
+// k_phase: the phase-step words of the n records of one class (a.hs at the
+// class start, DO series coefficients per record); nothing off a uniform grid
+hipError_t launch_phase(const SeriesArgs& a, int DO, int64_t n, hipStream_t s);
 // k_series<LL>: S(q) of one LL class from its nidx quartet records (a.hs at
 // the class start); first = 1 overwrites the partial rows
 hipError_t launch_series(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,
