@@ -16,6 +16,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -79,6 +80,7 @@ struct ThistlePlan {
     int ngto = 0, ng = 0, maxl = 0, nq = 0, nmo = 0, norb = 0;
     int npair = 0, npmo = 0, waves = 0, waves_alloc = 0, nrows = 0;
     bool zfact = false;  // total Z through the MO factorisation of T (else one dsyrkx)
+    bool far = false;    // some q h may reach the dphrec regime (k_series_far)
     int qlin = 0, q0off = 0;  // q[k] = qa + k dq for k >= 1; q[0] apart
     double qa = 0.0, dq = 0.0;
     int64_t npp = 0, nquart = 0, nmat = 0, units = 0;
@@ -235,6 +237,26 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     pl->ng = b->ng;
     pl->maxl = maxl;
     pl->nq = b->nq;
+    {
+        // q h >= SERIES_XDPH needs the far-argument kernel only if some q
+        // times some pair-centre distance gets there; a pair centre lies
+        // between its two group centres, so the widest centre distance
+        // bounds every h
+        std::vector<std::array<double, 3>> c;
+        for (const auto& g : groups) {
+            std::array<double, 3> v{g.x, g.y, g.z};
+            if (std::find(c.begin(), c.end(), v) == c.end()) c.push_back(v);
+        }
+        double d2 = 0.0, qmax = 0.0;
+        for (size_t i = 0; i < c.size(); ++i)
+            for (size_t j = i + 1; j < c.size(); ++j) {
+                const double dx = c[i][0] - c[j][0], dy = c[i][1] - c[j][1],
+                             dz = c[i][2] - c[j][2];
+                d2 = std::max(d2, dx * dx + dy * dy + dz * dz);
+            }
+        for (int k = 0; k < b->nq; ++k) qmax = std::max(qmax, std::fabs(b->q[k]));
+        pl->far = qmax * std::sqrt(d2) * (1.0 + 1e-9) >= SERIES_XDPH;
+    }
     pl->cutoffz = b->cutoffz;
     pl->cutoffmd = b->cutoffmd;
     pl->cutoffcentre = b->cutoffcentre;
@@ -326,7 +348,9 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     // a combination (d fastest): bits 0-7 a*cj+b, 8-15 c*cr+d (offsets in the
     // bra / ket pair blocks), 16-25 / 26-35 / 36-45 the x / y / z exponent
     // tuple index, 46-61 a, b, c, d (4 bits each).  The ket offset runs
-    // fastest, so that a wave's Z reads fall on few cache lines.
+    // fastest, so that a wave's Z reads fall on few cache lines.  (A copy
+    // with the bra offset fastest for the quartets whose bra elements are
+    // Z's rows measured slower on XY6: 8.93 against 8.83 ms.)
     std::vector<int64_t> comboff((size_t)S * S * S * S, 0);
     std::vector<uint64_t> qtab;
     for (int si = 0; si < S; ++si)
@@ -344,26 +368,26 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
                         int v = (*sg[k])[0];
                         L4[k] = (v >> 8) + ((v >> 4) & 15) + (v & 15);
                     }
+                    auto entry = [&](int a, int bb, int cc, int d) {
+                        int e4[4] = {(*sg[0])[a], (*sg[1])[bb], (*sg[2])[cc], (*sg[3])[d]};
+                        uint64_t t3[3];
+                        for (int ax = 0; ax < 3; ++ax) {
+                            int sh = 8 - 4 * ax, t = 0;
+                            for (int k = 0; k < 4; ++k)
+                                t = t * (L4[k] + 1) + ((e4[k] >> sh) & 15);
+                            t3[ax] = (uint64_t)t;
+                        }
+                        return (uint64_t)(a * cnt4[1] + bb) |
+                               ((uint64_t)(cc * cnt4[3] + d) << 8) | (t3[0] << 16) |
+                               (t3[1] << 26) | (t3[2] << 36) | ((uint64_t)a << 46) |
+                               ((uint64_t)bb << 50) | ((uint64_t)cc << 54) |
+                               ((uint64_t)d << 58);
+                    };
                     for (int a = 0; a < cnt4[0]; ++a)
                         for (int bb = 0; bb < cnt4[1]; ++bb)
                             for (int cc = 0; cc < cnt4[2]; ++cc)
-                                for (int d = 0; d < cnt4[3]; ++d) {
-                                    int e4[4] = {(*sg[0])[a], (*sg[1])[bb], (*sg[2])[cc],
-                                                 (*sg[3])[d]};
-                                    uint64_t t3[3];
-                                    for (int ax = 0; ax < 3; ++ax) {
-                                        int sh = 8 - 4 * ax, t = 0;
-                                        for (int k = 0; k < 4; ++k)
-                                            t = t * (L4[k] + 1) + ((e4[k] >> sh) & 15);
-                                        t3[ax] = (uint64_t)t;
-                                    }
-                                    uint64_t w = (uint64_t)(a * cnt4[1] + bb) |
-                                                 ((uint64_t)(cc * cnt4[3] + d) << 8) |
-                                                 (t3[0] << 16) | (t3[1] << 26) | (t3[2] << 36) |
-                                                 ((uint64_t)a << 46) | ((uint64_t)bb << 50) |
-                                                 ((uint64_t)cc << 54) | ((uint64_t)d << 58);
-                                    qtab.push_back(w);
-                                }
+                                for (int d = 0; d < cnt4[3]; ++d)
+                                    qtab.push_back(entry(a, bb, cc, d));
                 }
     pl->nsig = S;
     int L1 = maxl + 1, T1 = 2 * maxl + 1;
@@ -570,6 +594,7 @@ int run_series(ThistlePlan* pl, bool timing) {
             a.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
             // j2 carries its series two orders further (bessels0rr(LLmax+2))
             HCHECK(launch_series(a, a.j2 ? LL + 2 : LL, n, w, 0, ss));
+            if (pl->far) HCHECK(launch_series_far(a, a.j2 ? LL + 2 : LL, n, w, ss));
         }
         HCHECK(launch_reduce(a.part, a.partc, pl->waves, a.nqc, pl->S.as<double>() + q0, ss));
     }

@@ -445,13 +445,17 @@ __device__ __forceinline__ double f_pzero_j2(const double (&hs)[LL + 1], double 
 }
 
 // qinv = 1/q (0 at q = 0), so that 1/x = invh * qinv needs no reciprocal.
-// sx, cx = sin, cos of x = q h, computed by the caller.
+// sx, cx = sin, cos of x = q h, computed by the caller.  Points with
+// x >= XDPH are left to k_series_far (0 here): the dphrec call would cost
+// this kernel about 50 registers per lane for a regime most workloads never
+// reach.
 template <int LL, bool MIXED>
 __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
-                                           const double* hsg, double q, double qinv, double h,
-                                           double invh, double sx, double cx) {
+                                           double q, double qinv, double h, double invh,
+                                           double sx, double cx) {
     const double x = q * h;
-    if (MIXED && x > (double)LL + 6.0 && x < XDPH) {
+    if (x >= XDPH) return 0.0;
+    if (MIXED && x > (double)LL + 6.0) {
         float hf[LL + 1];
 #pragma unroll
         for (int b = 0; b <= LL; ++b) hf[b] = (float)hs[b];
@@ -460,7 +464,6 @@ __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const dou
     const double inv = invh * qinv;
     // q = 0 (x = 0): the j0 limit 1 instead of 0/0
     const double j0 = x > 0.0 ? sx * inv : 1.0;
-    if (x >= XDPH) return f_dphrec<LL>(hsg, q, h, x, sx, cx);
     if (LL == 0 || x >= SeriesRegime<LL>::xup) return f_up64<LL>(hs, cb, q, invh, j0, cx);
     return f_miller<LL>(hs, q, x, j0);
 }
@@ -470,15 +473,15 @@ __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const dou
 // reference's 0/0), or the j2 h = 0 polynomial.
 template <int LL, bool MIXED, bool J2>
 __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
-                                         const double* hsg, double q, double qinv, double h,
-                                         double invh, int br, double sx, double cx) {
+                                         double q, double qinv, double h, double invh, int br,
+                                         double sx, double cx) {
     if constexpr (J2) {
         if (br) return f_pzero_j2<LL>(hs, q);
-        const double F = f_series<LL, MIXED>(hs, cb, hsg, q, qinv, h, invh, sx, cx);
+        const double F = f_series<LL, MIXED>(hs, cb, q, qinv, h, invh, sx, cx);
         return q > 0.0 ? F / (q * q) : (LL >= 1 ? hs[LL >= 1 ? 1 : 0] * (1.0 / 3.0) : 0.0);
     } else {
         if (br) return f_pzero<LL>(hs, q);
-        return f_series<LL, MIXED>(hs, cb, hsg, q, qinv, h, invh, sx, cx);
+        return f_series<LL, MIXED>(hs, cb, q, qinv, h, invh, sx, cx);
     }
 }
 
@@ -562,10 +565,9 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
                     for (int s = 0; s < SQ; ++s) sincos_cw(qv[s] * h, sxs[s], cxs[s]);
                 }
             }
-            const double* hsg = A.hs + (kb + j) * R;
 #pragma unroll
             for (int s = 0; s < SQ; ++s) {
-                double v = mult * f_eval<LL, MIXED, J2>(hs, cb, hsg, qv[s], qiv[s], h, invh, br,
+                double v = mult * f_eval<LL, MIXED, J2>(hs, cb, qv[s], qiv[s], h, invh, br,
                                                         sxs[s], cxs[s]);
                 v = v * ebv[s];
                 v = v * ekv[s];
@@ -588,6 +590,64 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
             A.part[o] = ps;
             A.partc[o] = pc;
         }
+    }
+}
+
+// The x = q h >= XDPH points of one class (dphrec), with k_series's launch
+// shape and class partition; each wave adds its sums to the partial rows the
+// series kernel wrote.  A wave skips every quartet whose h keeps its q points
+// below XDPH, so the kernel is a read of the records where the regime is
+// rare.
+template <int LL, bool J2>
+__global__ void __launch_bounds__(WAVE) k_series_far(SeriesArgs A, int64_t nidx) {
+    constexpr int R = HS_ROW(LL + 1);
+    const int lane = threadIdx.x;
+    int iqs[SQ];
+    double qv[SQ], S[SQ], Sc[SQ];
+    double qmx = 0.0;
+#pragma unroll
+    for (int s = 0; s < SQ; ++s) {
+        const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
+        iqs[s] = A.q0 + (iql < A.nqc ? iql : 0);
+        qv[s] = A.q[iqs[s]];
+        qmx = fmax(qmx, qv[s]);
+        S[s] = Sc[s] = 0.0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) qmx = fmax(qmx, __shfl_xor(qmx, off, WAVE));
+    const int64_t k0 = (int64_t)blockIdx.x * nidx / gridDim.x,
+                  k1 = ((int64_t)blockIdx.x + 1) * nidx / gridDim.x;
+    for (int64_t kq = k0; kq < k1; ++kq) {
+        const double* rc = A.hs + kq * R;
+        const double h = rc[LL + 1];
+        const uint64_t meta = (uint64_t)__double_as_longlong(rc[LL + 3]);
+        if (((meta >> 44) & 1u) || qmx * h < XDPH) continue;
+        const int pb = (int)(meta & 0xFFFFFu), pk = (int)((meta >> 20) & 0xFFFFFu);
+        const double mult = (double)((meta >> 40) & 15u);
+#pragma unroll
+        for (int s = 0; s < SQ; ++s) {
+            const double x = qv[s] * h;
+            if (x < XDPH) continue;
+            double sx, cx;
+            sincos_cw(x, sx, cx);
+            double F = f_dphrec<LL>(rc, qv[s], h, x, sx, cx);
+            if constexpr (J2) F = F / (qv[s] * qv[s]);
+            double v = mult * F;
+            v = v * A.e12[(int64_t)pb * A.nq + iqs[s]];
+            v = v * A.e12[(int64_t)pk * A.nq + iqs[s]];
+            kahan(S[s], Sc[s], v);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SQ; ++s) {
+        const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
+        if (iql >= A.nqc) continue;
+        const int64_t o = (int64_t)blockIdx.x * A.nqc + iql;
+        double ps = A.part[o], pc = A.partc[o];
+        kahan(ps, pc, S[s]);
+        kahan(ps, pc, -Sc[s]);
+        A.part[o] = ps;
+        A.partc[o] = pc;
     }
 }
 
@@ -817,6 +877,26 @@ hipError_t launch_phase(const SeriesArgs& a, int DO, int64_t n, hipStream_t s) {
     if (n <= 0 || !a.qlin) return hipSuccess;
     hipLaunchKernelGGL(k_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.hs, n,
                        HS_ROW(DO), DO, a.dq);
+    return hipGetLastError();
+}
+
+hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int waves,
+                             hipStream_t s) {
+    if (nidx <= 0) return hipSuccess;
+    dim3 grid(waves, (a.nqc + SQ * WAVE - 1) / (SQ * WAVE));
+    switch (LL) {
+#define CASE(k)                                                                              \
+    case k:                                                                                  \
+        if (a.j2)                                                                            \
+            hipLaunchKernelGGL((k_series_far<k, true>), grid, dim3(WAVE), 0, s, a, nidx);      \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_series_far<k, false>), grid, dim3(WAVE), 0, s, a, nidx);     \
+        break;
+        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

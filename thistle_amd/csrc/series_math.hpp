@@ -10,8 +10,9 @@
 
 #define WAVE 64
 
-#define XDPH 74.0   // q h from which dphrec's start-100 recurrence is followed
-                    // step by step (it departs from the exact j_beta near 76)
+#define XDPH SERIES_XDPH  // q h from which dphrec's start-100 recurrence is
+                          // followed step by step (it departs from the exact
+                          // j_beta near 76)
 
 namespace {
 

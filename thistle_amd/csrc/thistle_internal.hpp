@@ -198,6 +198,12 @@ size_t contract_lds_bytes(int LL, int Tmax);
 hipError_t set_contract_lds_limit(size_t bytes);
 hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t* idx,
                            int64_t nidx, int blocks, hipStream_t s);
+// k_series_far<LL>: the q h >= XDPH points of the class (same launch shape
+// and wave partition as launch_series, added to its partial rows)
+hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int waves,
+                             hipStream_t s);
+// q h from which the series follows dphrec's start-100 recurrence
+#define SERIES_XDPH 74.0
 // k_phase: the phase-step words of the n records of one class (a.hs at the
 // class start, DO series coefficients per record); nothing off a uniform grid
 hipError_t launch_phase(const SeriesArgs& a, int DO, int64_t n, hipStream_t s);
