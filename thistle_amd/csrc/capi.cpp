@@ -445,7 +445,8 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if (waves <= 0) {
         hipDeviceProp_t prop;
         HCHECK(hipGetDeviceProperties(&prop, o->device));
-        waves = prop.multiProcessorCount * 4;
+        // series waves per CU: 12 (XY6 quartet stage 8.23 ms; 4: 8.78, 8: 8.29)
+        waves = prop.multiProcessorCount * 12;
     }
     pl->waves = (int)std::max<int64_t>(1, std::min<int64_t>(waves, Q));
     pl->waves_alloc = pl->waves;
