@@ -1016,7 +1016,7 @@ int batch_timed(ThistleBatch* b, float& ms) {
 
 // autotune on eager executions, then capture the chosen configuration
 int batch_prepare(ThistleBatch* b) {
-    const int cand[3] = {2, 4, 8};
+    const int cand[3] = {4, 8, 12};
     float best = 0.f;
     int pick = cand[0];
     for (int c : cand) {
@@ -1080,7 +1080,7 @@ int32_t thistle_batch_create(int32_t nframes, const ThistleBasis* bases, const i
     for (int f = 0; f < nframes; ++f) {
         ThistleOpts of = *o;
         of.stream = b->streams[f % ThistleBatch::NS];
-        of.waves = 8 * b->cus;
+        of.waves = 12 * b->cus;
         ThistlePlan* raw = nullptr;
         int rc = thistle_plan_create(1, &bases[f], nmat[f], mats[f], totals[f], 0, nullptr, &of,
                                      &raw);
