@@ -104,21 +104,40 @@ def build_workload(name):
     return w, inp, mat, tot
 
 
-def measured_traffic(workload, precision):
-    """HBM bytes per execution of the quartet stage, from the committed
-    FETCH_SIZE / WRITE_SIZE profile of this workload (copy-calibrated); None
-    when no such profile exists for it."""
+def _profile_text(workload, precision):
     if workload != "xy6" or precision != "fp64":
         return None
     path = os.path.join(ROOT, "profiles", "xy6_quartet_stage_traffic.md")
     try:
         with open(path) as f:
-            for line in f:
-                if line.startswith("quartet-stage HBM bytes per execution:"):
-                    return float(line.split(":")[1])
+            return f.read()
     except OSError:
         return None
+
+
+def measured_traffic(workload, precision):
+    """HBM bytes per execution of the quartet stage, from the committed
+    FETCH_SIZE / WRITE_SIZE profile of this workload (copy-calibrated); None
+    when no such profile exists for it."""
+    text = _profile_text(workload, precision)
+    if text is None:
+        return None
+    for line in text.splitlines():
+        if line.startswith("quartet-stage HBM bytes per execution:"):
+            return float(line.split(":")[1])
     return None
+
+
+def measured_executed_flops(workload, precision):
+    """FP64 lane flops the quartet stage executes per execution
+    (SQ_INSTS_VALU_FLOPS_FP64 x 64 in the committed profile); None when no
+    such profile exists for this workload."""
+    text = _profile_text(workload, precision)
+    if text is None:
+        return None
+    import re
+    m = re.search(r"i\.e\. ([0-9.e+]+) lane flops executed", text)
+    return float(m.group(1)) if m else None
 
 
 def group_L(inp):
@@ -336,6 +355,12 @@ def main():
                      "algorithmic_flops": W, "flops_hermite": W_herm, "flops_series": W_ser,
                      "kernel_ms": qk_ms},
     }
+    ex = measured_executed_flops(args.workload, chosen)
+    if ex is not None:
+        # the factorised kernels execute fewer flops than the metric's fixed
+        # count, so "frac" can pass 1; this is the utilisation of the pipe
+        line["roofline"]["executed_flops"] = ex
+        line["roofline"]["executed_frac"] = ex / (qk_ms / 1000.0) / 1e12 / peak
     if err is not None:
         line["mixed_vs_fp64_max_rel_err"] = err
         line["mixed_ms_per_step"] = out["mixed"][1]

@@ -33,3 +33,15 @@ def test_xy6_flops():
     assert W == herm + ser
     assert herm == pytest.approx(6.3e11, rel=0.02)
     assert ser == pytest.approx(4.7e10, rel=0.02)
+
+
+def test_profile_figures_are_read():
+    """The bench line's traffic and executed flops come from the committed
+    XY6 profile; other workloads report none."""
+    import bench
+    t = bench.measured_traffic("xy6", "fp64")
+    ex = bench.measured_executed_flops("xy6", "fp64")
+    assert t is not None and t > 2.798e9  # at least the Z triangle
+    assert ex is not None and 1e10 < ex < 6.811e11  # below the algorithmic count
+    assert bench.measured_executed_flops("h2", "fp64") is None
+    assert bench.measured_traffic("xy6", "mixed") is None
