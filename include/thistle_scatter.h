@@ -159,10 +159,19 @@ int64_t thistle_createtwordm(const int32_t *confs, int32_t nconf, int32_t lconfs
                              const double *civecs, int32_t nstates, int32_t state1,
                              int32_t state2, double cutoff, const ThistleOpts *opts,
                              int32_t *mat, double *total, int64_t cap);
-/* onerdm_creat / createonerdm (src/one_rdm.f90:27-222). */
+/* onerdm_creat / createonerdm (src/one_rdm.f90:27-222), host accumulation:
+ * onerdm is norb x norb column-major, norb = lconfs/2. */
 int32_t thistle_onerdm(const int32_t *confs, int32_t nconf, int32_t lconfs,
                        const double *civecs, int32_t nstates, int32_t state1,
                        int32_t state2, double *onerdm);
+/* The same 1-RDM built on the GPU (maxcoincidence + createonerdm,
+ * src/one_rdm.f90:27-222): each element sums its contributions in the
+ * reference's (c1, c2, slot) order, so the doubles equal thistle_onerdm's.
+ * Determinant lists not in the punch reader's canonical form go through the
+ * host accumulation.  The drop-in's elastic types use this entry's path. */
+int32_t thistle_createonerdm(const int32_t *confs, int32_t nconf, int32_t lconfs,
+                             const double *civecs, int32_t nstates, int32_t state1,
+                             int32_t state2, const ThistleOpts *opts, double *onerdm);
 
 /* ---- device stages, exposed for parity checks ------------------------- */
 

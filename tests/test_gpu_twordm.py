@@ -1,4 +1,5 @@
-"""createtwordm on the GPU (thistle_createtwordm) against the reference's own
+"""createtwordm and createonerdm on the GPU (thistle_createtwordm,
+thistle_createonerdm) against the reference's own
 rows and values (flang build of src/twordms.f90, the golden fixtures) and
 against the host accumulation, bit for bit: same rows in the same order and
 identical doubles.  Covers ndiff 0 / 2 / 4 pairs, core orbitals, transition
@@ -105,3 +106,47 @@ def test_noncanonical_goes_host():
     civs = rng.standard_normal(len(confs))
     mg, tg, mh, th = both(confs, civs)
     assert np.array_equal(mg, mh) and np.array_equal(tg, th)
+
+
+# ------------------------------------------------------------ createonerdm
+def both1(confs, civs, state1=1, state2=1):
+    return (sc.onerdm(confs, civs, state1, state2, device=True),
+            sc.onerdm(confs, civs, state1, state2, device=False))
+
+
+@pytest.mark.parametrize("name", ["h2", "sp3", "spd2", "f2", "xyx_cas44"])
+def test_onerdm_reference_goldens(name, golden):
+    g = golden(name)
+    gd, gh = both1(g["confs"], g["civs"])
+    assert np.array_equal(gd, gh), name
+    n = gd.shape[0]
+    np.testing.assert_allclose(gd, g["onerdm"][:n, :n], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("ncore,nact,na,nb,seed", [(0, 6, 3, 3, 1), (3, 6, 3, 2, 2),
+                                                   (5, 8, 4, 4, 3), (30, 4, 2, 2, 4)])
+def test_onerdm_cas_vs_host(ncore, nact, na, nb, seed):
+    """CAS spaces with core, a transition 1-RDM, and 68 slots (two mask words)."""
+    rng = np.random.default_rng(seed)
+    confs = cas_confs(ncore, nact, na, nb, rng)
+    civs = rng.standard_normal((len(confs), 2))
+    civs /= np.linalg.norm(civs, axis=0)
+    for s1, s2 in ((1, 1), (1, 2)):
+        gd, gh = both1(confs, civs, s1, s2)
+        assert np.array_equal(gd, gh), (ncore, nact, s1, s2)
+    gd, _ = both1(confs, civs)
+    assert np.trace(gd) == pytest.approx(2 * ncore + na + nb, abs=1e-12)
+
+
+def test_onerdm_duplicates_and_noncanonical():
+    rng = np.random.default_rng(7)
+    confs = cas_confs(1, 4, 2, 2, rng)
+    dup = np.vstack([confs, confs[:3]])  # ndiff 0 with c1 != c2
+    civs = rng.standard_normal(len(dup))
+    gd, gh = both1(dup, civs)
+    assert np.array_equal(gd, gh)
+    nc = confs.copy()
+    nc[0, nc[0] == 1] = 2  # not the punch reader's form: host route
+    civs = rng.standard_normal(len(nc))
+    gd, gh = both1(nc, civs)
+    assert np.array_equal(gd, gh)

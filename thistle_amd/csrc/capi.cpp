@@ -1410,7 +1410,136 @@ int64_t createtwordm_device(const int32_t* confs, int nconf, int lconfs, const d
 #undef RCHECK
 }
 
+// createonerdm on the GPU: the ndiff <= 2 pairs, each pair's contributions at
+// their positions in the reference's (c1, c2, slot) order, a stable sort by
+// element and one ordered sum per element, so that every element is the
+// reference's double.
+int createonerdm_device(const int32_t* confs, int nconf, int lconfs, const double* c1v_h,
+                        const double* c2v_h, int device, double* onerdm) {
+#define RCHECK(x)                                                               \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) {                                                 \
+            std::fprintf(stderr, "thistle: %s failed: %s (%s:%d)\n", #x,        \
+                         hipGetErrorString(e_), __FILE__, __LINE__);            \
+            return THISTLE_EDEVICE;                                             \
+        }                                                                       \
+    } while (0)
+#define RALLOC(b, n)                                        \
+    do {                                                    \
+        if (dalloc(b, n)) return THISTLE_ENOMEM;            \
+    } while (0)
+    RCHECK(hipSetDevice(device));
+    hipStream_t st;
+    RCHECK(hipStreamCreate(&st));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    const int N = lconfs / 2, nw = (lconfs + 63) / 64;
+    const int64_t n2 = (int64_t)N * N;
+    DevBuf dconfs, masks, civ, cnt, off, c1s, c2s, nds, ccnt, coff, keys, vals, keys2, vals2,
+        temp, start, g;
+    RALLOC(dconfs, sizeof(int32_t) * (size_t)nconf * lconfs);
+    RALLOC(masks, sizeof(uint64_t) * (size_t)nconf * nw);
+    RALLOC(civ, sizeof(double) * 2 * (size_t)nconf);
+    RCHECK(hipMemcpyAsync(dconfs.p, confs, dconfs.bytes, hipMemcpyHostToDevice, st));
+    RCHECK(hipMemcpyAsync(civ.p, c1v_h, sizeof(double) * nconf, hipMemcpyHostToDevice, st));
+    RCHECK(hipMemcpyAsync(civ.as<double>() + nconf, c2v_h, sizeof(double) * nconf,
+                          hipMemcpyHostToDevice, st));
+    RCHECK(rdm_pack(dconfs.as<int32_t>(), nconf, lconfs, nw, masks.as<uint64_t>(), st));
+    // 1. pairs with ndiff <= 2 in (c1, c2) order
+    RALLOC(cnt, sizeof(int64_t) * ((size_t)nconf + 1));
+    RALLOC(off, sizeof(int64_t) * ((size_t)nconf + 1));
+    RCHECK(launch_fill(reinterpret_cast<double*>(cnt.p), nconf + 1, 0.0, st));
+    RCHECK(rdm_pairs(masks.as<uint64_t>(), nconf, nw, 2, cnt.as<int64_t>(), nullptr, nullptr,
+                     nullptr, nullptr, st));
+    size_t tb = 0;
+    RCHECK(rdm_scan_i64(nullptr, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
+    RALLOC(temp, tb);
+    RCHECK(rdm_scan_i64(temp.p, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
+    int64_t np = 0;
+    RCHECK(hipMemcpyAsync(&np, off.as<int64_t>() + nconf, sizeof(int64_t), hipMemcpyDeviceToHost,
+                          st));
+    RCHECK(hipStreamSynchronize(st));
+    RALLOC(c1s, sizeof(int32_t) * (size_t)std::max<int64_t>(np, 1));
+    RALLOC(c2s, sizeof(int32_t) * (size_t)std::max<int64_t>(np, 1));
+    RALLOC(nds, (size_t)std::max<int64_t>(np, 1));
+    RCHECK(rdm_pairs(masks.as<uint64_t>(), nconf, nw, 2, nullptr, off.as<int64_t>(),
+                     c1s.as<int32_t>(), c2s.as<int32_t>(), nds.as<int8_t>(), st));
+    // 2.-3. contributions at their positions
+    RALLOC(ccnt, sizeof(int64_t) * (size_t)(np + 1));
+    RALLOC(coff, sizeof(int64_t) * (size_t)(np + 1));
+    RCHECK(launch_fill(reinterpret_cast<double*>(ccnt.p), np + 1, 0.0, st));
+    RCHECK(rdm1_ccount(masks.as<uint64_t>(), nw, c1s.as<int32_t>(), nds.as<int8_t>(), np,
+                       ccnt.as<int64_t>(), st));
+    tb = 0;
+    RCHECK(rdm_scan_i64(nullptr, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), np + 1, st));
+    if (tb > temp.bytes) RALLOC(temp, tb);
+    RCHECK(rdm_scan_i64(temp.p, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), np + 1, st));
+    int64_t nc = 0;
+    RCHECK(hipMemcpyAsync(&nc, coff.as<int64_t>() + np, sizeof(int64_t), hipMemcpyDeviceToHost,
+                          st));
+    RCHECK(hipStreamSynchronize(st));
+    const size_t ncs = (size_t)std::max<int64_t>(nc, 1);
+    RALLOC(keys, sizeof(uint32_t) * ncs);
+    RALLOC(vals, sizeof(double) * ncs);
+    RALLOC(keys2, sizeof(uint32_t) * ncs);
+    RALLOC(vals2, sizeof(double) * ncs);
+    RCHECK(rdm1_cwrite(masks.as<uint64_t>(), nw, c1s.as<int32_t>(), c2s.as<int32_t>(),
+                       nds.as<int8_t>(), np, civ.as<double>(), civ.as<double>() + nconf,
+                       coff.as<int64_t>(), N, keys.as<uint32_t>(), vals.as<double>(), st));
+    // 4. stable sort by element, 5. ordered sums
+    int keybits = 1;
+    while (((int64_t)1 << keybits) < n2) ++keybits;
+    tb = 0;
+    RCHECK(rdm_sort(nullptr, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
+                    vals2.as<double>(), nc, keybits, st));
+    if (tb > temp.bytes) RALLOC(temp, tb);
+    if (nc > 0)
+        RCHECK(rdm_sort(temp.p, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
+                        vals2.as<double>(), nc, keybits, st));
+    RALLOC(start, sizeof(int64_t) * (size_t)(n2 + 1));
+    RALLOC(g, sizeof(double) * (size_t)n2);
+    RCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, n2, start.as<int64_t>(),
+                    g.as<double>(), st));
+    RCHECK(hipMemcpyAsync(onerdm, g.p, sizeof(double) * (size_t)n2, hipMemcpyDeviceToHost, st));
+    RCHECK(hipStreamSynchronize(st));
+    return THISTLE_OK;
+#undef RALLOC
+#undef RCHECK
+}
+
+bool canonical_confs(const int32_t* confs, int nconf, int lconfs) {
+    for (int j = 0; j < lconfs; ++j)
+        for (int c = 0; c < nconf; ++c) {
+            const int v = confs[(size_t)c + (size_t)nconf * j];
+            if (v != 0 && v != ((j % 2 == 0) ? 1 : 2)) return false;
+        }
+    return true;
+}
+
+// createonerdm on the GPU for canonical determinant lists, on the host
+// otherwise; both give the reference's doubles
+int onerdm_any(const int32_t* confs, int nconf, int lconfs, const double* civecs, int nstates,
+               int state1, int state2, int device, double* onerdm) {
+    if (nconf <= 0 || lconfs <= 0 || (lconfs % 2) != 0 || lconfs > 256) return THISTLE_EINVAL;
+    if (state1 < 1 || state2 < 1 || state1 > nstates || state2 > nstates) return THISTLE_EINVAL;
+    if (!canonical_confs(confs, nconf, lconfs))
+        return thistle_onerdm_host(confs, nconf, lconfs, civecs, nstates, state1, state2, onerdm);
+    return createonerdm_device(confs, nconf, lconfs, civecs + (size_t)nconf * (state1 - 1),
+                               civecs + (size_t)nconf * (state2 - 1), device, onerdm);
+}
+
 }  // namespace
+
+int32_t thistle_createonerdm(const int32_t* confs, int32_t nconf, int32_t lconfs,
+                             const double* civecs, int32_t nstates, int32_t state1,
+                             int32_t state2, const ThistleOpts* opts, double* onerdm) {
+    if (!confs || !civecs || !onerdm) return THISTLE_EINVAL;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    return onerdm_any(confs, nconf, lconfs, civecs, nstates, state1, state2, o->device, onerdm);
+}
 
 int64_t thistle_createtwordm(const int32_t* confs, int32_t nconf, int32_t lconfs,
                              const double* civecs, int32_t nstates, int32_t state1,
@@ -1421,15 +1550,7 @@ int64_t thistle_createtwordm(const int32_t* confs, int32_t nconf, int32_t lconfs
     if (state1 < 1 || state2 < 1 || state1 > nstates || state2 > nstates) return -THISTLE_EINVAL;
     if (cap > 0 && (!mat || !total)) return -THISTLE_EINVAL;
     const ThistleOpts* o = opts ? opts : &kDefaultOpts;
-    bool canonical = true;
-    for (int j = 0; j < lconfs && canonical; ++j)
-        for (int c = 0; c < nconf; ++c) {
-            const int v = confs[(size_t)c + (size_t)nconf * j];
-            if (v != 0 && v != ((j % 2 == 0) ? 1 : 2)) {
-                canonical = false;
-                break;
-            }
-        }
+    const bool canonical = canonical_confs(confs, nconf, lconfs);
     const double* c1v = civecs + (size_t)nconf * (state1 - 1);
     const double* c2v = civecs + (size_t)nconf * (state2 - 1);
     if (canonical)
@@ -1472,8 +1593,8 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
         const int nq = basis->nq;
         std::vector<double> Iee(nq), Ine(nq), Inn(nq);
         std::vector<double> g((size_t)norb * norb);
-        int rc = thistle_onerdm_host(confs, nconf, lconfs, civecs, nstates, state1, state2,
-                                     g.data());
+        int rc = onerdm_any(confs, nconf, lconfs, civecs, nstates, state1, state2, o->device,
+                            g.data());
         if (rc) return rc;
         if (type == 5) {
             rc = thistle_total_scattering_calculation(1, natom, Zn, geom, state1, state2, state3,
@@ -1530,7 +1651,8 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
         return rc;
     }
     std::vector<double> g((size_t)norb * norb);
-    int rc = thistle_onerdm_host(confs, nconf, lconfs, civecs, nstates, state1, state2, g.data());
+    int rc = onerdm_any(confs, nconf, lconfs, civecs, nstates, state1, state2, o->device,
+                        g.data());
     if (rc) return rc;
     if (type == 2) return thistle_elastic_from_onerdm(basis, norb, g.data(), o, result_out);
     // type 8: elastic j2 (src/main_calculation.f90:780-816)

@@ -315,6 +315,66 @@ __global__ void k_rdm_cwrite(const uint64_t* __restrict__ masks, const int32_t* 
     });
 }
 
+// createonerdm (src/one_rdm.f90:109-222) on the same pair list (ndiff <= 2):
+// an ndiff-2 pair adds c1 c2 ep at (d1, d2), the orbitals of c1's and c2's
+// differing slots; an ndiff-0 pair adds c1v[c1] c2v[c1] at (o, o) for every
+// occupied slot, in slot order (the reference's loop reads c2v at c1).
+template <int NW>
+__global__ void k_rdm1_ccount(const uint64_t* __restrict__ masks, const int32_t* __restrict__ c1s,
+                              const int8_t* __restrict__ nds, int64_t np,
+                              int64_t* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int nd = nds[i];
+    int64_t n = 0;
+    if (nd == 2) {
+        n = 1;
+    } else if (nd == 0) {
+        const DMask<NW> m1 = load_mask<NW>(masks, c1s[i]);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) n += __popcll(m1.w[w]);
+    }
+    cnt[i] = n;
+}
+
+template <int NW>
+__global__ void k_rdm1_cwrite(const uint64_t* __restrict__ masks, const int32_t* __restrict__ c1s,
+                              const int32_t* __restrict__ c2s, const int8_t* __restrict__ nds,
+                              int64_t np, const double* __restrict__ c1v,
+                              const double* __restrict__ c2v, const int64_t* __restrict__ coff,
+                              int N, uint32_t* __restrict__ keys, double* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int c1 = c1s[i], c2 = c2s[i], nd = nds[i];
+    int64_t pos = coff[i];
+    const int64_t end = coff[i + 1];  // k_rdm1_ccount's count for this pair
+    const DMask<NW> m1 = load_mask<NW>(masks, c1);
+    if (nd == 2) {
+        const DMask<NW> m2 = load_mask<NW>(masks, c2);
+        const Exc x = excitation(m1, m2, 2);
+        if (pos < end) {
+            keys[pos] = (uint32_t)((x.d1[0] - 1) + N * (x.d2[0] - 1));
+            vals[pos] = c1v[c1] * c2v[c2] * (double)x.e;
+        }
+    } else if (nd == 0) {
+        const double v = c1v[c1] * c2v[c1] * 1.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            uint64_t b = m1.w[w];
+            while (b) {
+                const int j = 64 * w + __builtin_ctzll(b);
+                b &= b - 1;
+                const int o = orb_of(j) - 1;
+                if (pos < end) {
+                    keys[pos] = (uint32_t)(o + N * o);
+                    vals[pos] = v;
+                }
+                ++pos;
+            }
+        }
+    }
+}
+
 // start of each element's run in the sorted keys (lower bound)
 __global__ void k_rdm_bounds(const uint32_t* __restrict__ keys, int64_t n, int64_t nel,
                              int64_t* __restrict__ start) {
@@ -425,6 +485,38 @@ hipError_t rdm_cwrite(const uint64_t* masks, int nw, const int32_t* c1s, const i
     case k:                                                                                   \
         hipLaunchKernelGGL(k_rdm_cwrite<k>, dim3(nb64(np, 256)), dim3(256), 0, s, masks, c1s, c2s, \
                            nds, np, c1v, c2v, coff, N, keys, vals);                            \
+        break;
+        RDM_NW(P_)
+#undef P_
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+hipError_t rdm1_ccount(const uint64_t* masks, int nw, const int32_t* c1s, const int8_t* nds,
+                       int64_t np, int64_t* cnt, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    switch (nw) {
+#define P_(k)                                                                                  \
+    case k:                                                                                    \
+        hipLaunchKernelGGL(k_rdm1_ccount<k>, dim3(nb64(np, 256)), dim3(256), 0, s, masks, c1s,  \
+                           nds, np, cnt);                                                       \
+        break;
+        RDM_NW(P_)
+#undef P_
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t rdm1_cwrite(const uint64_t* masks, int nw, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t np, const double* c1v, const double* c2v,
+                       const int64_t* coff, int N, uint32_t* keys, double* vals, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    switch (nw) {
+#define P_(k)                                                                                  \
+    case k:                                                                                    \
+        hipLaunchKernelGGL(k_rdm1_cwrite<k>, dim3(nb64(np, 256)), dim3(256), 0, s, masks, c1s,  \
+                           c2s, nds, np, c1v, c2v, coff, N, keys, vals);                        \
         break;
         RDM_NW(P_)
 #undef P_

@@ -105,6 +105,13 @@ hipError_t rdm_ccount(const uint64_t* masks, int nw, const int32_t* c1s, const i
 hipError_t rdm_cwrite(const uint64_t* masks, int nw, const int32_t* c1s, const int32_t* c2s,
                       const int8_t* nds, int64_t np, const double* c1v, const double* c2v,
                       const int64_t* coff, int N, uint32_t* keys, double* vals, hipStream_t s);
+// createonerdm's contributions on the ndiff <= 2 pair list (counts, then
+// keys (d1-1) + N (d2-1) and values at the scanned positions)
+hipError_t rdm1_ccount(const uint64_t* masks, int nw, const int32_t* c1s, const int8_t* nds,
+                       int64_t np, int64_t* cnt, hipStream_t s);
+hipError_t rdm1_cwrite(const uint64_t* masks, int nw, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t np, const double* c1v, const double* c2v,
+                       const int64_t* coff, int N, uint32_t* keys, double* vals, hipStream_t s);
 hipError_t rdm_scan_i64(void* temp, size_t& bytes, const int64_t* in, int64_t* out, int64_t n,
                         hipStream_t s);
 hipError_t rdm_sort(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* kout,

@@ -25,7 +25,8 @@ EXPORTS = [
     "thistle_total_scattering_adjoint",
     "thistle_elastic_from_onerdm", "thistle_twordm_dense", "thistle_twordm_select",
     "thistle_createtwordm", "thistle_nuclei_electron",
-    "thistle_onerdm", "thistle_quartet_list", "thistle_quartet_list_screened",
+    "thistle_onerdm", "thistle_createonerdm", "thistle_quartet_list",
+    "thistle_quartet_list_screened",
     "thistle_unique_total", "thistle_pair_tables", "thistle_plan_create",
     "thistle_plan_execute", "thistle_plan_result", "thistle_plan_timings",
     "thistle_plan_destroy", "thistle_batch_create", "thistle_batch_execute",
@@ -93,6 +94,8 @@ def _declare(L):
     L.thistle_nuclei_electron.restype = i32
     L.thistle_onerdm.argtypes = [p, i32, i32, p, i32, i32, i32, p]
     L.thistle_onerdm.restype = i32
+    L.thistle_createonerdm.argtypes = [p, i32, i32, p, i32, i32, i32, p, p]
+    L.thistle_createonerdm.restype = i32
     L.thistle_quartet_list.argtypes = [p, p, i64, p]
     L.thistle_quartet_list.restype = i64
     L.thistle_quartet_list_screened.argtypes = [i32, p, i64, p, p, i32, p, p, i64, p]

@@ -216,7 +216,11 @@ def createtwordm(confs, civs, state1=1, state2=1, cutoff=1e-30, device=None):
     return np.ascontiguousarray(mat), tot
 
 
-def onerdm(confs, civs, state1=1, state2=1):
+def onerdm(confs, civs, state1=1, state2=1, device=None):
+    """onerdm_creat / createonerdm: the norb x norb 1-RDM (src/one_rdm.f90:27-222).
+
+    device=None builds it on the GPU when one is present (thistle_createonerdm)
+    and on the host otherwise; both give the reference's doubles."""
     lib = L.load()
     confs = np.require(np.asarray(confs), dtype=np.int32, requirements=["F"])
     civs = np.require(np.asarray(civs, dtype=np.float64), requirements=["F"])
@@ -224,8 +228,15 @@ def onerdm(confs, civs, state1=1, state2=1):
         civs = civs.reshape(-1, 1, order="F")
     norb = confs.shape[1] // 2
     g = np.zeros((norb, norb), order="F")
-    L.check(lib.thistle_onerdm(L._ptr(confs), confs.shape[0], confs.shape[1], L._ptr(civs),
-                               civs.shape[1], state1, state2, L._ptr(g)), "onerdm")
+    if device is None:
+        device = L.device_count() > 0
+    if device:
+        L.check(lib.thistle_createonerdm(L._ptr(confs), confs.shape[0], confs.shape[1],
+                                         L._ptr(civs), civs.shape[1], state1, state2, None,
+                                         L._ptr(g)), "createonerdm")
+    else:
+        L.check(lib.thistle_onerdm(L._ptr(confs), confs.shape[0], confs.shape[1], L._ptr(civs),
+                                   civs.shape[1], state1, state2, L._ptr(g)), "onerdm")
     return g
 
 
