@@ -97,6 +97,7 @@ struct ThistlePlan {
     DevBuf part, partc, S, cls;
     std::vector<int> gstart, gcount, gL;  // per primitive shell (host)
     int64_t cls_off[18] = {0};  // per-LL ranges in `cls` (positions, ref order)
+    int64_t cls_hoff[18] = {0}; // per-LL start of the class's records in `hs` (doubles)
     int cls_tmax[17] = {0};     // largest exponent-tuple count per LL class
     DevBuf hs;                  // h(beta) per quartet
     DevBuf quadtab, comboff;    // GTO-quadruple tables per shell-signature combination
@@ -340,6 +341,11 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
         if (need > 65536) HCHECK(set_contract_lds_limit(need));
         pl->cls_off[0] = 0;
         for (int k = 0; k < 17; ++k) pl->cls_off[k + 1] = pl->cls_off[k] + cnt[k];
+        // records of class LL: DO = LL + 1 series coefficients (LL + 3 for
+        // j2) plus the per-quartet words, HS_ROW(DO) doubles each
+        const int dj = (pl->type == 7 || pl->type == 8) ? 3 : 1;
+        for (int k = 0; k < 17; ++k)
+            pl->cls_hoff[k + 1] = pl->cls_hoff[k] + cnt[k] * HS_ROW(k + dj);
         std::vector<int64_t> fill(pl->cls_off, pl->cls_off + 17);
         for (const auto& R : rows)
             for (int u = 0; u < R.count; ++u) cls[fill[LLof(R, R.r0 + u)]++] = (int32_t)(R.out + u);
@@ -407,7 +413,8 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if ((rc = dalloc(pl->xa, sizeof(int32_t) * pl->npp))) return rc;
     if ((rc = dalloc(pl->xb, sizeof(int32_t) * pl->npp))) return rc;
     if ((rc = dalloc(pl->cls, sizeof(int32_t) * Q))) return rc;
-    if ((rc = dalloc(pl->hs, sizeof(double) * HS * (size_t)Q))) return rc;
+    if ((rc = dalloc(pl->hs, sizeof(double) * (size_t)std::max<int64_t>(pl->cls_hoff[17], 1))))
+        return rc;
     if ((rc = dalloc(pl->quadtab, sizeof(uint64_t) * qtab.size()))) return rc;
     if ((rc = dalloc(pl->comboff, sizeof(int64_t) * comboff.size()))) return rc;
     HCHECK(hipMemcpyAsync(pl->quadtab.p, qtab.data(), sizeof(uint64_t) * qtab.size(),
@@ -577,7 +584,7 @@ int run_series(ThistlePlan* pl, bool timing) {
         // one workgroup per quartet group up to 2^20 groups (grid-stride
         // beyond): XY6's quartet stage 8.0 ms against 8.15 with 16384
         int blocks = (int)std::min<int64_t>(1 << 20, n);
-        a.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
+        a.hs = pl->hs.as<double>() + pl->cls_hoff[LL];
         HCHECK(launch_contract(a, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
                                n, blocks, pl->stream));
         HCHECK(launch_phase(a, a.j2 ? LL + 3 : LL + 1, n, pl->stream));
@@ -594,7 +601,7 @@ int run_series(ThistlePlan* pl, bool timing) {
             if (n <= 0) continue;
             if (ov && q0 == 0) HCHECK(hipStreamWaitEvent(ss, pl->cev[LL], 0));
             int w = (int)std::min<int64_t>(pl->waves, n);
-            a.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
+            a.hs = pl->hs.as<double>() + pl->cls_hoff[LL];
             // j2 carries its series two orders further (bessels0rr(LLmax+2))
             HCHECK(launch_series(a, a.j2 ? LL + 2 : LL, n, w, 0, ss));
             if (pl->far) HCHECK(launch_series_far(a, a.j2 ? LL + 2 : LL, n, w, ss));
@@ -890,7 +897,7 @@ int run_adjoint(ThistlePlan* pl, const double* w_host, double* gg_host, double* 
         bool glob;
         int blocks;
         shape(LL, n, glob, blocks);
-        j.s.hs = pl->hs.as<double>() + pl->cls_off[LL] * HS;
+        j.s.hs = pl->hs.as<double>() + pl->cls_hoff[LL];
         HCHECK(launch_contract_adj(j, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
                                    n, blocks, glob, lds_cap, pl->stream));
     }

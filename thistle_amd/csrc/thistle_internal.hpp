@@ -40,10 +40,9 @@ struct EnumRow {
 // LL + 3), then h = |P_ket - P_bra|, 1/h (0 on the h = 0 branch), the packed
 // pair indices and flags of HS_META, and sin, cos of 64 dq h (the phase step
 // between a lane's q slices on a uniform grid, written by k_phase; unset
-// otherwise).  The records of
-// LL class c start at (class start) * HS doubles.
+// otherwise).  The classes' records follow one another (largest LL last);
+// each class starts at the plan's offset for it.
 #define HS_ROW(DO) ((DO) + 5)
-#define HS HS_ROW(17)
 // record word DO + 2: bra pair (bits 0-19), ket pair (20-39), multiplicity
 // (40-43), h = 0 branch (44)
 #define HS_META(pb, pk, mult, br)                                                \
