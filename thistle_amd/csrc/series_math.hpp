@@ -137,12 +137,14 @@ __constant__ ACoef kAcoef = make_acoef();
 }  // namespace
 
 // G lanes per quartet: small classes (a few GTO quadruples and exponent
-// tuples per quartet) share one wave between 64/G quartets; from LL = 10 on
-// a quartet has thousands of quadruples and takes a 256-thread workgroup.
+// tuples per quartet) share one wave between 64/G quartets (two quartets per
+// wave up to LL = 6: XY6's quartet stage 7.75 against 7.97 ms with one; LL =
+// 7 is slower with two); from LL = 10 on a quartet has thousands of
+// quadruples and takes a 256-thread workgroup.
 template <int LL>
 struct ContractShape {
     static constexpr int CT = LL < 10 ? 64 : 256;
-    static constexpr int G = LL <= 1 ? 4 : LL == 2 ? 8 : LL == 3 ? 16 : LL == 4 ? 32 : CT;
+    static constexpr int G = LL <= 1 ? 4 : LL == 2 ? 8 : LL == 3 ? 16 : LL <= 6 ? 32 : CT;
     static constexpr int NG = CT / G;
     // row stride of the px/py/pz tables: odd, so that rows gathered by the
     // lanes of a wave spread over all LDS banks (an even stride of 8 doubles
