@@ -840,8 +840,6 @@ hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t*
     return hipGetLastError();
 }
 
-int contract_threads(int LL) { return LL < 10 ? 64 : 256; }
-
 hipError_t set_contract_lds_limit(size_t bytes) {
 #define C_(k)                                                                            \
     {                                                                                    \
