@@ -139,11 +139,12 @@ __constant__ ACoef kAcoef = make_acoef();
 // G lanes per quartet: small classes (a few GTO quadruples and exponent
 // tuples per quartet) share one wave between 64/G quartets (two quartets per
 // wave up to LL = 6: XY6's quartet stage 7.75 against 7.97 ms with one; LL =
-// 7 is slower with two); from LL = 10 on a quartet has thousands of
-// quadruples and takes a 256-thread workgroup.
+// 7 is slower with two); LL = 9 quartets take two waves and from LL = 10 on,
+// with thousands of quadruples, four (XY6: 7.50 ms; LL 9 on one wave 7.82,
+// on four 7.55; LL 8 on four 7.92; LL 10+ on two 8.11, on eight 8.10).
 template <int LL>
 struct ContractShape {
-    static constexpr int CT = LL < 10 ? 64 : 256;
+    static constexpr int CT = LL < 9 ? 64 : LL == 9 ? 128 : 256;
     static constexpr int G = LL <= 1 ? 4 : LL == 2 ? 8 : LL == 3 ? 16 : LL <= 6 ? 32 : CT;
     static constexpr int NG = CT / G;
     // row stride of the px/py/pz tables: odd, so that rows gathered by the
