@@ -142,6 +142,31 @@ def test_z_transforms_agree(case, monkeypatch):
 
 
 
+@pytest.mark.parametrize("ncore", [2, 10, 26])
+def test_z_fused_transform_agrees(ncore, monkeypatch):
+    """The factorised transform fused on the matrix cores (MO widths 8, 16
+    and 32) gives the S(q) of the two-kernel form and of the rank-k product."""
+    w = synth.make_small(seed=40 + ncore, natom=4, spec={"s": 2, "p": 2, "d": 1}, nact=4,
+                         nelec=4, ncore=ncore, nq=48, qmax=8.0)
+    with tempfile.TemporaryDirectory() as d:
+        mld, pun = synth.write_inputs(w, d)
+        inp = driver.prepare_inputs(mld, pun, w.ncore)
+    mat, tot = sc.createtwordm(inp.confs, inp.civs)
+    print(f"ncore {ncore}: ngto={inp.ngto} norb={ncore + 4}", flush=True)
+    out = {}
+    for key, env in (("fused", {}), ("two", {"THISTLE_ZMFMA": "0"}),
+                     ("syrk", {"THISTLE_ZFACT": "0"})):
+        monkeypatch.delenv("THISTLE_ZMFMA", raising=False)
+        monkeypatch.delenv("THISTLE_ZFACT", raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        out[key] = sc.total_from_twordm(*_args(inp), w.q, inp.group, mat, tot)
+    for key in ("two", "syrk"):
+        rel = np.max(np.abs(out["fused"] - out[key]) / np.abs(out[key]))
+        print(f"ncore {ncore}: fused vs {key} max rel diff {rel:.3e}", flush=True)
+        assert rel < RTOL_FP64
+
+
 def test_stream_overlap_bit_identical(case, monkeypatch):
     """Contraction and series on two streams (the default) give the same
     bits as every stage on one stream: the series launches keep one order."""

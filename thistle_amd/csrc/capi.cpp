@@ -80,6 +80,7 @@ struct ThistlePlan {
     int ngto = 0, ng = 0, maxl = 0, nq = 0, nmo = 0, norb = 0;
     int npair = 0, npmo = 0, waves = 0, waves_alloc = 0, nrows = 0;
     bool zfact = false;  // total Z through the MO factorisation of T (else one dsyrkx)
+    int zgrid = 0;       // > 0: that factorisation fused on the matrix cores, this many blocks
     bool far = false;    // some q h may reach the dphrec regime (k_series_far)
     int qlin = 0, q0off = 0;  // q[k] = qa + k dq for k >= 1; q[0] apart
     double qa = 0.0, dq = 0.0;
@@ -514,7 +515,7 @@ int run_transform_total(ThistlePlan* pl) {
         HCHECK(launch_zfact(pl->M.as<double>(), pl->mmod.as<double>(), pl->nmo, norb, pl->ngto,
                             pl->npp, pl->jtab.as<int32_t>(), pl->aend.as<int32_t>(),
                             pl->Cp.as<double>(), pl->W3.as<double>(), pl->Z.as<double>(),
-                            pl->stream));
+                            pl->zgrid, pl->stream));
         return THISTLE_OK;
     }
     // M = Gs * T   (npmo x npp)
@@ -706,7 +707,14 @@ int plan_create_impl(int32_t type, const ThistleBasis* b, int64_t nmat, const in
         }
         if (pl->zfact) {
             const int ngt = b->ngto;
-            if ((rc = dalloc(pl->W3, sizeof(double) * (size_t)pl->norb * ngt * pl->npp)))
+            if (zfact_fused(pl->norb, ngt)) {
+                // one persistent block per CU
+                HCHECK(hipDeviceGetAttribute(&pl->zgrid, hipDeviceAttributeMultiprocessorCount,
+                                             o->device));
+                HCHECK(zfact_fused_prepare(pl->norb, ngt));
+            }
+            if (pl->zgrid <= 0 &&
+                (rc = dalloc(pl->W3, sizeof(double) * (size_t)pl->norb * ngt * pl->npp)))
                 return rc;
             if ((rc = dalloc(pl->Cp, sizeof(double) * (size_t)ngt * zfact_width(pl->norb))))
                 return rc;

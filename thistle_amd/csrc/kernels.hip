@@ -18,6 +18,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdlib>
 
 #define WAVE 64
 
@@ -533,17 +535,179 @@ __global__ void __launch_bounds__(ZF_T) k_zform(const double* __restrict__ W3,
     }
 }
 
+// The same two contractions fused on the matrix cores, W3 never leaving
+// the CU.  Persistent blocks of ZmShape::W waves, one per CU (the LDS holds C whole
+// plus one W chunk); each block walks tiles of ZM_TI primitive pairs i,
+// largest i first.  Per tile, Mo(p, :, i) of each wave's NO / W MO rows p
+// stays in registers as MFMA operands, and per chunk of ZM_BC GTOs b:
+//   stage 1  W(p, b, i)  = sum_s C_sb Mo(p, s, i)     (D[b][i] per p)
+//   stage 2  Z(i, j(a,b)) = sum_p C_pa W(p, b, i)     (D[a][i] per b)
+// both as v_mfma_f64_16x16x4 products (A[m][k] from lane (m = l % 16,
+// k = l / 16), B[k][n] from lane (k = l / 16, n = l % 16), D[m][n] in item
+// m / 4 of lane n + 16 (m % 4), as measured on gfx950).  Chunks whose first
+// pair column lies beyond the tile are skipped: the triangle i >= j bounds
+// both stages.
+#define ZM_TI 16  // primitive pairs i per tile
+#define ZM_BC 16  // GTOs b per chunk
+#ifndef ZM_WMAX
+#define ZM_WMAX 16  // waves per block at NO = 32
+#endif
+typedef double zdbl4 __attribute__((ext_vector_type(4)));
+
+template <int NO>
+struct ZmShape {
+    static constexpr int W = NO >= 32 ? ZM_WMAX : NO / 2;  // waves per block
+    static constexpr int CS = NO + 2;               // C row stride in LDS (doubles)
+    static constexpr int WB = NO * ZM_TI + 16;      // W chunk stride per b (doubles)
+    static constexpr int PW = NO / W;               // MO rows p per wave
+    static constexpr int KS = NO / 4;               // MFMA k steps
+};
+
+static inline size_t zm_lds_bytes(int NO, int ngto) {
+    const int ng16 = (ngto + 15) & ~15;
+    return sizeof(double) * ((size_t)ng16 * (NO + 2) + (size_t)ZM_BC * (NO * ZM_TI + 16)) +
+           sizeof(int32_t) * (size_t)ZM_BC * ng16;
+}
+
+template <int NO>
+__global__ void __launch_bounds__(ZmShape<NO>::W * 64) k_zmfma(const double* __restrict__ Mt,
+                                                     const double* __restrict__ Cp,
+                                                     const int32_t* __restrict__ jtab,
+                                                     const int32_t* __restrict__ aend_of,
+                                                     int norb, int ngto, int64_t npp, int ntile,
+                                                     double* __restrict__ Z) {
+    using S = ZmShape<NO>;
+    extern __shared__ double zsm[];
+    const int ng16 = (ngto + 15) & ~15;
+    double* Cs = zsm;                      // [ng16][CS]: C(p, a) at a * CS + p
+    double* Wc = zsm + (size_t)ng16 * S::CS;  // [ZM_BC][WB]: W(p, b, i) at b * WB + p * ZM_TI + i
+    // [ZM_BC][ng16]: the chunk's pair columns j(a, b), INT_MAX past aend[b]
+    int32_t* Jt = (int32_t*)(Wc + ZM_BC * S::WB);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    for (int t = tid; t < ng16 * S::CS; t += S::W * 64) {
+        const int a = t / S::CS, p = t - a * S::CS;
+        Cs[t] = (a < ngto && p < NO) ? Cp[(int64_t)a * NO + p] : 0.0;
+    }
+    __syncthreads();
+    for (int tile = ntile - 1 - (int)blockIdx.x; tile >= 0; tile -= (int)gridDim.x) {
+        const int64_t i0 = (int64_t)tile * ZM_TI, ilast = i0 + ZM_TI - 1;
+        const int64_t i = i0 + l16;
+        const bool live = i < npp;
+        // stage-1 B operands: Mo(p, s, i), k = s
+        double mo[S::PW][S::KS];
+#pragma unroll
+        for (int pr = 0; pr < S::PW; ++pr) {
+            const int p = wid * S::PW + pr;
+#pragma unroll
+            for (int ks = 0; ks < S::KS; ++ks) {
+                const int s = l4 + 4 * ks;
+                double v = 0.0;
+                if (live && p < norb && s < norb) {
+                    const int x = p < s ? pair_index(p, s, norb) : pair_index(s, p, norb);
+                    v = Mt[i + npp * x];
+                    if (s == p) v *= 2.0;
+                }
+                mo[pr][ks] = v;
+            }
+        }
+        for (int b0 = 0; b0 < ngto; b0 += ZM_BC) {
+            // first column of the chunk's first GTO: columns grow with b
+            if (jtab[(int64_t)b0 * ngto] > ilast) break;
+            for (int t = tid; t < ZM_BC * ng16; t += S::W * 64) {
+                const int bl = t / ng16, a = t - bl * ng16, b = b0 + bl;
+                Jt[t] = (b < ngto && a < aend_of[b]) ? jtab[(int64_t)b * ngto + a] : INT_MAX;
+            }
+            {
+                double cb[S::KS];
+#pragma unroll
+                for (int ks = 0; ks < S::KS; ++ks) cb[ks] = Cs[(b0 + l16) * S::CS + l4 + 4 * ks];
+#pragma unroll
+                for (int pr = 0; pr < S::PW; ++pr) {
+                    zdbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int ks = 0; ks < S::KS; ++ks)
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(cb[ks], mo[pr][ks], acc, 0, 0, 0);
+                    const int p = wid * S::PW + pr;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Wc[(4 * r + l4) * S::WB + p * ZM_TI + l16] = acc[r];
+                }
+            }
+            __syncthreads();
+            for (int bl = wid; bl < ZM_BC; bl += S::W) {
+                const int32_t* jb = Jt + bl * ng16;
+                if (jb[0] > ilast) continue;
+                double wv[S::KS];
+#pragma unroll
+                for (int ks = 0; ks < S::KS; ++ks)
+                    wv[ks] = Wc[bl * S::WB + (l4 + 4 * ks) * ZM_TI + l16];
+                // a tiles while the tile's first column reaches the triangle
+                for (int a0 = 0; a0 < ng16 && jb[a0] <= ilast; a0 += 16) {
+                    int jv[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) jv[r] = jb[a0 + 4 * r + l4];
+                    zdbl4 c0 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int ks = 0; ks < S::KS; ++ks)
+                        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                            Cs[(a0 + l16) * S::CS + l4 + 4 * ks], wv[ks], c0, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (live && i >= jv[r]) __builtin_nontemporal_store(c0[r], &Z[i + npp * jv[r]]);
+                }
+            }
+            __syncthreads();  // the chunk's W is consumed
+        }
+    }
+}
+
+// the fused transform applies while C and one W chunk fit the LDS
+bool zfact_fused(int norb, int ngto) {
+    if (const char* ev = std::getenv("THISTLE_ZMFMA"))
+        if (ev[0] == '0') return false;
+    const int NO = zfact_width(norb);
+    return NO >= 8 && NO <= 32 && zm_lds_bytes(NO, ngto) <= 160 * 1024;
+}
+
+hipError_t zfact_fused_prepare(int norb, int ngto) {
+    const int NO = zfact_width(norb);
+    const int lds = (int)zm_lds_bytes(NO, ngto);
+    switch (NO) {
+        case 8: return hipFuncSetAttribute((const void*)k_zmfma<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        case 16: return hipFuncSetAttribute((const void*)k_zmfma<16>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        case 32: return hipFuncSetAttribute((const void*)k_zmfma<32>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 int zfact_width(int norb) {
     return norb <= 8 ? 8 : norb <= 16 ? 16 : norb <= 32 ? 32 : norb <= 48 ? 48 : norb <= 64 ? 64 : 0;
 }
 
 hipError_t launch_zfact(const double* Mt, const double* C, int nmo, int norb, int ngto,
                         int64_t npp, const int32_t* jtab, const int32_t* aend_of, double* Cp,
-                        double* W3, double* Z, hipStream_t s) {
+                        double* W3, double* Z, int zgrid, hipStream_t s) {
     const int NO = zfact_width(norb);
     if (NO == 0 || npp <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cpad, dim3((ngto * NO + 255) / 256), dim3(256), 0, s, C, nmo, norb,
                        ngto, NO, Cp);
+    if (zgrid > 0) {
+        if (NO > 32 || zm_lds_bytes(NO, ngto) > 160 * 1024) return hipErrorInvalidValue;
+        const size_t lds = zm_lds_bytes(NO, ngto);
+        const int ntile = (int)((npp + ZM_TI - 1) / ZM_TI);
+        const unsigned grid = (unsigned)std::min(ntile, zgrid);
+#define ZMF(NOV)                                                                             \
+    case NOV:                                                                                \
+        hipLaunchKernelGGL(k_zmfma<NOV>, dim3(grid), dim3(ZmShape<NOV>::W * 64), lds, s, Mt, Cp, jtab,   \
+                           aend_of, norb, ngto, npp, ntile, Z);                              \
+        break;
+        switch (NO) {
+            ZMF(8) ZMF(16) ZMF(32)
+            default: return hipErrorInvalidValue;
+        }
+#undef ZMF
+        return hipGetLastError();
+    }
     const unsigned nt = (unsigned)((npp + ZF_T - 1) / ZF_T);
 #define ZF(NOV)                                                                              \
     case NOV:                                                                                \

@@ -231,11 +231,16 @@ hipError_t launch_zel(const double* mmod, int nmo, int norb,
                       const double* onerdm, int ngto, double* Z, hipStream_t s);
 // Lower-triangle Z = T^T Gs T through the MO factorisation of T (norb <= 64):
 // Mt = (Gs T)^T (npp x npmo), Cp scratch ngto x zfact_width(norb), W3 scratch
-// norb x ngto x npp, jtab[b][a] the primitive pair of GTOs (a, b) for a < aend[b]
+// norb x ngto x npp (unused when fused), jtab[b][a] the primitive pair of GTOs
+// (a, b) for a < aend[b].  zgrid > 0: both contractions in one matrix-core
+// kernel of zgrid persistent blocks (zfact_fused(norb, ngto) says where it
+// applies, THISTLE_ZMFMA=0 turns it off; zfact_fused_prepare once before).
 int zfact_width(int norb);
+bool zfact_fused(int norb, int ngto);
+hipError_t zfact_fused_prepare(int norb, int ngto);
 hipError_t launch_zfact(const double* Mt, const double* C, int nmo, int norb, int ngto,
                         int64_t npp, const int32_t* jtab, const int32_t* aend_of, double* Cp,
-                        double* W3, double* Z, hipStream_t s);
+                        double* W3, double* Z, int zgrid, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(const double* in, double* out, int64_t n, hipStream_t s);
 hipError_t launch_to_float(const double* in, float* out, int64_t n,
