@@ -611,13 +611,15 @@ __global__ void __launch_bounds__(ZmShape<NO>::W * 64) k_zmfma(const double* __r
                 mo[pr][ks] = v;
             }
         }
-        for (int b0 = 0; b0 < ngto; b0 += ZM_BC) {
-            // first column of the chunk's first GTO: columns grow with b
-            if (jtab[(int64_t)b0 * ngto] > ilast) break;
-            for (int t = tid; t < ZM_BC * ng16; t += S::W * 64) {
-                const int bl = t / ng16, a = t - bl * ng16, b = b0 + bl;
-                Jt[t] = (b < ngto && a < aend_of[b]) ? jtab[(int64_t)b * ngto + a] : INT_MAX;
-            }
+        // chunks that reach the triangle: the first column of a chunk's first
+        // GTO grows with b, so they are a prefix (one load per lane, a ballot)
+        int nchunk = 0;
+        for (int c0 = 0; c0 * ZM_BC < ngto; c0 += 64) {
+            const int c = c0 + lane;
+            const bool in = c * ZM_BC < ngto && jtab[(int64_t)c * ZM_BC * ngto] <= ilast;
+            nchunk += __popcll(__ballot(in));
+        }
+        for (int b0 = 0; b0 < nchunk * ZM_BC; b0 += ZM_BC) {
             {
                 double cb[S::KS];
 #pragma unroll
@@ -632,6 +634,11 @@ __global__ void __launch_bounds__(ZmShape<NO>::W * 64) k_zmfma(const double* __r
 #pragma unroll
                     for (int r = 0; r < 4; ++r) Wc[(4 * r + l4) * S::WB + p * ZM_TI + l16] = acc[r];
                 }
+            }
+            // the chunk's pair columns, loaded while the MFMAs run
+            for (int t = tid; t < ZM_BC * ng16; t += S::W * 64) {
+                const int bl = t / ng16, a = t - bl * ng16, b = b0 + bl;
+                Jt[t] = (b < ngto && a < aend_of[b]) ? jtab[(int64_t)b * ngto + a] : INT_MAX;
             }
             __syncthreads();
             for (int bl = wid; bl < ZM_BC; bl += S::W) {
