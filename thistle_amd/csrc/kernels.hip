@@ -549,6 +549,7 @@ __global__ void __launch_bounds__(ZF_T) k_zform(const double* __restrict__ W3,
 // both stages.
 #define ZM_TI 16  // primitive pairs i per tile
 #define ZM_BC 16  // GTOs b per chunk
+#define ZM_NGMAX 320  // GTOs at most (the LDS bounds ngto below this anyway)
 #ifndef ZM_WMAX
 #define ZM_WMAX 16  // waves per block at NO = 32
 #endif
@@ -591,8 +592,9 @@ __global__ void __launch_bounds__(ZmShape<NO>::W * 64) k_zmfma(const double* __r
     }
     __syncthreads();
     for (int tile = ntile - 1 - (int)blockIdx.x; tile >= 0; tile -= (int)gridDim.x) {
-        const int64_t i0 = (int64_t)tile * ZM_TI, ilast = i0 + ZM_TI - 1;
-        const int64_t i = i0 + l16;
+        // npp < 2^24 (checked at plan set-up): pair indices fit 32 bits
+        const int i0 = tile * ZM_TI, ilast = i0 + ZM_TI - 1;
+        const int i = i0 + l16;
         const bool live = i < npp;
         // stage-1 B operands: Mo(p, s, i), k = s
         double mo[S::PW][S::KS];
@@ -605,7 +607,7 @@ __global__ void __launch_bounds__(ZmShape<NO>::W * 64) k_zmfma(const double* __r
                 double v = 0.0;
                 if (live && p < norb && s < norb) {
                     const int x = p < s ? pair_index(p, s, norb) : pair_index(s, p, norb);
-                    v = Mt[i + npp * x];
+                    v = Mt[i + npp * (int64_t)x];
                     if (s == p) v *= 2.0;
                 }
                 mo[pr][ks] = v;
@@ -636,31 +638,54 @@ __global__ void __launch_bounds__(ZmShape<NO>::W * 64) k_zmfma(const double* __r
                 }
             }
             // the chunk's pair columns, loaded while the MFMAs run
-            for (int t = tid; t < ZM_BC * ng16; t += S::W * 64) {
-                const int bl = t / ng16, a = t - bl * ng16, b = b0 + bl;
-                Jt[t] = (b < ngto && a < aend_of[b]) ? jtab[(int64_t)b * ngto + a] : INT_MAX;
+            // (jtab holds -1 past aend[b]); independent loads, one row per wave
+            for (int bl = wid; bl < ZM_BC; bl += S::W) {
+                const int b = b0 + bl;
+                const int32_t* row = jtab + (int64_t)b * ngto;
+                int v[(ZM_NGMAX + 63) / 64];
+#pragma unroll
+                for (int k = 0; k < (ZM_NGMAX + 63) / 64; ++k) {
+                    const int a = lane + 64 * k;
+                    v[k] = (b < ngto && a < ngto) ? row[a] : -1;
+                }
+#pragma unroll
+                for (int k = 0; k < (ZM_NGMAX + 63) / 64; ++k) {
+                    const int a = lane + 64 * k;
+                    if (a < ng16) Jt[bl * ng16 + a] = v[k] < 0 ? INT_MAX : v[k];
+                }
             }
             __syncthreads();
             for (int bl = wid; bl < ZM_BC; bl += S::W) {
                 const int32_t* jb = Jt + bl * ng16;
-                if (jb[0] > ilast) continue;
+                // a tiles whose first column reaches the triangle (a prefix)
+                int nat = 0;
+                for (int c0 = 0; c0 * 16 < ng16; c0 += 64) {
+                    const int c = c0 + lane;
+                    nat += __popcll(__ballot(c * 16 < ng16 && jb[c * 16] <= ilast));
+                }
+                nat = __builtin_amdgcn_readfirstlane(nat);
+                if (nat == 0) continue;
                 double wv[S::KS];
 #pragma unroll
                 for (int ks = 0; ks < S::KS; ++ks)
                     wv[ks] = Wc[bl * S::WB + (l4 + 4 * ks) * ZM_TI + l16];
-                // a tiles while the tile's first column reaches the triangle
-                for (int a0 = 0; a0 < ng16 && jb[a0] <= ilast; a0 += 16) {
+                for (int at = 0; at < nat; ++at) {
+                    const int a0 = 16 * at;
+                    double ca[S::KS];
+#pragma unroll
+                    for (int ks = 0; ks < S::KS; ++ks) ca[ks] = Cs[(a0 + l16) * S::CS + l4 + 4 * ks];
                     int jv[4];
 #pragma unroll
                     for (int r = 0; r < 4; ++r) jv[r] = jb[a0 + 4 * r + l4];
                     zdbl4 c0 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                     for (int ks = 0; ks < S::KS; ++ks)
-                        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                            Cs[(a0 + l16) * S::CS + l4 + 4 * ks], wv[ks], c0, 0, 0, 0);
+                        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[ks], wv[ks], c0, 0, 0, 0);
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (live && i >= jv[r]) __builtin_nontemporal_store(c0[r], &Z[i + npp * jv[r]]);
+                        if (live && i >= jv[r])
+                            __builtin_nontemporal_store(
+                                c0[r], Z + (uint64_t)(uint32_t)jv[r] * (uint64_t)npp + i);
                 }
             }
             __syncthreads();  // the chunk's W is consumed
@@ -673,7 +698,7 @@ bool zfact_fused(int norb, int ngto) {
     if (const char* ev = std::getenv("THISTLE_ZMFMA"))
         if (ev[0] == '0') return false;
     const int NO = zfact_width(norb);
-    return NO >= 8 && NO <= 32 && zm_lds_bytes(NO, ngto) <= 160 * 1024;
+    return NO >= 8 && NO <= 32 && ngto <= ZM_NGMAX && zm_lds_bytes(NO, ngto) <= 160 * 1024;
 }
 
 hipError_t zfact_fused_prepare(int norb, int ngto) {
@@ -699,7 +724,8 @@ hipError_t launch_zfact(const double* Mt, const double* C, int nmo, int norb, in
     hipLaunchKernelGGL(k_cpad, dim3((ngto * NO + 255) / 256), dim3(256), 0, s, C, nmo, norb,
                        ngto, NO, Cp);
     if (zgrid > 0) {
-        if (NO > 32 || zm_lds_bytes(NO, ngto) > 160 * 1024) return hipErrorInvalidValue;
+        if (NO > 32 || ngto > ZM_NGMAX || zm_lds_bytes(NO, ngto) > 160 * 1024)
+            return hipErrorInvalidValue;
         const size_t lds = zm_lds_bytes(NO, ngto);
         const int ntile = (int)((npp + ZM_TI - 1) / ZM_TI);
         const unsigned grid = (unsigned)std::min(ntile, zgrid);
