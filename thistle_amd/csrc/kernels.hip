@@ -550,9 +550,7 @@ __global__ void __launch_bounds__(ZF_T) k_zform(const double* __restrict__ W3,
 #define ZM_TI 16  // primitive pairs i per tile
 #define ZM_BC 16  // GTOs b per chunk
 #define ZM_NGMAX 320  // GTOs at most (the LDS bounds ngto below this anyway)
-#ifndef ZM_WMAX
-#define ZM_WMAX 16  // waves per block at NO = 32
-#endif
+#define ZM_WMAX 16  // waves per block at NO = 32 (8 measured 1.75 against 1.47 ms)
 typedef double zdbl4 __attribute__((ext_vector_type(4)));
 
 template <int NO>
@@ -574,7 +572,6 @@ template <int NO>
 __global__ void __launch_bounds__(ZmShape<NO>::W * 64) k_zmfma(const double* __restrict__ Mt,
                                                      const double* __restrict__ Cp,
                                                      const int32_t* __restrict__ jtab,
-                                                     const int32_t* __restrict__ aend_of,
                                                      int norb, int ngto, int64_t npp, int ntile,
                                                      double* __restrict__ Z) {
     using S = ZmShape<NO>;
@@ -731,8 +728,8 @@ hipError_t launch_zfact(const double* Mt, const double* C, int nmo, int norb, in
         const unsigned grid = (unsigned)std::min(ntile, zgrid);
 #define ZMF(NOV)                                                                             \
     case NOV:                                                                                \
-        hipLaunchKernelGGL(k_zmfma<NOV>, dim3(grid), dim3(ZmShape<NOV>::W * 64), lds, s, Mt, Cp, jtab,   \
-                           aend_of, norb, ngto, npp, ntile, Z);                              \
+        hipLaunchKernelGGL(k_zmfma<NOV>, dim3(grid), dim3(ZmShape<NOV>::W * 64), lds, s, Mt, Cp, \
+                           jtab, norb, ngto, npp, ntile, Z);                                 \
         break;
         switch (NO) {
             ZMF(8) ZMF(16) ZMF(32)
