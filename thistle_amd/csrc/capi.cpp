@@ -588,7 +588,6 @@ int run_series(ThistlePlan* pl, bool timing) {
         a.hs = pl->hs.as<double>() + pl->cls_hoff[LL];
         HCHECK(launch_contract(a, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
                                n, blocks, pl->stream));
-        HCHECK(launch_phase(a, a.j2 ? LL + 3 : LL + 1, n, pl->stream));
         if (ov) HCHECK(hipEventRecord(pl->cev[LL], pl->stream));
     }
     for (int q0 = 0; q0 < pl->nq; q0 += 512) {

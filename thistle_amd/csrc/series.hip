@@ -490,6 +490,7 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
     constexpr int R = HS_ROW(LL + 1);
     constexpr int NB = 32;  // quartet records staged in LDS per batch
     __shared__ double recs[NB * R];
+    __shared__ double phs[2 * NB];  // sin, cos of the records' phase steps
     const int lane = threadIdx.x;
     int iqs[SQ];
     double qv[SQ], qiv[SQ], S[SQ], Sc[SQ];
@@ -510,6 +511,18 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
         const double* src = A.hs + kb * R;
         __syncthreads();  // the previous batch is consumed
         for (int t = lane; t < nb * R; t += WAVE) recs[t] = src[t];
+        __syncthreads();
+        // the phase step sin, cos(64 dq h) between a lane's q slices on a
+        // uniform grid, one lane per staged record (0 on the h = 0 branch,
+        // which uses no sine); once per record and wave rather than per lane
+        if (A.qlin && lane < nb) {
+            const double* r = recs + lane * R;
+            const uint64_t m = (uint64_t)__double_as_longlong(r[LL + 3]);
+            double sd = 0.0, cd = 0.0;
+            if (!((m >> 44) & 1u)) sincos_cw((double)WAVE * A.dq * r[LL + 1], sd, cd);
+            phs[2 * lane] = sd;
+            phs[2 * lane + 1] = cd;
+        }
         __syncthreads();
         for (int j = 0; j < nb; ++j) {
             // the record is wave-uniform: kept in scalar registers
@@ -550,8 +563,7 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
                     sxs[0] = sb;
                     cxs[0] = cbb;
                     if (off) sincos_cw(qv[0] * h, sxs[0], cxs[0]);
-                    // the phase step comes with the record
-                    const double sd = uniform(rc[LL + 4]), cd = uniform(rc[LL + 5]);
+                    const double sd = uniform(phs[2 * j]), cd = uniform(phs[2 * j + 1]);
 #pragma unroll
                     for (int s = 1; s < SQ; ++s) {
                         const double ns = fma(sb, cd, cbb * sd), nc = fma(cbb, cd, -(sb * sd));
@@ -854,28 +866,6 @@ hipError_t set_contract_lds_limit(size_t bytes) {
     ALL_LL(C_)
 #undef C_
     return hipSuccess;
-}
-
-// sin, cos of 64 dq h of every record of one class, one thread per record
-// (a kernel of its own: inside k_contract the polynomial's registers would
-// lower the contraction's occupancy)
-__global__ void k_phase(double* __restrict__ rec, int64_t n, int R, int DO, double dq) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    double* r = rec + k * R;
-    const double h = r[DO];
-    const bool br = (((uint64_t)__double_as_longlong(r[DO + 2]) >> 44) & 1u) != 0;
-    double s = 0.0, c = 0.0;
-    if (!br) sincos_cw((double)WAVE * dq * h, s, c);
-    r[DO + 3] = s;
-    r[DO + 4] = c;
-}
-
-hipError_t launch_phase(const SeriesArgs& a, int DO, int64_t n, hipStream_t s) {
-    if (n <= 0 || !a.qlin) return hipSuccess;
-    hipLaunchKernelGGL(k_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.hs, n,
-                       HS_ROW(DO), DO, a.dq);
-    return hipGetLastError();
 }
 
 hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int waves,

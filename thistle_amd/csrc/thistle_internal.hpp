@@ -37,12 +37,10 @@ struct EnumRow {
 
 // Per-quartet records written by the contraction and read by the series
 // kernel, in class-list order: the DO series coefficients (DO = LL + 1, j2
-// LL + 3), then h = |P_ket - P_bra|, 1/h (0 on the h = 0 branch), the packed
-// pair indices and flags of HS_META, and sin, cos of 64 dq h (the phase step
-// between a lane's q slices on a uniform grid, written by k_phase; unset
-// otherwise).  The classes' records follow one another (largest LL last);
-// each class starts at the plan's offset for it.
-#define HS_ROW(DO) ((DO) + 5)
+// LL + 3), then h = |P_ket - P_bra|, 1/h (0 on the h = 0 branch) and the
+// packed pair indices and flags of HS_META.  The classes' records follow one
+// another (largest LL last); each class starts at the plan's offset for it.
+#define HS_ROW(DO) ((DO) + 3)
 // record word DO + 2: bra pair (bits 0-19), ket pair (20-39), multiplicity
 // (40-43), h = 0 branch (44)
 #define HS_META(pb, pk, mult, br)                                                \
@@ -210,9 +208,6 @@ hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int wave
                              hipStream_t s);
 // q h from which the series follows dphrec's start-100 recurrence
 #define SERIES_XDPH 74.0
-// k_phase: the phase-step words of the n records of one class (a.hs at the
-// class start, DO series coefficients per record); nothing off a uniform grid
-hipError_t launch_phase(const SeriesArgs& a, int DO, int64_t n, hipStream_t s);
 // k_series<LL>: S(q) of one LL class from its nidx quartet records (a.hs at
 // the class start); first = 1 overwrites the partial rows
 hipError_t launch_series(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,
