@@ -488,7 +488,7 @@ __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const doubl
 template <int LL, bool MIXED, bool J2>
 __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int first) {
     constexpr int R = HS_ROW(LL + 1);
-    constexpr int NB = 32;  // quartet records staged in LDS per batch
+    constexpr int NB = 64;  // quartet records staged in LDS per batch
     __shared__ double recs[NB * R];
     __shared__ double phs[2 * NB];  // sin, cos of the records' phase steps
     const int lane = threadIdx.x;
