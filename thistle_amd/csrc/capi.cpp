@@ -453,8 +453,9 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if (waves <= 0) {
         hipDeviceProp_t prop;
         HCHECK(hipGetDeviceProperties(&prop, o->device));
-        // series waves per CU: 12 (XY6 quartet stage 8.23 ms; 4: 8.78, 8: 8.29)
-        waves = prop.multiProcessorCount * 12;
+        // series waves per CU: 8 (XY6 quartet stage 7.03 ms; 6: 7.04, 12: 7.06
+        // with the classes in increasing LL)
+        waves = prop.multiProcessorCount * 8;
     }
     pl->waves = (int)std::max<int64_t>(1, std::min<int64_t>(waves, Q));
     pl->waves_alloc = pl->waves;
