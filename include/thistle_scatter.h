@@ -46,10 +46,23 @@ enum {
                                 one shell (same l+m+n, at most 15 GTOs)     */
     THISTLE_EDEVICE = 5,     /* HIP or rocBLAS failure                       */
     THISTLE_ENOMEM = 6,      /* device allocation failed                     */
-    THISTLE_ETYPE = 7        /* calculation type other than 1 (total j0),
+    THISTLE_ETYPE = 7,       /* calculation type other than 1 (total j0),
                                 2 (elastic j0), 5 / 6 (total / elastic
                                 electron), 7 (total j2) or 8 (elastic j2)    */
+    THISTLE_ECUTOFF = 8      /* cutoffmd above THISTLE_CUTOFFMD_MAX         */
 };
+
+/* The reference drops every McMurchie-Davidson product with
+ * |prod6| <= cutoffmd (src/Integrals.f90:5198-5203).  The device contraction
+ * sums the MD products factorised per axis and keeps them all, so it equals
+ * the reference only where the dropped products are negligible.  Measured
+ * with the oracle (the reference's loop, screen included) on an s/p/d/f
+ * three-atom case over q up to 14: cutoffmd 1e-20 -> 3.6e-14 per q,
+ * 1e-16 -> 3.6e-13, 1e-12 -> 7.3e-4 (the screen now removes real terms).
+ * Calls with cutoffmd above this bound return THISTLE_ECUTOFF instead of a
+ * silently different S(q); the reference's default and suggested value is
+ * 1e-20. */
+#define THISTLE_CUTOFFMD_MAX 1e-18
 
 enum { THISTLE_FP64 = 0, THISTLE_MIXED = 1 };
 
@@ -64,8 +77,10 @@ typedef struct ThistleBasis {
     int32_t nmo;                  /* rows of mmod                          */
     const double *mmod;           /* (nmo, ngto) MO coefficients           */
     int32_t nq;
-    const double *q;              /* [nq] |q|, a.u.                        */
-    double cutoffz, cutoffmd, cutoffcentre;
+    const double *q;              /* [nq] |q|, a.u.: finite and >= 0, else
+                                     THISTLE_EINVAL                        */
+    double cutoffz, cutoffmd, cutoffcentre;  /* finite, >= 0; cutoffmd <=
+                                     THISTLE_CUTOFFMD_MAX                  */
 } ThistleBasis;
 
 typedef struct ThistleOpts {
@@ -230,6 +245,28 @@ int32_t thistle_plan_result(ThistlePlan *plan, double *result_out);
 int32_t thistle_plan_timings(ThistlePlan *plan, double *ms5, int64_t *nquart,
                              int64_t *units);
 void thistle_plan_destroy(ThistlePlan *plan);
+/* Mixed precision (THISTLE_MIXED) evaluates the series in FP32 where
+ * q h > LLmax + 6 (the upward Bessel recurrence), FP64 elsewhere, and sums
+ * S(q) in FP64 with compensation.  It also sums A2(q), the squares of the
+ * FP32-evaluated terms' error scales |weight| sum_b (b + 2) |h_b u_b|, and
+ * then re-evaluates in FP64, from the same q-independent h(beta) records,
+ * every q point with |S(q)| < 4 * 2^-24 * 1e6 * sqrt(A2(q)): near the zeros
+ * of S(q) the FP32 error could otherwise exceed 1e-6 |S(q)|.  After an
+ * execution this returns the number of re-evaluated points, their indices
+ * (idx, when cap >= count) and A2(q) (scale, nq doubles, optional).  count
+ * is 0 for an FP64 plan.  No counterpart in the reference, which runs in
+ * FP64. */
+int32_t thistle_plan_mixed_fallback(ThistlePlan *plan, int32_t *count,
+                                    int32_t *idx, int32_t cap, double *scale);
+
+/* Restricts the plan's quartet stage to the given positions of the
+ * reference enumeration (0-based, strictly increasing, < Q_ref), or
+ * restores every quartet when positions is NULL.  Later executions return
+ * the partial S(q) of those quartets: sums over a sample, or one shard of
+ * the quartet list.  No counterpart in the reference, whose five nests
+ * (src/Integrals.f90:79-416) always run whole. */
+int32_t thistle_plan_set_quartets(ThistlePlan *plan, const int64_t *positions,
+                                  int64_t n);
 
 /* Batched frames (total j0).  No counterpart in the reference, which runs
  * one geometry per call.  nframes independent problems, frame f given by

@@ -57,7 +57,7 @@ def _declare(L):
     L.orc_total_j2.restype = _i
     L.orc_set_p0_j2.argtypes = [_p, _i, _p, _i]
     L.orc_total_j0_sample.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
-                                      _p, _i, _p, _p, _i, _p, _i, _d, _d, _d, _i,
+                                      _p, _i, _p, _p, _i, _p, _i, _d, _d, _d, _i, _i,
                                       C.c_int64, _p, _p, _p]
     L.orc_total_j0_sample.restype = _i
     L.orc_elastic_j0.argtypes = [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
@@ -237,8 +237,10 @@ def set_p0_j2(lmax4, q):
 
 def total_j0_sample(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q, stride,
                     cutoffz=1e-15, cutoffmd=1e-20, cutoffcentre=0.001, nthreads=0,
-                    maxl=None):
+                    maxl=None, zmode=1):
     """Time the per-quartet kernels on an LL-stratified 1/stride sample.
+    zmode 0 contracts the reference's z11/z22 arrays per quartet (its dgemm
+    step) inside the timed region; zmode 1 supplies the blocks untimed.
 
     Returns (partial S(q), kernel seconds, quartets sampled)."""
     l, m, n, group = _I(l), _I(m), _I(n), _I(group)
@@ -256,7 +258,8 @@ def total_j0_sample(l, m, n, group, ga, xx, yy, zz, mmod, mat, total, q, stride,
                                    _ptr(ga), _ptr(xx), _ptr(yy), _ptr(zz), _ptr(mmod),
                                    mmod.shape[0], _ptr(mat), _ptr(total), len(total),
                                    _ptr(q), len(q), cutoffz, cutoffmd, cutoffcentre,
-                                   nthreads, int(stride), _ptr(out), _ptr(secs), _ptr(ns))
+                                   int(zmode), nthreads, int(stride), _ptr(out), _ptr(secs),
+                                   _ptr(ns))
     if rc:
         raise RuntimeError(f"oracle total_j0_sample status {rc}")
     return out, float(secs[0]), int(ns[0])

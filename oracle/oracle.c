@@ -1236,6 +1236,13 @@ static int total_j0_impl(int ngto, int ng, int maxl, const int *l, const int *m,
             int gi = o[0] - 1, gj = o[1] - 1, gk = o[2] - 1, gr = o[3] - 1;
             int mult = o[4], br = o[5];
             int ci = S.gc[gi], cj = S.gc[gj], ck = S.gc[gk], cr = S.gc[gr];
+            /* zmode 0 follows the reference's per-quartet contraction of the
+             * z11/z22 arrays, part of each quartet's work in tot_integration,
+             * so the kernel timer covers it */
+            double t0 = 0.0;
+#ifdef _OPENMP
+            if (zmode == 0) t0 = omp_get_wtime();
+#endif
             /* quartet 2-RDM block: zcontrred + zcontrred2 (the dgemm calls of
              * src/Integrals.f90:109-127 and the four analogues) */
             for (int a = 0; a < ci; ++a)
@@ -1272,9 +1279,8 @@ static int total_j0_impl(int ngto, int ng, int maxl, const int *l, const int *m,
                             }
                             zb[((a * cj + b) * ck + c) * cr + d] = v1 + v2;
                         }
-            double t0 = 0.0;
 #ifdef _OPENMP
-            t0 = omp_get_wtime();
+            if (zmode != 0) t0 = omp_get_wtime();
 #endif
             if (br) {
                 ijkr_pzero(&C, gi, gj, gk, gr, zb, cutoffz, cutoffmd, S.P0, S.D0,
@@ -1359,13 +1365,13 @@ int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
                         const double *zz, const double *mmod, int nmo,
                         const int *mat, const double *total, int nmat,
                         const double *q, int nq, double cutoffz,
-                        double cutoffmd, double cutoffcentre, int nthreads,
-                        int64_t stride, double *tsi, double *kernel_seconds,
-                        int64_t *nsampled)
+                        double cutoffmd, double cutoffcentre, int zmode,
+                        int nthreads, int64_t stride, double *tsi,
+                        double *kernel_seconds, int64_t *nsampled)
 {
     return total_j0_impl(ngto, ng, maxl, l, m, n, group, ga, xx, yy, zz, mmod,
                          nmo, mat, total, nmat, q, nq, cutoffz, cutoffmd,
-                         cutoffcentre, 1, nthreads, tsi, stride,
+                         cutoffcentre, zmode, nthreads, tsi, stride,
                          kernel_seconds, nsampled, 0);
 }
 

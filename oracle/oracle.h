@@ -117,9 +117,9 @@ int orc_total_j0_sample(int ngto, int ng, int maxl, const int *l,
                         const double *zz, const double *mmod, int nmo,
                         const int *mat, const double *total, int nmat,
                         const double *q, int nq, double cutoffz,
-                        double cutoffmd, double cutoffcentre, int nthreads,
-                        int64_t stride, double *tsi, double *kernel_seconds,
-                        int64_t *nsampled);
+                        double cutoffmd, double cutoffcentre, int zmode,
+                        int nthreads, int64_t stride, double *tsi,
+                        double *kernel_seconds, int64_t *nsampled);
 int orc_elastic_j0(int ngto, int ng, int maxl, const int *l, const int *m,
                    const int *n, const int *group, const double *ga,
                    const double *xx, const double *yy, const double *zz,

@@ -112,9 +112,10 @@ def _small_basis():
     return ga, l, m, n, xyz, group, mmod, q
 
 
-def _status(type_, ga, l, m, n, xyz, group, mmod, q, mat, tot, norb=0, onerdm=None):
+def _status(type_, ga, l, m, n, xyz, group, mmod, q, mat, tot, norb=0, onerdm=None,
+            cutoffs=(1e-15, 1e-20, 1e-3)):
     lib = L.load()
-    b = L.make_basis(ga, l, m, n, xyz, xyz, xyz, group, mmod, q, 1e-15, 1e-20, 1e-3)
+    b = L.make_basis(ga, l, m, n, xyz, xyz, xyz, group, mmod, q, *cutoffs)
     o = L.make_opts(device=0)
     plan = C.c_void_p()
     mat = None if mat is None else np.require(mat, np.int32, ["F"])
@@ -151,6 +152,43 @@ def test_status_codes_before_device(lib):
     assert _status(7, ga, gl, m, n, xyz, group, mmod, q, mat, tot) == 3
     assert _status(2, ga, l, m, n, xyz, group, mmod, q, None, None, 3, np.eye(3)) == 1
     assert _status(2, ga, l, m, n, xyz, group, mmod, q, None, None, 0, None) == 1
+
+
+def test_q_grid_and_cutoffs_are_checked(lib):
+    """|q| must be finite and >= 0 (x = q h < 0 would break the series, whose
+    reference form sin(pmu)/pmu is even in q); cutoffs finite and >= 0."""
+    ga, l, m, n, xyz, group, mmod, q = _small_basis()
+    mat = np.array([[1, 1, 1, 1]])
+    tot = np.array([1.0])
+    for bad in (-0.5, np.nan, np.inf):
+        qb = q.copy()
+        qb[1] = bad
+        assert _status(1, ga, l, m, n, xyz, group, mmod, qb, mat, tot) == 1
+    for cut in ((-1.0, 1e-20, 1e-3), (1e-15, np.nan, 1e-3), (1e-15, 1e-20, -1e-3)):
+        assert _status(1, ga, l, m, n, xyz, group, mmod, q, mat, tot, cutoffs=cut) == 1
+
+
+def test_cutoffmd_rule(lib):
+    """The device contraction keeps every MD product; the reference drops
+    |prod6| <= cutoffmd (src/Integrals.f90:5198-5203).  Above
+    THISTLE_CUTOFFMD_MAX the call returns THISTLE_ECUTOFF (8) instead of a
+    different S(q); at and below it the arguments pass (the next status is
+    the invalid 2-RDM row, checked after the basis)."""
+    ga, l, m, n, xyz, group, mmod, q = _small_basis()
+    tot = np.array([1.0])
+    badrow = np.array([[1, 1, 3, 1]])
+    assert _status(1, ga, l, m, n, xyz, group, mmod, q, badrow, tot,
+                   cutoffs=(1e-15, 1e-9, 1e-3)) == 1          # row check comes first
+    good = np.array([[1, 1, 1, 1]])
+    for md, want in ((1e-9, 8), (1e-17, 8), (2 * L.CUTOFFMD_MAX, 8)):
+        assert _status(2, ga, l, m, n, xyz, group, mmod, q, None, None, 2, np.eye(2),
+                       cutoffs=(1e-15, md, 1e-3)) == want
+    with pytest.raises(L.ThistleError) as e:
+        sc.total_from_twordm(ga, l, m, n, xyz, xyz, xyz, mmod, q, group, good, tot,
+                             cutoffmd=1e-12)
+    assert e.value.code == 8
+    header = open(HEADER).read()
+    assert re.search(r"#define THISTLE_CUTOFFMD_MAX 1e-18", header)
 
 
 def test_python_errors_are_raised(lib):

@@ -73,6 +73,14 @@ int dalloc(DevBuf& b, size_t bytes) {
 
 const ThistleOpts kDefaultOpts = {THISTLE_FP64, 1e-30, nullptr, 0, 0};
 
+// Mixed precision: a q point is re-evaluated in FP64 when |S(q)| <
+// kMixedKappa * sqrt(A2(q)), A2(q) = sum over the FP32-evaluated terms of
+// e^2, e = |weight| sum_b (b + 2) |h_b u_b| (the term's FP32 error scale).
+// The terms' rounding errors are independent, so their sum has the size of
+// 2^-24 sqrt(A2); kMixedKappa = 4 * 2^-24 / 1e-6 flags every point where four
+// times that could exceed 1e-6 |S(q)|.
+constexpr double kMixedKappa = 4.0 * 5.9604644775390625e-08 / 1e-6;
+
 }  // namespace
 
 struct ThistlePlan {
@@ -85,6 +93,7 @@ struct ThistlePlan {
     int qlin = 0, q0off = 0;  // q[k] = qa + k dq for k >= 1; q[0] apart
     double qa = 0.0, dq = 0.0;
     int64_t npp = 0, nquart = 0, nmat = 0, units = 0;
+    int64_t nlisted = 0;  // quartets in the class lists (nquart, or a subset)
     double cutoffz = 0, cutoffmd = 0, cutoffcentre = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -96,6 +105,11 @@ struct ThistlePlan {
     DevBuf mmod, mat, total, rmat, rtot, hkeys, hmin, hcnt, hslot, uscan;
     DevBuf V, Gs, T, M, Z, Cp, W3, jtab, aend, mopairs, xa, xb, onerdm;
     DevBuf part, partc, S, cls;
+    // mixed precision: FP32 error scale per wave / per q, the q points
+    // re-evaluated in FP64 and their count, the re-evaluated sums
+    DevBuf parta, Aq, qsel, nsel, Sfix;
+    double kappa = 0.0;
+    int32_t last_nsel = 0;
     std::vector<int> gstart, gcount, gL;  // per primitive shell (host)
     int64_t cls_off[18] = {0};  // per-LL ranges in `cls` (positions, ref order)
     int64_t cls_hoff[18] = {0}; // per-LL start of the class's records in `hs` (doubles)
@@ -134,6 +148,13 @@ int check_basis(const ThistleBasis* b, std::vector<GroupInfo>& groups, int& maxl
         !b->n || !b->xx || !b->yy || !b->zz || !b->group || !b->q)
         return THISTLE_EINVAL;
     if (b->ng > 1024) return THISTLE_EINVAL;
+    // |q| grid: the series is even in q, so the reference's sin(pmu)/pmu
+    // stays right for q < 0 while x = q h < 0 would not; reject it
+    for (int k = 0; k < b->nq; ++k)
+        if (!std::isfinite(b->q[k]) || b->q[k] < 0.0) return THISTLE_EINVAL;
+    for (double c : {b->cutoffz, b->cutoffmd, b->cutoffcentre})
+        if (!std::isfinite(c) || c < 0.0) return THISTLE_EINVAL;
+    if (b->cutoffmd > THISTLE_CUTOFFMD_MAX) return THISTLE_ECUTOFF;
     int maxv = 0;
     for (int j = 0; j < b->ngto; ++j) {
         if (b->l[j] < 0 || b->m[j] < 0 || b->n[j] < 0) return THISTLE_EINVAL;
@@ -308,6 +329,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     int64_t Q = 0;
     std::vector<EnumRow> rows = enum_rows(ng, Q, pl->type == 8);
     pl->nquart = Q;
+    pl->nlisted = Q;
     pl->nrows = (int)rows.size();
     // positions of each LL = L_i+L_j+L_k+L_r class, in reference order
     std::vector<int32_t> cls(Q);
@@ -463,6 +485,15 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if ((rc = dalloc(pl->part, sizeof(double) * (size_t)pl->waves * nqc))) return rc;
     if ((rc = dalloc(pl->partc, sizeof(double) * (size_t)pl->waves * nqc))) return rc;
     if ((rc = dalloc(pl->S, sizeof(double) * b->nq))) return rc;
+    if (pl->precision == THISTLE_MIXED) {
+        if ((rc = dalloc(pl->parta, sizeof(double) * (size_t)pl->waves * nqc))) return rc;
+        if ((rc = dalloc(pl->Aq, sizeof(double) * (size_t)b->nq))) return rc;
+        if ((rc = dalloc(pl->qsel, sizeof(int32_t) * b->nq))) return rc;
+        if ((rc = dalloc(pl->nsel, sizeof(int32_t) * 2))) return rc;
+        if ((rc = dalloc(pl->Sfix, sizeof(double) * nqc))) return rc;
+        pl->kappa = kMixedKappa;
+        if (const char* ev = std::getenv("THISTLE_MIXED_KAPPA")) pl->kappa = std::atof(ev);
+    }
     if (b->nmo > 0 && b->mmod) {
         pl->nmo = b->nmo;
         if ((rc = dalloc(pl->mmod, sizeof(double) * (size_t)b->nmo * b->ngto))) return rc;
@@ -553,6 +584,9 @@ SeriesArgs series_args(const ThistlePlan* pl) {
     a.cutoffz = pl->cutoffz;
     a.part = pl->part.as<double>();
     a.partc = pl->partc.as<double>();
+    a.parta = pl->parta.as<double>();
+    a.qsel = pl->qsel.as<int32_t>();
+    a.nsel = pl->nsel.as<int32_t>();
     a.flags = pl->flags.as<uint8_t>();
     a.hs = pl->hs.as<double>();
     a.quadtab = pl->quadtab.as<uint64_t>();
@@ -601,6 +635,8 @@ int run_series(ThistlePlan* pl, bool timing) {
         const int64_t rows = (int64_t)pl->waves * a.nqc;
         HCHECK(launch_fill(pl->part.as<double>(), rows, 0.0, ss));
         HCHECK(launch_fill(pl->partc.as<double>(), rows, 0.0, ss));
+        a.pstride = rows;
+        if (a.mixed) HCHECK(launch_fill(pl->parta.as<double>(), rows, 0.0, ss));
         for (int LL = 0; LL <= 16; ++LL) {
             int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
             if (n <= 0) continue;
@@ -612,6 +648,32 @@ int run_series(ThistlePlan* pl, bool timing) {
             if (pl->far) HCHECK(launch_series_far(a, a.j2 ? LL + 2 : LL, n, w, ss));
         }
         HCHECK(launch_reduce(a.part, a.partc, pl->waves, a.nqc, pl->S.as<double>() + q0, ss));
+        if (a.mixed)
+            HCHECK(launch_reduce_plain(a.parta, pl->waves, a.nqc, pl->Aq.as<double>() + q0, ss));
+    }
+    if (a.mixed) {
+        // second pass: the q points whose FP32 error scale is not small
+        // against |S(q)|, in FP64 (every regime) from the same records
+        HCHECK(launch_qflag(pl->S.as<double>(), pl->Aq.as<double>(), pl->nq, pl->kappa,
+                            pl->qsel.as<int32_t>(), pl->nsel.as<int32_t>(), ss));
+        const int nqc = std::min(512, pl->nq);
+        a.nqc = nqc;
+        for (int pbase = 0; pbase < pl->nq; pbase += nqc) {
+            a.pbase = pbase;
+            const int64_t rows = (int64_t)pl->waves * nqc;
+            HCHECK(launch_fill(pl->part.as<double>(), rows, 0.0, ss));
+            HCHECK(launch_fill(pl->partc.as<double>(), rows, 0.0, ss));
+            for (int LL = 0; LL <= 16; ++LL) {
+                int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
+                if (n <= 0) continue;
+                int w = (int)std::min<int64_t>(pl->waves, n);
+                a.hs = pl->hs.as<double>() + pl->cls_hoff[LL];
+                HCHECK(launch_series_pts(a, a.j2 ? LL + 2 : LL, n, w, 0, ss));
+            }
+            HCHECK(launch_reduce(a.part, a.partc, pl->waves, nqc, pl->Sfix.as<double>(), ss));
+            HCHECK(launch_qscatter(pl->Sfix.as<double>(), pl->qsel.as<int32_t>(),
+                                   pl->nsel.as<int32_t>(), pbase, nqc, pl->S.as<double>(), ss));
+        }
     }
     if (ov) {
         HCHECK(hipEventRecord(pl->join_ev, pl->aux));
@@ -623,7 +685,54 @@ int run_series(ThistlePlan* pl, bool timing) {
         HCHECK(hipEventRecord(pl->ev[4], pl->stream));
         HCHECK(hipEventRecord(pl->ev[5], pl->stream));
     }
-    pl->units = pl->nquart * (int64_t)pl->nq;
+    pl->units = pl->nlisted * (int64_t)pl->nq;
+    return THISTLE_OK;
+}
+
+// Class lists of a subset of the enumeration (positions strictly
+// increasing, so every class keeps reference order), or of every quartet
+// when pos is null.
+int set_quartets(ThistlePlan* pl, const int64_t* pos, int64_t n) {
+    int64_t Q = 0;
+    std::vector<EnumRow> rows = enum_rows(pl->ng, Q, pl->type == 8);
+    if (Q != pl->nquart) return THISTLE_EINVAL;
+    std::vector<int64_t> all;
+    if (!pos) {
+        n = Q;
+    } else {
+        if (n < 0 || n > Q) return THISTLE_EINVAL;
+        for (int64_t k = 0; k < n; ++k)
+            if (pos[k] < 0 || pos[k] >= Q || (k > 0 && pos[k] <= pos[k - 1]))
+                return THISTLE_EINVAL;
+    }
+    std::vector<int8_t> LLof((size_t)Q);
+    for (const auto& R : rows)
+        for (int u = 0; u < R.count; ++u) {
+            int g[4];
+            row_quartet(R, R.r0 + u, g);
+            LLof[(size_t)(R.out + u)] =
+                (int8_t)(pl->gL[g[0]] + pl->gL[g[1]] + pl->gL[g[2]] + pl->gL[g[3]]);
+        }
+    std::vector<int64_t> cnt(18, 0);
+    for (int64_t k = 0; k < n; ++k) cnt[LLof[(size_t)(pos ? pos[k] : k)]]++;
+    const int dj = (pl->type == 7 || pl->type == 8) ? 3 : 1;
+    pl->cls_off[0] = 0;
+    pl->cls_hoff[0] = 0;
+    for (int k = 0; k < 17; ++k) {
+        pl->cls_off[k + 1] = pl->cls_off[k] + cnt[k];
+        pl->cls_hoff[k + 1] = pl->cls_hoff[k] + cnt[k] * HS_ROW(k + dj);
+    }
+    std::vector<int32_t> cls((size_t)std::max<int64_t>(n, 1));
+    std::vector<int64_t> fill(pl->cls_off, pl->cls_off + 17);
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t e = pos ? pos[k] : k;
+        cls[(size_t)fill[LLof[(size_t)e]]++] = (int32_t)e;
+    }
+    pl->nlisted = n;
+    if (n > 0)
+        HCHECK(hipMemcpyAsync(pl->cls.p, cls.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                              pl->stream));
+    HCHECK(hipStreamSynchronize(pl->stream));
     return THISTLE_OK;
 }
 
@@ -1235,6 +1344,25 @@ int32_t thistle_plan_execute(ThistlePlan* pl, double* result_out) {
         HCHECK(hipEventElapsedTime(&t[k], pl->ev[k], pl->ev[k + 1]));
         pl->ms[k] = t[k];
     }
+    if (pl->precision == THISTLE_MIXED)
+        HCHECK(hipMemcpy(&pl->last_nsel, pl->nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    return THISTLE_OK;
+}
+
+int32_t thistle_plan_mixed_fallback(ThistlePlan* pl, int32_t* count, int32_t* idx, int32_t cap,
+                                    double* scale) {
+    if (!pl || !count) return THISTLE_EINVAL;
+    *count = 0;
+    if (pl->precision != THISTLE_MIXED) return THISTLE_OK;
+    HCHECK(hipSetDevice(pl->device));
+    HCHECK(hipStreamSynchronize(pl->stream));
+    int32_t n = 0;
+    HCHECK(hipMemcpy(&n, pl->nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    *count = n;
+    if (idx && cap >= n && n > 0)
+        HCHECK(hipMemcpy(idx, pl->qsel.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (scale)
+        HCHECK(hipMemcpy(scale, pl->Aq.p, sizeof(double) * pl->nq, hipMemcpyDeviceToHost));
     return THISTLE_OK;
 }
 
@@ -1248,9 +1376,15 @@ int32_t thistle_plan_timings(ThistlePlan* pl, double* ms5, int64_t* nquart, int6
     if (!pl) return THISTLE_EINVAL;
     if (ms5)
         for (int k = 0; k < 5; ++k) ms5[k] = pl->ms[k];
-    if (nquart) *nquart = pl->nquart;
+    if (nquart) *nquart = pl->nlisted;
     if (units) *units = pl->units;
     return THISTLE_OK;
+}
+
+int32_t thistle_plan_set_quartets(ThistlePlan* pl, const int64_t* positions, int64_t n) {
+    if (!pl) return THISTLE_EINVAL;
+    HCHECK(hipSetDevice(pl->device));
+    return set_quartets(pl, positions, n);
 }
 
 void thistle_plan_destroy(ThistlePlan* pl) { delete pl; }

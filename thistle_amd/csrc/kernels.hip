@@ -845,6 +845,24 @@ hipError_t launch_reduce(const double* part, const double* partc, int waves, int
     return hipGetLastError();
 }
 
+// out[q] = sum of part[w][q] over the waves in order (the FP32 error scale of
+// mixed mode: a magnitude, summed without compensation)
+__global__ void k_reduce_plain(const double* __restrict__ part, int waves, int nqc,
+                               double* __restrict__ out) {
+    const int iq = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iq >= nqc) return;
+    double s = 0.0;
+    for (int w = 0; w < waves; ++w) s += part[(int64_t)w * nqc + iq];
+    out[iq] = s;
+}
+
+hipError_t launch_reduce_plain(const double* part, int waves, int nqc, double* out,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_plain, dim3(nblk(nqc, 128)), dim3(128), 0, s, part, waves, nqc,
+                       out);
+    return hipGetLastError();
+}
+
 hipError_t launch_exclusive_scan_u8(const uint8_t* in, int64_t n, int64_t* out,
                                     int64_t* total, hipStream_t s) {
     // bsum lives right after `out` (caller allocates n + nb + 1 int64)

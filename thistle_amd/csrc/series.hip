@@ -402,28 +402,37 @@ __device__ __forceinline__ double f_up64(const double (&hs)[LL + 1], const doubl
     return F;
 }
 
+// Mixed precision: the upward recurrence of f_up64 in FP32, on the same
+// u_b = j_b(x) (q/h)^b with the quartet's coefficients hf and cbf = (2b+1)/h^2
+// rounded once per record.  sin, cos x come from the FP64 phase (an FP32
+// argument x would carry an absolute error of 6e-8 x into them).  Fa
+// receives sum_b |hw_b u_b|, hw_b = (b + 2) |h_b|: the rounding error of u_b
+// grows about one unit per recurrence step, so this is the scale of the
+// term's FP32 error.
 template <int LL>
-__device__ __forceinline__ double f_up32(const float (&hf)[LL + 1], double q, double invh,
-                                         double x) {
-    float s, c;
-    sincos_mixed(x, s, c);
-    const float inv = __builtin_amdgcn_rcpf((float)x);
-    const float j0 = s * inv;
+__device__ __forceinline__ float f_up32(const float (&hf)[LL + 1], const float (&hw)[LL + 1],
+                                        const float (&cbf)[LL + 1], float q, float invh,
+                                        float x, float sx, float cx, float& Fa) {
+    const float inv = __builtin_amdgcn_rcpf(x);
+    const float j0 = sx * inv;
     float F = j0 * hf[0];
+    Fa = fabsf(j0) * hw[0];
     if (LL >= 1) {
-        const float r = (float)(q * invh);
-        float jm = j0, jc = (j0 - c) * inv, rb = r;
-        F = fmaf(jc * rb, hf[1], F);
+        const float r = q * invh;
+        const float nr2 = -(r * r);
+        float um = j0, uc = (j0 - cx) * cbf[0];
+        F = fmaf(uc, hf[1], F);
+        Fa = fmaf(fabsf(uc), hw[1], Fa);
 #pragma unroll
         for (int b = 1; b < LL; ++b) {
-            float jn = fmaf((float)(2 * b + 1) * inv, jc, -jm);
-            jm = jc;
-            jc = jn;
-            rb *= r;
-            F = fmaf(jc * rb, hf[b + 1], F);
+            const float un = fmaf(cbf[b], uc, nr2 * um);
+            um = uc;
+            uc = un;
+            F = fmaf(uc, hf[b + 1], F);
+            Fa = fmaf(fabsf(uc), hw[b + 1], Fa);
         }
     }
-    return (double)F;
+    return F;
 }
 
 #define SQ 4   // q slices (of 64 points) per wave
@@ -448,18 +457,24 @@ __device__ __forceinline__ double f_pzero_j2(const double (&hs)[LL + 1], double 
 // sx, cx = sin, cos of x = q h, computed by the caller.  Points with
 // x >= XDPH are left to k_series_far (0 here): the dphrec call would cost
 // this kernel about 50 registers per lane for a regime most workloads never
-// reach.
+// reach.  MIXED: x > LL + 6 runs in FP32 (hf, cbf: the record's coefficients
+// rounded to FP32) and Fa receives the scale of its rounding error; Fa is 0
+// on every FP64 branch.
 template <int LL, bool MIXED>
 __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
+                                           const float (&hf)[LL + 1], const float (&hw)[LL + 1],
+                                           const float (&cbf)[LL + 1],
                                            double q, double qinv, double h, double invh,
-                                           double sx, double cx) {
+                                           double sx, double cx, double& Fa) {
     const double x = q * h;
+    Fa = 0.0;
     if (x >= XDPH) return 0.0;
     if (MIXED && x > (double)LL + 6.0) {
-        float hf[LL + 1];
-#pragma unroll
-        for (int b = 0; b <= LL; ++b) hf[b] = (float)hs[b];
-        return f_up32<LL>(hf, q, invh, x);
+        float fa;
+        const float F = f_up32<LL>(hf, hw, cbf, (float)q, (float)invh, (float)x, (float)sx,
+                                   (float)cx, fa);
+        Fa = (double)fa;
+        return (double)F;
     }
     const double inv = invh * qinv;
     // q = 0 (x = 0): the j0 limit 1 instead of 0/0
@@ -473,15 +488,27 @@ __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const dou
 // reference's 0/0), or the j2 h = 0 polynomial.
 template <int LL, bool MIXED, bool J2>
 __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const double (&cb)[LL + 1],
+                                         const float (&hf)[LL + 1], const float (&hw)[LL + 1],
+                                         const float (&cbf)[LL + 1],
                                          double q, double qinv, double h, double invh, int br,
-                                         double sx, double cx) {
+                                         double sx, double cx, double& Fa) {
     if constexpr (J2) {
-        if (br) return f_pzero_j2<LL>(hs, q);
-        const double F = f_series<LL, MIXED>(hs, cb, q, qinv, h, invh, sx, cx);
-        return q > 0.0 ? F / (q * q) : (LL >= 1 ? hs[LL >= 1 ? 1 : 0] * (1.0 / 3.0) : 0.0);
+        if (br) {
+            Fa = 0.0;
+            return f_pzero_j2<LL>(hs, q);
+        }
+        const double F = f_series<LL, MIXED>(hs, cb, hf, hw, cbf, q, qinv, h, invh, sx, cx, Fa);
+        if (q > 0.0) {
+            Fa = Fa / (q * q);
+            return F / (q * q);
+        }
+        return LL >= 1 ? hs[LL >= 1 ? 1 : 0] * (1.0 / 3.0) : 0.0;
     } else {
-        if (br) return f_pzero<LL>(hs, q);
-        return f_series<LL, MIXED>(hs, cb, q, qinv, h, invh, sx, cx);
+        if (br) {
+            Fa = 0.0;
+            return f_pzero<LL>(hs, q);
+        }
+        return f_series<LL, MIXED>(hs, cb, hf, hw, cbf, q, qinv, h, invh, sx, cx, Fa);
     }
 }
 
@@ -493,14 +520,14 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
     __shared__ double phs[2 * NB];  // sin, cos of the records' phase steps
     const int lane = threadIdx.x;
     int iqs[SQ];
-    double qv[SQ], qiv[SQ], S[SQ], Sc[SQ];
+    double qv[SQ], qiv[SQ], S[SQ], Sc[SQ], Sa2[SQ];
 #pragma unroll
     for (int s = 0; s < SQ; ++s) {
         const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
         iqs[s] = A.q0 + (iql < A.nqc ? iql : 0);
         qv[s] = A.q[iqs[s]];
         qiv[s] = qv[s] > 0.0 ? rcp64(qv[s]) : 0.0;
-        S[s] = Sc[s] = 0.0;
+        S[s] = Sc[s] = Sa2[s] = 0.0;
     }
     // a contiguous chunk of the class's records per wave, staged NB at a time
     // by coalesced loads; each record then is an LDS broadcast
@@ -549,6 +576,14 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
             double cb[LL + 1];
 #pragma unroll
             for (int b = 0; b <= LL; ++b) cb[b] = (double)(2 * b + 1) * ih2;
+            // mixed: the FP32 regime's coefficients, rounded once per record
+            float hf[LL + 1], hw[LL + 1], cbf[LL + 1];
+#pragma unroll
+            for (int b = 0; b <= LL; ++b) {
+                hf[b] = MIXED ? (float)hs[b] : 0.0f;
+                hw[b] = MIXED ? (float)(b + 2) * fabsf(hf[b]) : 0.0f;
+                cbf[b] = MIXED ? (float)cb[b] : 0.0f;
+            }
             // sin, cos of q h on the lane's SQ q points.  On a uniform grid the
             // points are x0 + s D, D = 64 dq h, so one sincos of x0, one of D and
             // rotations give them all; a q[0] off the grid gets its own.
@@ -579,11 +614,18 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
             }
 #pragma unroll
             for (int s = 0; s < SQ; ++s) {
-                double v = mult * f_eval<LL, MIXED, J2>(hs, cb, qv[s], qiv[s], h, invh, br,
-                                                        sxs[s], cxs[s]);
+                double Fa;
+                const double F = f_eval<LL, MIXED, J2>(hs, cb, hf, hw, cbf, qv[s], qiv[s], h,
+                                                       invh, br, sxs[s], cxs[s], Fa);
+                double v = mult * F;
                 v = v * ebv[s];
                 v = v * ekv[s];
                 kahan(S[s], Sc[s], v);
+                // error scale of the FP32 terms: |weight| sum_b |h_b u_b|
+                if (MIXED) {
+                    const double e = fabs((mult * ebv[s]) * ekv[s]) * Fa;
+                    Sa2[s] = fma(e, e, Sa2[s]);
+                }
             }
         }
     }
@@ -592,6 +634,7 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
         const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
         if (iql >= A.nqc) continue;
         int64_t o = (int64_t)blockIdx.x * A.nqc + iql;
+        if (MIXED) A.parta[o] = first ? Sa2[s] : A.parta[o] + Sa2[s];
         if (first) {
             A.part[o] = S[s];
             A.partc[o] = Sc[s];
@@ -661,6 +704,122 @@ __global__ void __launch_bounds__(WAVE) k_series_far(SeriesArgs A, int64_t nidx)
         A.part[o] = ps;
         A.partc[o] = pc;
     }
+}
+
+// Mixed precision, second pass: the flagged q points (A.qsel[A.pbase + k],
+// k < min(*A.nsel - A.pbase, nqc)) evaluated in FP64 in every regime,
+// dphrec's included, from the class's records.  The lanes of a wave split
+// into NF point lanes (NF the power of two covering the points, at most 64)
+// times 64 / NF quartet lanes; each lane walks its quartets in order, and
+// the quartet lanes of one point are combined in a fixed butterfly, so the
+// sums are deterministic.  Row w of part / partc holds wave w's sums.
+template <int LL, bool J2>
+__global__ void __launch_bounds__(WAVE) k_series_pts(SeriesArgs A, int64_t nidx) {
+    constexpr int R = HS_ROW(LL + 1);
+    const int lane = threadIdx.x;
+    const int left = *A.nsel - A.pbase;
+    const int np = left < A.nqc ? left : A.nqc;
+    if (np <= 0) return;
+    int NF = 1;
+    while (NF < np && NF < WAVE) NF <<= 1;
+    const int NJ = WAVE / NF, k = lane & (NF - 1), jl = lane / NF;
+    const int64_t k0 = (int64_t)blockIdx.x * nidx / gridDim.x,
+                  k1 = ((int64_t)blockIdx.x + 1) * nidx / gridDim.x;
+    const float zf[LL + 1] = {};
+    for (int p0 = 0; p0 < np; p0 += NF) {
+        const int pt = p0 + k;
+        const bool live = pt < np;
+        const int iq = A.qsel[A.pbase + (live ? pt : 0)];
+        const double q = A.q[iq], qinv = q > 0.0 ? rcp64(q) : 0.0;
+        double S = 0.0, Sc = 0.0;
+        for (int64_t kq = k0 + jl; kq < k1; kq += NJ) {
+            const double* rc = A.hs + kq * R;
+            double hs[LL + 1];
+#pragma unroll
+            for (int b = 0; b <= LL; ++b) hs[b] = rc[b];
+            const double h = rc[LL + 1], invh = rc[LL + 2];
+            const uint64_t meta = (uint64_t)__double_as_longlong(rc[LL + 3]);
+            const int pb = (int)(meta & 0xFFFFFu), pk = (int)((meta >> 20) & 0xFFFFFu);
+            const double mult = (double)((meta >> 40) & 15u);
+            const int br = (int)((meta >> 44) & 1u);
+            const double ih2 = invh * invh;
+            double cb[LL + 1];
+#pragma unroll
+            for (int b = 0; b <= LL; ++b) cb[b] = (double)(2 * b + 1) * ih2;
+            double F, Fa;
+            const double x = q * h;
+            if (!br && x >= XDPH) {
+                double sx, cx;
+                sincos_cw(x, sx, cx);
+                F = f_dphrec<LL>(rc, q, h, x, sx, cx);
+                if constexpr (J2) F = F / (q * q);
+            } else {
+                double sx = 0.0, cx = 0.0;
+                if (!br) sincos_cw(x, sx, cx);
+                F = f_eval<LL, false, J2>(hs, cb, zf, zf, zf, q, qinv, h, invh, br, sx, cx, Fa);
+            }
+            double v = mult * F;
+            v = v * A.e12[(int64_t)pb * A.nq + iq];
+            v = v * A.e12[(int64_t)pk * A.nq + iq];
+            kahan(S, Sc, v);
+        }
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            if (off < NF) continue;
+            const double os = __shfl_xor(S, off, WAVE), oc = __shfl_xor(Sc, off, WAVE);
+            kahan(S, Sc, os);
+            kahan(S, Sc, -oc);
+        }
+        if (jl == 0 && live) {
+            const int64_t o = (int64_t)blockIdx.x * A.nqc + pt;
+            double ps = A.part[o], pc = A.partc[o];
+            kahan(ps, pc, S);
+            kahan(ps, pc, -Sc);
+            A.part[o] = ps;
+            A.partc[o] = pc;
+        }
+    }
+}
+
+// One workgroup: the q points with |S(q)| < kappa sqrt(A2(q)), in
+// increasing q order (a NaN S(q) is flagged as well).
+#define QF_T 1024
+__global__ void __launch_bounds__(QF_T) k_qflag(const double* __restrict__ S,
+                                                const double* __restrict__ Aq, int nq,
+                                                double kappa, int32_t* __restrict__ qsel,
+                                                int32_t* __restrict__ nsel) {
+    __shared__ int wsum_[QF_T / WAVE];
+    __shared__ int base;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int q0 = 0; q0 < nq; q0 += QF_T) {
+        const int iq = q0 + tid;
+        const bool f = iq < nq && Aq[iq] > 0.0 && !(fabs(S[iq]) >= kappa * sqrt(Aq[iq]));
+        const uint64_t bal = __ballot(f);
+        const int below = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum_[wid] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wid; ++w) off += wsum_[w];
+        if (f) qsel[off + below] = iq;
+        __syncthreads();
+        if (tid == 0) {
+            int t = 0;
+            for (int w = 0; w < QF_T / WAVE; ++w) t += wsum_[w];
+            base += t;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *nsel = base;
+}
+
+__global__ void k_qscatter(const double* __restrict__ Sfix, const int32_t* __restrict__ qsel,
+                           const int32_t* __restrict__ nsel, int pbase, int nqc,
+                           double* __restrict__ S) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int left = *nsel - pbase;
+    if (k < nqc && k < left) S[qsel[pbase + k]] = Sfix[k];
 }
 
 // --------------------------------------------- electron-nucleus (types 5/6)
@@ -885,6 +1044,39 @@ hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int wave
 #undef CASE
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_series_pts(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,
+                             hipStream_t s) {
+    (void)first;
+    if (nidx <= 0) return hipSuccess;
+    switch (LL) {
+#define CASE(k)                                                                              \
+    case k:                                                                                  \
+        if (a.j2)                                                                            \
+            hipLaunchKernelGGL((k_series_pts<k, true>), dim3(waves), dim3(WAVE), 0, s, a, nidx); \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_series_pts<k, false>), dim3(waves), dim3(WAVE), 0, s, a, nidx); \
+        break;
+        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_qflag(const double* S, const double* Aq, int nq, double kappa, int32_t* qsel,
+                        int32_t* nsel, hipStream_t s) {
+    hipLaunchKernelGGL(k_qflag, dim3(1), dim3(QF_T), 0, s, S, Aq, nq, kappa, qsel, nsel);
+    return hipGetLastError();
+}
+
+hipError_t launch_qscatter(const double* Sfix, const int32_t* qsel, const int32_t* nsel,
+                           int pbase, int nqc, double* S, hipStream_t s) {
+    hipLaunchKernelGGL(k_qscatter, dim3((nqc + 255) / 256), dim3(256), 0, s, Sfix, qsel, nsel,
+                       pbase, nqc, S);
     return hipGetLastError();
 }
 

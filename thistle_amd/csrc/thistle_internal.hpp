@@ -154,6 +154,14 @@ struct SeriesArgs {
     double cutoffz;
     double* part;             // [waves][nqc] sums
     double* partc;            // [waves][nqc] Kahan compensations
+    double* parta;            // mixed: [2][waves][nqc] error scale of the FP32 terms
+                              // (sum of magnitudes, then of their squares)
+    int64_t pstride;          // waves x nqc of the plan's partial rows
+    // FP64 re-evaluation of flagged q points (mixed mode): their indices in
+    // increasing order and their count, both on the device
+    const int32_t* qsel;
+    const int32_t* nsel;
+    int pbase;                // first flagged point of this launch
     uint8_t* flags;           // [nquart] block passes the ztot screen
     double* hs;               // [nquart][17] h(beta) per quartet
     const uint64_t* quadtab;  // GTO-quadruple tables (packed, see setup_common)
@@ -214,6 +222,21 @@ hipError_t launch_series(const SeriesArgs& a, int LL, int64_t nidx, int waves, i
                          hipStream_t s);
 hipError_t launch_reduce(const double* part, const double* partc, int waves,
                          int nqc, double* out, hipStream_t s);
+// out[q] = sum over waves of part[w][q], waves in order (no compensation)
+hipError_t launch_reduce_plain(const double* part, int waves, int nqc, double* out,
+                               hipStream_t s);
+// Mixed mode: the q points whose FP32 error scale may exceed the per-point
+// bound, |S(q)| < kappa * A(q); qsel receives them in increasing order and
+// *nsel their count
+hipError_t launch_qflag(const double* S, const double* Aq, int nq, double kappa,
+                        int32_t* qsel, int32_t* nsel, hipStream_t s);
+// FP64 S(q) of one class at the flagged points pbase .. pbase + 511 (a.qsel,
+// a.nsel), one row of nqc = 512 partial sums per wave; first = 1 overwrites
+hipError_t launch_series_pts(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,
+                             hipStream_t s);
+// S[qsel[pbase + k]] = Sfix[k] for the flagged points of one launch
+hipError_t launch_qscatter(const double* Sfix, const int32_t* qsel, const int32_t* nsel,
+                           int pbase, int nqc, double* S, hipStream_t s);
 hipError_t launch_compact(const uint8_t* flags, const uint64_t* in, int64_t n,
                           int64_t* scratch, uint64_t* out, int64_t* count_dev,
                           hipStream_t s);

@@ -17,7 +17,11 @@ THISTLE_OK = 0
 STATUS = {0: "ok", 1: "invalid argument", 2: "a GTO has l, m or n > 4",
           3: "quartet LLmax + 1 > 20", 4: "a group is not one contiguous single-shell run",
           5: "HIP / rocBLAS failure", 6: "device allocation failed",
-          7: "calculation type must be 1 (total j0), 2 (elastic j0) or 7 (total j2)"}
+          7: "calculation type must be 1, 2, 5, 6, 7 or 8 (total / elastic j0, total / "
+             "elastic electron, total / elastic j2)",
+          8: "cutoffmd above THISTLE_CUTOFFMD_MAX (1e-18): the factorised contraction keeps "
+             "every MD product"}
+CUTOFFMD_MAX = 1e-18
 FP64, MIXED = 0, 1
 
 EXPORTS = [
@@ -29,7 +33,7 @@ EXPORTS = [
     "thistle_quartet_list_screened",
     "thistle_unique_total", "thistle_pair_tables", "thistle_plan_create",
     "thistle_plan_execute", "thistle_plan_result", "thistle_plan_timings",
-    "thistle_plan_destroy", "thistle_batch_create", "thistle_batch_execute",
+    "thistle_plan_destroy", "thistle_plan_set_quartets", "thistle_plan_mixed_fallback", "thistle_batch_create", "thistle_batch_execute",
     "thistle_batch_info", "thistle_batch_destroy", "thistle_device_count",
     "thistle_device_synchronize",
     "thistle_version",
@@ -114,6 +118,10 @@ def _declare(L):
     L.thistle_plan_timings.restype = i32
     L.thistle_plan_destroy.argtypes = [p]
     L.thistle_plan_destroy.restype = None
+    L.thistle_plan_set_quartets.argtypes = [p, p, i64]
+    L.thistle_plan_set_quartets.restype = i32
+    L.thistle_plan_mixed_fallback.argtypes = [p, p, p, i32, p]
+    L.thistle_plan_mixed_fallback.restype = i32
     L.thistle_batch_create.argtypes = [i32, p, p, p, p, p, C.POINTER(C.c_void_p)]
     L.thistle_batch_create.restype = i32
     L.thistle_batch_execute.argtypes = [p, p]

@@ -348,6 +348,28 @@ class Plan:
         L.check(self.lib.thistle_plan_result(self.h, L._ptr(out)), "plan_result")
         return out
 
+    def mixed_fallback(self):
+        """After a mixed-precision execution: (indices of the q points
+        re-evaluated in FP64, A2(q) the sum of the squared error scales of
+        the FP32-evaluated terms)."""
+        n = C.c_int32()
+        idx = np.zeros(self.nq, dtype=np.int32)
+        scale = np.zeros(self.nq)
+        L.check(self.lib.thistle_plan_mixed_fallback(self.h, C.byref(n), L._ptr(idx), self.nq,
+                                                     L._ptr(scale)), "plan_mixed_fallback")
+        return idx[:n.value].copy(), scale
+
+    def set_quartets(self, positions=None):
+        """Restrict later executions to these 0-based positions of the
+        reference quartet enumeration (strictly increasing); None restores
+        every quartet."""
+        if positions is None:
+            L.check(self.lib.thistle_plan_set_quartets(self.h, None, 0), "plan_set_quartets")
+            return
+        pos = np.require(np.asarray(positions, dtype=np.int64), requirements=["C"])
+        L.check(self.lib.thistle_plan_set_quartets(self.h, L._ptr(pos), int(pos.size)),
+                "plan_set_quartets")
+
     def timings(self):
         ms = np.zeros(5)
         nq = C.c_int64()
