@@ -244,6 +244,12 @@ int32_t thistle_plan_result(ThistlePlan *plan, double *result_out);
  * (quartets x q points) of the last execution. */
 int32_t thistle_plan_timings(ThistlePlan *plan, double *ms5, int64_t *nquart,
                              int64_t *units);
+/* Last execution's kernel spans in ms, by hipEvents on the streams the
+ * kernels run on: the contraction kernels (every LL class, plan stream,
+ * first launch to last) and the series kernels (series stream, first series
+ * launch to the last reduction, waits on the contractions included). */
+int32_t thistle_plan_kernel_timings(ThistlePlan *plan, double *contract_ms,
+                                    double *series_ms);
 void thistle_plan_destroy(ThistlePlan *plan);
 /* Mixed precision (THISTLE_MIXED) evaluates the series in FP32 where
  * q h > LLmax + 6 (the upward Bessel recurrence), FP64 elsewhere, and sums

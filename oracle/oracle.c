@@ -1113,6 +1113,7 @@ static void build_z12(const double *mmod, int nmo, int ngto, const int *mat,
                       const double *total, int nmat, double *z1, double *z2)
 {
 #define MM(i, j) mmod[(size_t)(i) + (size_t)nmo * (size_t)(j)]
+#pragma omp parallel for schedule(static)
     for (int b = 0; b < ngto; ++b)
         for (int a = 0; a < ngto; ++a) {
             double *o1 = z1 + (size_t)nmat * (a + (size_t)ngto * b);

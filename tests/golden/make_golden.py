@@ -44,6 +44,11 @@ CASES = {
                                    nact=2, nelec=2, ncore=0, nq=16, qmax=5.0),
     # X-Y-X, 4s2p2d per atom (66 functions), 3 core + CAS(4,4), 100 q
     "xyx_cas44": lambda: synth.make_xyx(nact=4, nelec=4, nq=100),
+    # the X-Y-X workload itself: 3 core + CAS(8,8) (4900 determinants), 500 q
+    # (the harness selects the 2-RDM at the reference's 1e-30)
+    "xyx": lambda: synth.make_xyx(),
+    # atoms 12 bohr apart, q to 10: q h up to about 120, dphrec's far regime
+    "far": lambda: synth.make_far(),
 }
 
 

@@ -36,12 +36,15 @@ def test_xy6_flops():
 
 
 def test_profile_figures_are_read():
-    """The bench line's traffic and executed flops come from the committed
-    XY6 profile; other workloads report none."""
+    """The bench line's k_contract traffic and executed flops come from the
+    committed XY6 profile; other workloads report none."""
     import bench
-    t = bench.measured_traffic("xy6", "fp64")
-    ex = bench.measured_executed_flops("xy6", "fp64")
-    assert t is not None and t > 2.798e9  # at least the Z triangle
-    assert ex is not None and 1e10 < ex < 6.811e11  # below the algorithmic count
-    assert bench.measured_executed_flops("h2", "fp64") is None
-    assert bench.measured_traffic("xy6", "mixed") is None
+    c = bench.contract_counters("xy6")
+    assert c is not None
+    assert c["bytes"] > 2.798e9  # at least the Z triangle
+    assert 1e10 < c["flops"] < 6.811e11  # below the algorithmic count
+    assert bench.contract_counters("h2") is None
+
+
+def test_cpu_model_is_named():
+    assert isinstance(bench._cpu_model(), str) and bench._cpu_model()

@@ -12,7 +12,7 @@ from thistle_amd import scattering as sc
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["h2", "sp3", "spd2", "f2", "xyx_cas44"]
+CASES = ["h2", "sp3", "spd2", "f2", "xyx_cas44", "xyx", "far"]
 RTOL_FP64 = 1e-10
 
 

@@ -48,7 +48,7 @@ def test_version(lib):
     assert b"gfx950" in lib.thistle_version()
 
 
-CASES = ["h2", "sp3", "spd2", "f2", "xyx_cas44"]
+CASES = ["h2", "sp3", "spd2", "f2", "xyx_cas44", "xyx", "far"]
 
 
 @pytest.mark.parametrize("name", CASES)

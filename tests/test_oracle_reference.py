@@ -10,8 +10,16 @@ import pytest
 
 import oracle
 
-CASES = ["h2", "sp3", "spd2", "f2", "xyx_cas44"]
+CASES = ["h2", "sp3", "spd2", "f2", "xyx_cas44", "xyx", "far"]
 PI_SP = 3.1415927410125732421875   # acos(-1.0000) in single precision (src/Variables.f90:994)
+
+
+def _close(S, ref, rtol=1e-12, ntol=1e-14):
+    """Per q within rtol, except near a zero of S(q), where the absolute error
+    is held to ntol of max |S| (both sums round differently there)."""
+    err = np.abs(S - ref)
+    ok = err <= rtol * np.abs(ref) + ntol * np.max(np.abs(ref))
+    return bool(np.all(ok)), float(np.max(err / np.abs(ref)))
 
 
 @pytest.fixture(scope="module", params=CASES)
@@ -49,8 +57,8 @@ def test_total_j0(case, zmode):
     mat, tot = g["mat"].astype(np.int32), g["total"]
     S = oracle.total_j0(g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"], g["yy"],
                         g["zz"], g["mmod"], mat, tot, g["q"], zmode=zmode, **_cut(g))
-    rel = np.max(np.abs(S - g["S_total"]) / np.abs(g["S_total"]))
-    assert rel < 1e-12, (name, rel)
+    ok, rel = _close(S, g["S_total"])
+    assert ok, (name, rel)
 
 
 def test_elastic_j0(case):

@@ -119,6 +119,10 @@ struct ThistlePlan {
     int nsig = 0;
     hipEvent_t ev[6] = {};
     double ms[5] = {0, 0, 0, 0, 0};
+    // contraction kernels (plan stream) and series kernels (series stream)
+    // of the last timed execution: first launch to last launch
+    hipEvent_t kev[4] = {};
+    double kms[2] = {0, 0};
     // series kernels on a second stream, each behind its class's contraction
     hipStream_t aux = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr, cev[17] = {};
@@ -132,6 +136,8 @@ struct ThistlePlan {
     ~ThistlePlan() {
         drop_overlap();
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : kev)
             if (e) (void)hipEventDestroy(e);
         if (blas) rocblas_destroy_handle(blas);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -292,6 +298,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
         pl->own_stream = true;
     }
     for (auto& e : pl->ev) HCHECK(hipEventCreate(&e));
+    for (auto& e : pl->kev) HCHECK(hipEventCreate(&e));
     // THISTLE_OVERLAP=0 keeps every stage on the one stream
     const char* ov = std::getenv("THISTLE_OVERLAP");
     if (!(ov && ov[0] == '0')) {
@@ -618,6 +625,7 @@ int run_series(ThistlePlan* pl, bool timing) {
     // high classes' the other way round (Johnson's two-stage order), so the
     // series stream has work while the heavy contractions run and little is
     // left after the last one (XY6 quartet stage 7.30 ms in decreasing order)
+    if (timing) HCHECK(hipEventRecord(pl->kev[0], pl->stream));
     for (int LL = 0; LL <= 16; ++LL) {
         int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
         if (n <= 0) continue;
@@ -629,6 +637,8 @@ int run_series(ThistlePlan* pl, bool timing) {
                                n, blocks, pl->stream));
         if (ov) HCHECK(hipEventRecord(pl->cev[LL], pl->stream));
     }
+    if (timing) HCHECK(hipEventRecord(pl->kev[1], pl->stream));
+    bool first_series = true;
     for (int q0 = 0; q0 < pl->nq; q0 += 512) {
         a.q0 = q0;
         a.nqc = std::min(512, pl->nq - q0);
@@ -641,6 +651,8 @@ int run_series(ThistlePlan* pl, bool timing) {
             int64_t n = pl->cls_off[LL + 1] - pl->cls_off[LL];
             if (n <= 0) continue;
             if (ov && q0 == 0) HCHECK(hipStreamWaitEvent(ss, pl->cev[LL], 0));
+            if (timing && first_series) HCHECK(hipEventRecord(pl->kev[2], ss));
+            first_series = false;
             int w = (int)std::min<int64_t>(pl->waves, n);
             a.hs = pl->hs.as<double>() + pl->cls_hoff[LL];
             // j2 carries its series two orders further (bessels0rr(LLmax+2))
@@ -650,6 +662,10 @@ int run_series(ThistlePlan* pl, bool timing) {
         HCHECK(launch_reduce(a.part, a.partc, pl->waves, a.nqc, pl->S.as<double>() + q0, ss));
         if (a.mixed)
             HCHECK(launch_reduce_plain(a.parta, pl->waves, a.nqc, pl->Aq.as<double>() + q0, ss));
+    }
+    if (timing) {
+        if (first_series) HCHECK(hipEventRecord(pl->kev[2], ss));
+        HCHECK(hipEventRecord(pl->kev[3], ss));
     }
     if (a.mixed) {
         // second pass: the q points whose FP32 error scale is not small
@@ -1344,8 +1360,20 @@ int32_t thistle_plan_execute(ThistlePlan* pl, double* result_out) {
         HCHECK(hipEventElapsedTime(&t[k], pl->ev[k], pl->ev[k + 1]));
         pl->ms[k] = t[k];
     }
+    float kt[2];
+    HCHECK(hipEventElapsedTime(&kt[0], pl->kev[0], pl->kev[1]));
+    HCHECK(hipEventElapsedTime(&kt[1], pl->kev[2], pl->kev[3]));
+    pl->kms[0] = kt[0];
+    pl->kms[1] = kt[1];
     if (pl->precision == THISTLE_MIXED)
         HCHECK(hipMemcpy(&pl->last_nsel, pl->nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    return THISTLE_OK;
+}
+
+int32_t thistle_plan_kernel_timings(ThistlePlan* pl, double* contract_ms, double* series_ms) {
+    if (!pl) return THISTLE_EINVAL;
+    if (contract_ms) *contract_ms = pl->kms[0];
+    if (series_ms) *series_ms = pl->kms[1];
     return THISTLE_OK;
 }
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "thistle_quartet_list_screened",
     "thistle_unique_total", "thistle_pair_tables", "thistle_plan_create",
     "thistle_plan_execute", "thistle_plan_result", "thistle_plan_timings",
-    "thistle_plan_destroy", "thistle_plan_set_quartets", "thistle_plan_mixed_fallback", "thistle_batch_create", "thistle_batch_execute",
+    "thistle_plan_destroy", "thistle_plan_set_quartets", "thistle_plan_kernel_timings", "thistle_plan_mixed_fallback", "thistle_batch_create", "thistle_batch_execute",
     "thistle_batch_info", "thistle_batch_destroy", "thistle_device_count",
     "thistle_device_synchronize",
     "thistle_version",
@@ -118,6 +118,8 @@ def _declare(L):
     L.thistle_plan_timings.restype = i32
     L.thistle_plan_destroy.argtypes = [p]
     L.thistle_plan_destroy.restype = None
+    L.thistle_plan_kernel_timings.argtypes = [p, p, p]
+    L.thistle_plan_kernel_timings.restype = i32
     L.thistle_plan_set_quartets.argtypes = [p, p, i64]
     L.thistle_plan_set_quartets.restype = i32
     L.thistle_plan_mixed_fallback.argtypes = [p, p, p, i32, p]

@@ -376,8 +376,12 @@ class Plan:
         units = C.c_int64()
         L.check(self.lib.thistle_plan_timings(self.h, L._ptr(ms), C.byref(nq), C.byref(units)),
                 "plan_timings")
+        kc, ks = C.c_double(), C.c_double()
+        L.check(self.lib.thistle_plan_kernel_timings(self.h, C.byref(kc), C.byref(ks)),
+                "plan_kernel_timings")
         return {"pair_tables": ms[0], "enumeration": ms[1], "transform": ms[2],
-                "quartets": ms[3], "reduction": ms[4], "nquart": nq.value, "units": units.value}
+                "quartets": ms[3], "reduction": ms[4], "nquart": nq.value, "units": units.value,
+                "contract": kc.value, "series": ks.value}
 
     def close(self):
         if getattr(self, "h", None):

@@ -186,6 +186,18 @@ def make_xyx(seed=2, nact=8, nelec=8, ncore=3, nq=500, cutoff=1e-12):
                    qgrid(10.0, nq), cutoff)
 
 
+def make_far(seed=7, sep=12.0, nq=64, qmax=10.0):
+    """Two atoms `sep` bohr apart and one between them, s/p/d shells, 1 core +
+    CAS(2,2): q h reaches past 74 (q h up to qmax * sep), where the reference's
+    dphrec, started at order 100, departs from the exact j_beta."""
+    rng = np.random.default_rng(seed)
+    coords = np.array([[0.0, 0.0, 0.0], [0.0, 0.3 * sep, 0.5 * sep], [0.0, 0.0, sep]])
+    shells = []
+    for a in range(3):
+        shells += _shells_for(a, {"s": 2, "p": 1, "d": 1}, rng, 0.1, 20.0)
+    return _finish("far", rng, [1, 1, 1], coords, shells, 1, 2, 2, qgrid(qmax, nq))
+
+
 def make_xy6(seed=3, nact=10, nelec=10, ncore=20, nq=512):
     rng = np.random.default_rng(seed)
     d = 2.95
