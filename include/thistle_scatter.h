@@ -215,6 +215,30 @@ int64_t thistle_unique_total(const int32_t *matrix, int64_t nrow,
                              double *rtot, int32_t *iuni, int32_t *irec,
                              const ThistleOpts *opts);
 
+/* Allocation-free unique_total.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops).  Same result as thistle_unique_total, on device pointers:
+ * matrix_dev (nrow, ncol) column-major, total_dev [nrow]; out rmat_dev
+ * (nrow, ncol) column-major with leading dimension nrow (the first *count
+ * rows hold the unique rows), rtot_dev / iuni_dev [nrow] (first *count
+ * valid), irec_dev [nrow], count_dev one int64 (any of rmat/rtot/iuni/irec
+ * may be NULL).  Values must lie in 0..65535 (ncol <= 4) or 0..255
+ * (ncol <= 8); device data is not checked.  The work is enqueued on
+ * opts->stream with no allocation, copy to the host or synchronisation;
+ * nrow above THISTLE_NOALLOC_MAX_ROWS returns THISTLE_EINVAL.  The hash
+ * table and scan live in one static device buffer of
+ * THISTLE_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends and
+ * cleared by the call itself: calls that overlap on one GPU (other streams
+ * or threads) race on it and must be serialised by the caller. */
+#define THISTLE_NOALLOC_MAX_ROWS (1 << 18)
+#define THISTLE_NOALLOC_SCRATCH_BYTES 11535376
+int32_t thistle_unique_total_noalloc(const int32_t *matrix_dev, int64_t nrow,
+                                     int32_t ncol, const double *total_dev,
+                                     int32_t *rmat_dev, double *rtot_dev,
+                                     int32_t *iuni_dev, int32_t *irec_dev,
+                                     int64_t *count_dev,
+                                     const ThistleOpts *opts);
+
 /* Pair tables of variables_total (src/Variables.f90:1113-1150): for each
  * group pair gi <= gj (pair index p in row-major upper-triangle order)
  * the Hermite tables E[p][axis][la][lb][t] (strides (maxl+1)^2(2maxl+1)),

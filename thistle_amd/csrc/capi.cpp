@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "thistle_internal.hpp"
@@ -1952,6 +1953,72 @@ int64_t thistle_unique_total(const int32_t* matrix, int64_t nrow, int32_t ncol, 
     if (irec && nrow > 0 && hipMemcpy(irec, dir.p, sizeof(int32_t) * nrow, hipMemcpyDeviceToHost))
         return -THISTLE_EDEVICE;
     return u;
+}
+
+}  // extern "C"
+
+// ---- allocation-free unique_total --------------------------------------
+// The hash table, first-occurrence flags and scan of launch_unique live in
+// one file-scope device array, sized for THISTLE_NOALLOC_MAX_ROWS rows.  In
+// host code `scratch` only names the symbol: its device address is taken
+// once per device with hipGetSymbolAddress and that address is what the
+// kernels receive.
+namespace {
+constexpr int64_t kNaRows = THISTLE_NOALLOC_MAX_ROWS;
+constexpr int64_t kNaHash = 2 * kNaRows;                      // slots at most
+constexpr int64_t kNaScan = kNaRows + (kNaRows + 2047) / 2048 + 2;
+constexpr size_t kNaKeys = 0;                                  // uint64[kNaHash]
+constexpr size_t kNaMin = kNaKeys + 8 * (size_t)kNaHash;       // int32[kNaHash]
+constexpr size_t kNaCnt = kNaMin + 4 * (size_t)kNaHash;        // int32[kNaHash]
+constexpr size_t kNaSlot = kNaCnt + 4 * (size_t)kNaHash;       // int32[kNaRows]
+constexpr size_t kNaScanOff = (kNaSlot + 4 * (size_t)kNaRows + 255) & ~(size_t)255;
+constexpr size_t kNaBytes = kNaScanOff + 8 * (size_t)kNaScan;  // int64[kNaScan]
+static_assert(kNaBytes == THISTLE_NOALLOC_SCRATCH_BYTES, "scratch size");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+std::mutex g_na_mu;
+void* g_na_ptr[64] = {};
+
+// the device address of `scratch` on the current device (looked up once)
+int noalloc_scratch(int device, unsigned char** out) {
+    if (device < 0 || device >= 64) return THISTLE_EINVAL;
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void* p = nullptr;
+        HCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return THISTLE_EDEVICE;
+        g_na_ptr[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_na_ptr[device]);
+    return THISTLE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t thistle_unique_total_noalloc(const int32_t* matrix_dev, int64_t nrow, int32_t ncol,
+                                     const double* total_dev, int32_t* rmat_dev,
+                                     double* rtot_dev, int32_t* iuni_dev, int32_t* irec_dev,
+                                     int64_t* count_dev, const ThistleOpts* opts) {
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    if (nrow < 0 || nrow > kNaRows || ncol < 1 || ncol > 8 || !count_dev ||
+        (nrow > 0 && (!matrix_dev || !total_dev)))
+        return THISTLE_EINVAL;
+    HCHECK(hipSetDevice(o->device));
+    unsigned char* sc = nullptr;
+    int rc = noalloc_scratch(o->device, &sc);
+    if (rc) return rc;
+    int64_t hs = 16;
+    while (hs < 2 * std::max<int64_t>(nrow, 1)) hs <<= 1;
+    UniqueWork w{reinterpret_cast<uint64_t*>(sc + kNaKeys), reinterpret_cast<int32_t*>(sc + kNaMin),
+                 reinterpret_cast<int32_t*>(sc + kNaCnt), reinterpret_cast<int32_t*>(sc + kNaSlot),
+                 reinterpret_cast<int64_t*>(sc + kNaScanOff), hs};
+    HCHECK(launch_unique(matrix_dev, nrow, ncol, total_dev, w, rmat_dev, rtot_dev, iuni_dev,
+                         irec_dev, count_dev, (hipStream_t)o->stream));
+    return THISTLE_OK;
 }
 
 int32_t thistle_pair_tables(const ThistleBasis* basis, double* E, double* P, double* e12,

@@ -31,7 +31,7 @@ EXPORTS = [
     "thistle_createtwordm", "thistle_nuclei_electron",
     "thistle_onerdm", "thistle_createonerdm", "thistle_quartet_list",
     "thistle_quartet_list_screened",
-    "thistle_unique_total", "thistle_pair_tables", "thistle_plan_create",
+    "thistle_unique_total", "thistle_unique_total_noalloc", "thistle_pair_tables", "thistle_plan_create",
     "thistle_plan_execute", "thistle_plan_result", "thistle_plan_timings",
     "thistle_plan_destroy", "thistle_plan_set_quartets", "thistle_plan_kernel_timings", "thistle_plan_mixed_fallback", "thistle_batch_create", "thistle_batch_execute",
     "thistle_batch_info", "thistle_batch_destroy", "thistle_device_count",
@@ -106,6 +106,8 @@ def _declare(L):
     L.thistle_quartet_list_screened.restype = i64
     L.thistle_unique_total.argtypes = [p, i64, i32, p, p, p, p, p, p]
     L.thistle_unique_total.restype = i64
+    L.thistle_unique_total_noalloc.argtypes = [p, i64, i32, p, p, p, p, p, p, p]
+    L.thistle_unique_total_noalloc.restype = i32
     L.thistle_pair_tables.argtypes = [p, p, p, p, p]
     L.thistle_pair_tables.restype = i32
     L.thistle_plan_create.argtypes = [i32, p, i64, p, p, i32, p, p, C.POINTER(C.c_void_p)]
