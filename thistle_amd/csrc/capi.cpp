@@ -250,6 +250,84 @@ std::vector<EnumRow> enum_rows(int ng, int64_t& total, bool ej2 = false) {
     return rows;
 }
 
+// Group pairs gi <= gj (pair_index order) with their blocks in primitive
+// pair space, and the GTOs (a, b) of every primitive pair.  Returns npp.
+int64_t build_pairs(const std::vector<GroupInfo>& groups, std::vector<PairInfo>& pairs,
+                    std::vector<int32_t>& xa, std::vector<int32_t>& xb) {
+    const int ng = (int)groups.size();
+    pairs.assign((size_t)ng * (ng + 1) / 2, PairInfo{});
+    xa.clear();
+    xb.clear();
+    int64_t off = 0;
+    for (int gi = 0; gi < ng; ++gi)
+        for (int gj = gi; gj < ng; ++gj) {
+            PairInfo& p = pairs[pair_index(gi, gj, ng)];
+            std::memset(&p, 0, sizeof p);
+            p.gi = gi;
+            p.gj = gj;
+            p.off = (int32_t)off;
+            p.nab = groups[gi].count * groups[gj].count;
+            for (int a = 0; a < groups[gi].count; ++a)
+                for (int c = 0; c < groups[gj].count; ++c) {
+                    xa.push_back(groups[gi].start + a);
+                    xb.push_back(groups[gj].start + c);
+                }
+            off += p.nab;
+        }
+    return off;
+}
+
+// GTO-quadruple tables of the signature combinations in use.  Entry u of
+// a combination (d fastest): bits 0-7 a*cj+b, 8-15 c*cr+d (offsets in the
+// bra / ket pair blocks), 16-25 / 26-35 / 36-45 the x / y / z exponent
+// tuple index, 46-61 a, b, c, d (4 bits each).  The ket offset runs
+// fastest, so that a wave's Z reads fall on few cache lines.  (A copy
+// with the bra offset fastest for the quartets whose bra elements are
+// Z's rows measured slower on XY6: 8.93 against 8.83 ms.)
+void build_quad_tables(const std::vector<std::vector<int32_t>>& sigs,
+                       const std::vector<uint8_t>& used, std::vector<int64_t>& comboff,
+                       std::vector<uint64_t>& qtab) {
+    const int S = (int)sigs.size();
+    comboff.assign((size_t)S * S * S * S, 0);
+    qtab.clear();
+    for (int si = 0; si < S; ++si)
+        for (int sj = 0; sj < S; ++sj)
+            for (int sk = 0; sk < S; ++sk)
+                for (int sr = 0; sr < S; ++sr) {
+                    size_t c = ((size_t)(si * S + sj) * S + sk) * S + sr;
+                    if (!used[c]) continue;
+                    comboff[c] = (int64_t)qtab.size();
+                    const std::vector<int32_t>* sg[4] = {&sigs[si], &sigs[sj], &sigs[sk],
+                                                         &sigs[sr]};
+                    int cnt4[4], L4[4];
+                    for (int k = 0; k < 4; ++k) {
+                        cnt4[k] = (int)sg[k]->size();
+                        int v = (*sg[k])[0];
+                        L4[k] = (v >> 8) + ((v >> 4) & 15) + (v & 15);
+                    }
+                    auto entry = [&](int a, int bb, int cc, int d) {
+                        int e4[4] = {(*sg[0])[a], (*sg[1])[bb], (*sg[2])[cc], (*sg[3])[d]};
+                        uint64_t t3[3];
+                        for (int ax = 0; ax < 3; ++ax) {
+                            int sh = 8 - 4 * ax, t = 0;
+                            for (int k = 0; k < 4; ++k)
+                                t = t * (L4[k] + 1) + ((e4[k] >> sh) & 15);
+                            t3[ax] = (uint64_t)t;
+                        }
+                        return (uint64_t)(a * cnt4[1] + bb) |
+                               ((uint64_t)(cc * cnt4[3] + d) << 8) | (t3[0] << 16) |
+                               (t3[1] << 26) | (t3[2] << 36) | ((uint64_t)a << 46) |
+                               ((uint64_t)bb << 50) | ((uint64_t)cc << 54) |
+                               ((uint64_t)d << 58);
+                    };
+                    for (int a = 0; a < cnt4[0]; ++a)
+                        for (int bb = 0; bb < cnt4[1]; ++bb)
+                            for (int cc = 0; cc < cnt4[2]; ++cc)
+                                for (int d = 0; d < cnt4[3]; ++d)
+                                    qtab.push_back(entry(a, bb, cc, d));
+                }
+}
+
 int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     std::vector<GroupInfo> groups;
     std::vector<std::vector<int32_t>> sigs;
@@ -291,6 +369,24 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     pl->cutoffmd = b->cutoffmd;
     pl->cutoffcentre = b->cutoffcentre;
     pl->device = o->device;
+    int ng = b->ng;
+    pl->npair = ng * (ng + 1) / 2;
+    std::vector<PairInfo> pairs;
+    std::vector<int32_t> xa, xb;
+    pl->npp = build_pairs(groups, pairs, xa, xb);
+    if (pl->npp > 0x7fffffff) return THISTLE_EINVAL;
+    std::vector<char4> lx(b->ngto);
+    for (int j = 0; j < b->ngto; ++j)
+        lx[j] = make_char4((char)b->l[j], (char)b->m[j], (char)b->n[j],
+                           (char)(b->l[j] + b->m[j] + b->n[j]));
+    int64_t Q = 0;
+    std::vector<EnumRow> rows = enum_rows(ng, Q, pl->type == 8);
+    pl->nquart = Q;
+    pl->nlisted = Q;
+    pl->nrows = (int)rows.size();
+    if (Q > 0x7fffffff) return THISTLE_EINVAL;
+
+    // ---- device from here on
     HCHECK(hipSetDevice(o->device));
     if (o->stream) {
         pl->stream = (hipStream_t)o->stream;
@@ -308,124 +404,77 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
         HCHECK(hipEventCreateWithFlags(&pl->join_ev, hipEventDisableTiming));
         for (auto& e : pl->cev) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    int ng = b->ng;
-    pl->npair = ng * (ng + 1) / 2;
-    std::vector<PairInfo> pairs(pl->npair);
-    std::vector<int32_t> xa, xb;
-    int64_t off = 0;
-    for (int gi = 0; gi < ng; ++gi)
-        for (int gj = gi; gj < ng; ++gj) {
-            PairInfo& p = pairs[pair_index(gi, gj, ng)];
-            std::memset(&p, 0, sizeof p);
-            p.gi = gi;
-            p.gj = gj;
-            p.off = (int32_t)off;
-            p.nab = groups[gi].count * groups[gj].count;
-            for (int a = 0; a < groups[gi].count; ++a)
-                for (int c = 0; c < groups[gj].count; ++c) {
-                    xa.push_back(groups[gi].start + a);
-                    xb.push_back(groups[gj].start + c);
-                }
-            off += p.nab;
-        }
-    if (off > 0x7fffffff) return THISTLE_EINVAL;
-    pl->npp = off;
-    std::vector<char4> lx(b->ngto);
-    for (int j = 0; j < b->ngto; ++j)
-        lx[j] = make_char4((char)b->l[j], (char)b->m[j], (char)b->n[j],
-                           (char)(b->l[j] + b->m[j] + b->n[j]));
-    int64_t Q = 0;
-    std::vector<EnumRow> rows = enum_rows(ng, Q, pl->type == 8);
-    pl->nquart = Q;
-    pl->nlisted = Q;
-    pl->nrows = (int)rows.size();
-    // positions of each LL = L_i+L_j+L_k+L_r class, in reference order
-    std::vector<int32_t> cls(Q);
-    std::vector<char> used((size_t)S * S * S * S, 0);
+    if ((rc = dalloc(pl->rows, sizeof(EnumRow) * rows.size()))) return rc;
+    if ((rc = dalloc(pl->cls, sizeof(int32_t) * Q))) return rc;
+    HCHECK(hipMemcpyAsync(pl->rows.p, rows.data(), sizeof(EnumRow) * rows.size(),
+                          hipMemcpyHostToDevice, pl->stream));
+    // positions of each LL = L_i+L_j+L_k+L_r class in reference order: a
+    // class key per quartet on the GPU, then a stable radix sort by key
+    std::vector<uint8_t> used((size_t)S * S * S * S, 0);
     {
-        std::vector<int64_t> cnt(18, 0);
-        auto LLof = [&](const EnumRow& R, int x) {
-            int g[4];
-            row_quartet(R, x, g);
-            const int gi = g[0], gj = g[1], gk = g[2], gr = g[3];
-            int Ls[4] = {groups[gi].L, groups[gj].L, groups[gk].L, groups[gr].L};
-            used[((groups[gi].pad * S + groups[gj].pad) * S + groups[gk].pad) * S +
-                 groups[gr].pad] = 1;
-            int T = (Ls[0] + 1) * (Ls[1] + 1) * (Ls[2] + 1) * (Ls[3] + 1);
-            int LL = Ls[0] + Ls[1] + Ls[2] + Ls[3];
-            pl->cls_tmax[LL] = std::max(pl->cls_tmax[LL], T);
-            return LL;
-        };
-        for (const auto& R : rows)
-            for (int u = 0; u < R.count; ++u) cnt[LLof(R, R.r0 + u)]++;
+        std::vector<int8_t> gL8(ng), gs8(ng);
+        for (int g = 0; g < ng; ++g) {
+            gL8[g] = (int8_t)groups[g].L;
+            gs8[g] = (int8_t)groups[g].pad;
+        }
+        DevBuf dgL, dgs, key, key2, pos, cnt, tmx, dused, tmp;
+        if ((rc = dalloc(dgL, ng)) || (rc = dalloc(dgs, ng)) ||
+            (rc = dalloc(key, sizeof(uint32_t) * Q)) || (rc = dalloc(key2, sizeof(uint32_t) * Q)) ||
+            (rc = dalloc(pos, sizeof(int32_t) * Q)) || (rc = dalloc(cnt, 18 * 8)) ||
+            (rc = dalloc(tmx, 18 * 4)) || (rc = dalloc(dused, used.size())))
+            return rc;
+        HCHECK(hipMemcpyAsync(dgL.p, gL8.data(), ng, hipMemcpyHostToDevice, pl->stream));
+        HCHECK(hipMemcpyAsync(dgs.p, gs8.data(), ng, hipMemcpyHostToDevice, pl->stream));
+        HCHECK(hipMemsetAsync(cnt.p, 0, 18 * 8, pl->stream));
+        HCHECK(hipMemsetAsync(tmx.p, 0, 18 * 4, pl->stream));
+        HCHECK(hipMemsetAsync(dused.p, 0, used.size(), pl->stream));
+        HCHECK(launch_classify(pl->rows.as<EnumRow>(), pl->nrows, dgL.as<int8_t>(),
+                               dgs.as<int8_t>(), S, key.as<uint32_t>(), pos.as<int32_t>(),
+                               cnt.as<unsigned long long>(), tmx.as<int>(), dused.as<uint8_t>(),
+                               pl->stream));
+        size_t tb = 0;
+        HCHECK(class_sort(nullptr, tb, key.as<uint32_t>(), key2.as<uint32_t>(), pos.as<int32_t>(),
+                          pl->cls.as<int32_t>(), Q, pl->stream));
+        if ((rc = dalloc(tmp, std::max<size_t>(tb, 16)))) return rc;
+        HCHECK(class_sort(tmp.p, tb, key.as<uint32_t>(), key2.as<uint32_t>(), pos.as<int32_t>(),
+                          pl->cls.as<int32_t>(), Q, pl->stream));
+        unsigned long long hc[18];
+        int ht[18];
+        HCHECK(hipMemcpyAsync(hc, cnt.p, sizeof hc, hipMemcpyDeviceToHost, pl->stream));
+        HCHECK(hipMemcpyAsync(ht, tmx.p, sizeof ht, hipMemcpyDeviceToHost, pl->stream));
+        HCHECK(hipMemcpyAsync(used.data(), dused.p, used.size(), hipMemcpyDeviceToHost,
+                              pl->stream));
+        HCHECK(hipStreamSynchronize(pl->stream));
+        for (int k = 0; k < 17; ++k) pl->cls_tmax[k] = ht[k];
+        pl->cls_off[0] = 0;
+        pl->cls_hoff[0] = 0;
+        // records of class LL: DO = LL + 1 series coefficients (LL + 3 for
+        // j2) plus the per-quartet words, HS_ROW(DO) doubles each
+        const int dj = (pl->type == 7 || pl->type == 8) ? 3 : 1;
+        for (int k = 0; k < 17; ++k) {
+            pl->cls_off[k + 1] = pl->cls_off[k] + (int64_t)hc[k];
+            pl->cls_hoff[k + 1] = pl->cls_hoff[k] + (int64_t)hc[k] * HS_ROW(k + dj);
+        }
+        if (pl->cls_off[17] != Q) return THISTLE_EDEVICE;
+    }
+    {
         // the contraction kernel keeps per-exponent-tuple tables in LDS
         hipDeviceProp_t prop;
         HCHECK(hipGetDeviceProperties(&prop, o->device));
         size_t need = 0;
         for (int LL = 0; LL <= 16; ++LL)
-            if (cnt[LL] > 0) need = std::max(need, contract_lds_bytes(LL, pl->cls_tmax[LL]));
+            if (pl->cls_off[LL + 1] > pl->cls_off[LL])
+                need = std::max(need, contract_lds_bytes(LL, pl->cls_tmax[LL]));
         if (need > prop.sharedMemPerBlock) {
             std::fprintf(stderr, "thistle: quartet tables need %zu B of LDS (device: %zu B)\n",
                          need, (size_t)prop.sharedMemPerBlock);
             return THISTLE_ELMAX;
         }
         if (need > 65536) HCHECK(set_contract_lds_limit(need));
-        pl->cls_off[0] = 0;
-        for (int k = 0; k < 17; ++k) pl->cls_off[k + 1] = pl->cls_off[k] + cnt[k];
-        // records of class LL: DO = LL + 1 series coefficients (LL + 3 for
-        // j2) plus the per-quartet words, HS_ROW(DO) doubles each
-        const int dj = (pl->type == 7 || pl->type == 8) ? 3 : 1;
-        for (int k = 0; k < 17; ++k)
-            pl->cls_hoff[k + 1] = pl->cls_hoff[k] + cnt[k] * HS_ROW(k + dj);
-        std::vector<int64_t> fill(pl->cls_off, pl->cls_off + 17);
-        for (const auto& R : rows)
-            for (int u = 0; u < R.count; ++u) cls[fill[LLof(R, R.r0 + u)]++] = (int32_t)(R.out + u);
     }
-    // GTO-quadruple tables of the signature combinations in use.  Entry u of
-    // a combination (d fastest): bits 0-7 a*cj+b, 8-15 c*cr+d (offsets in the
-    // bra / ket pair blocks), 16-25 / 26-35 / 36-45 the x / y / z exponent
-    // tuple index, 46-61 a, b, c, d (4 bits each).  The ket offset runs
-    // fastest, so that a wave's Z reads fall on few cache lines.  (A copy
-    // with the bra offset fastest for the quartets whose bra elements are
-    // Z's rows measured slower on XY6: 8.93 against 8.83 ms.)
-    std::vector<int64_t> comboff((size_t)S * S * S * S, 0);
+    std::vector<int64_t> comboff;
     std::vector<uint64_t> qtab;
-    for (int si = 0; si < S; ++si)
-        for (int sj = 0; sj < S; ++sj)
-            for (int sk = 0; sk < S; ++sk)
-                for (int sr = 0; sr < S; ++sr) {
-                    size_t c = ((size_t)(si * S + sj) * S + sk) * S + sr;
-                    if (!used[c]) continue;
-                    comboff[c] = (int64_t)qtab.size();
-                    const std::vector<int32_t>* sg[4] = {&sigs[si], &sigs[sj], &sigs[sk],
-                                                         &sigs[sr]};
-                    int cnt4[4], L4[4];
-                    for (int k = 0; k < 4; ++k) {
-                        cnt4[k] = (int)sg[k]->size();
-                        int v = (*sg[k])[0];
-                        L4[k] = (v >> 8) + ((v >> 4) & 15) + (v & 15);
-                    }
-                    auto entry = [&](int a, int bb, int cc, int d) {
-                        int e4[4] = {(*sg[0])[a], (*sg[1])[bb], (*sg[2])[cc], (*sg[3])[d]};
-                        uint64_t t3[3];
-                        for (int ax = 0; ax < 3; ++ax) {
-                            int sh = 8 - 4 * ax, t = 0;
-                            for (int k = 0; k < 4; ++k)
-                                t = t * (L4[k] + 1) + ((e4[k] >> sh) & 15);
-                            t3[ax] = (uint64_t)t;
-                        }
-                        return (uint64_t)(a * cnt4[1] + bb) |
-                               ((uint64_t)(cc * cnt4[3] + d) << 8) | (t3[0] << 16) |
-                               (t3[1] << 26) | (t3[2] << 36) | ((uint64_t)a << 46) |
-                               ((uint64_t)bb << 50) | ((uint64_t)cc << 54) |
-                               ((uint64_t)d << 58);
-                    };
-                    for (int a = 0; a < cnt4[0]; ++a)
-                        for (int bb = 0; bb < cnt4[1]; ++bb)
-                            for (int cc = 0; cc < cnt4[2]; ++cc)
-                                for (int d = 0; d < cnt4[3]; ++d)
-                                    qtab.push_back(entry(a, bb, cc, d));
-                }
+    build_quad_tables(sigs, used, comboff, qtab);
     pl->nsig = S;
     int L1 = maxl + 1, T1 = 2 * maxl + 1;
     if ((rc = dalloc(pl->G, sizeof(GroupInfo) * ng))) return rc;
@@ -434,7 +483,6 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if ((rc = dalloc(pl->e12, sizeof(double) * (size_t)pl->npair * b->nq))) return rc;
     if ((rc = dalloc(pl->q, sizeof(double) * b->nq))) return rc;
     if ((rc = dalloc(pl->lxyz, sizeof(char4) * b->ngto))) return rc;
-    if ((rc = dalloc(pl->rows, sizeof(EnumRow) * rows.size()))) return rc;
     if ((rc = dalloc(pl->qlist, sizeof(uint64_t) * Q))) return rc;
     if ((rc = dalloc(pl->flags, Q))) return rc;
     int64_t nb = (Q + 2047) / 2048 + 1;
@@ -443,7 +491,6 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if ((rc = dalloc(pl->count, sizeof(int64_t) * 2))) return rc;
     if ((rc = dalloc(pl->xa, sizeof(int32_t) * pl->npp))) return rc;
     if ((rc = dalloc(pl->xb, sizeof(int32_t) * pl->npp))) return rc;
-    if ((rc = dalloc(pl->cls, sizeof(int32_t) * Q))) return rc;
     if ((rc = dalloc(pl->hs, sizeof(double) * (size_t)std::max<int64_t>(pl->cls_hoff[17], 1))))
         return rc;
     if ((rc = dalloc(pl->quadtab, sizeof(uint64_t) * qtab.size()))) return rc;
@@ -452,8 +499,6 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
                           hipMemcpyHostToDevice, pl->stream));
     HCHECK(hipMemcpyAsync(pl->comboff.p, comboff.data(), sizeof(int64_t) * comboff.size(),
                           hipMemcpyHostToDevice, pl->stream));
-    HCHECK(hipMemcpyAsync(pl->cls.p, cls.data(), sizeof(int32_t) * Q, hipMemcpyHostToDevice,
-                          pl->stream));
     HCHECK(hipStreamSynchronize(pl->stream));
     pl->h_q.assign(b->q, b->q + b->nq);
     // a uniform grid from q[1] on (q[0] may be set apart, as 1e-10 is by the
@@ -476,7 +521,6 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     HCHECK(hipMemcpyAsync(pl->P.p, pairs.data(), sizeof(PairInfo) * pl->npair, hipMemcpyHostToDevice, pl->stream));
     HCHECK(hipMemcpyAsync(pl->q.p, b->q, sizeof(double) * b->nq, hipMemcpyHostToDevice, pl->stream));
     HCHECK(hipMemcpyAsync(pl->lxyz.p, lx.data(), sizeof(char4) * b->ngto, hipMemcpyHostToDevice, pl->stream));
-    HCHECK(hipMemcpyAsync(pl->rows.p, rows.data(), sizeof(EnumRow) * rows.size(), hipMemcpyHostToDevice, pl->stream));
     HCHECK(hipMemcpyAsync(pl->xa.p, xa.data(), sizeof(int32_t) * pl->npp, hipMemcpyHostToDevice, pl->stream));
     HCHECK(hipMemcpyAsync(pl->xb.p, xb.data(), sizeof(int32_t) * pl->npp, hipMemcpyHostToDevice, pl->stream));
     int waves = o->waves;
@@ -508,6 +552,8 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
         HCHECK(hipMemcpyAsync(pl->mmod.p, b->mmod, sizeof(double) * (size_t)b->nmo * b->ngto,
                               hipMemcpyHostToDevice, pl->stream));
     }
+    // the host tables above go out of scope here
+    HCHECK(hipStreamSynchronize(pl->stream));
     return THISTLE_OK;
 }
 

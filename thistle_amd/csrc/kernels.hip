@@ -165,6 +165,43 @@ __global__ void k_enumerate(const EnumRow* __restrict__ rows,
     }
 }
 
+// LL class of every quartet position (one 64-thread block per enumeration
+// row, as k_enumerate): key[e] = L_i + L_j + L_k + L_r and pos[e] = e for a
+// stable sort by class; per block, the class histogram and the largest
+// exponent-tuple count prod (L+1) per class (merged with one atomic per
+// class); the shell-signature combinations that occur.
+__global__ void k_classify(const EnumRow* __restrict__ rows, const int8_t* __restrict__ gL,
+                           const int8_t* __restrict__ gsig, int S, uint32_t* __restrict__ key,
+                           int32_t* __restrict__ pos, unsigned long long* __restrict__ cnt,
+                           int* __restrict__ tmax, uint8_t* __restrict__ used) {
+    __shared__ unsigned int hc[18];
+    __shared__ int ht[18];
+    if (threadIdx.x < 18) {
+        hc[threadIdx.x] = 0;
+        ht[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const EnumRow R = rows[blockIdx.x];
+    for (int u = threadIdx.x; u < R.count; u += blockDim.x) {
+        int g[4];
+        row_quartet(R, R.r0 + u, g);
+        const int L0 = gL[g[0]], L1 = gL[g[1]], L2 = gL[g[2]], L3 = gL[g[3]];
+        const int LL = L0 + L1 + L2 + L3;
+        const int T = (L0 + 1) * (L1 + 1) * (L2 + 1) * (L3 + 1);
+        const int64_t e = R.out + u;
+        key[e] = (uint32_t)LL;
+        pos[e] = (int32_t)e;
+        atomicAdd(&hc[LL], 1u);
+        atomicMax(&ht[LL], T);
+        used[((gsig[g[0]] * S + gsig[g[1]]) * S + gsig[g[2]]) * S + gsig[g[3]]] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 18 && hc[threadIdx.x]) {
+        atomicAdd(&cnt[threadIdx.x], (unsigned long long)hc[threadIdx.x]);
+        atomicMax(&tmax[threadIdx.x], ht[threadIdx.x]);
+    }
+}
+
 __global__ void k_unpack_quartets(const uint64_t* __restrict__ in, int64_t n,
                                   int32_t* __restrict__ out) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -829,6 +866,15 @@ hipError_t launch_enumerate(const EnumRow* rows, int nrows, const PairInfo* P, i
                             double cutoffcentre, uint64_t* out, hipStream_t s) {
     if (nrows <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_enumerate, dim3(nrows), dim3(64), 0, s, rows, P, ng, cutoffcentre, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_classify(const EnumRow* rows, int nrows, const int8_t* gL, const int8_t* gsig,
+                           int S, uint32_t* key, int32_t* pos, unsigned long long* cnt, int* tmax,
+                           uint8_t* used, hipStream_t s) {
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_classify, dim3(nrows), dim3(64), 0, s, rows, gL, gsig, S, key, pos, cnt,
+                       tmax, used);
     return hipGetLastError();
 }
 

@@ -536,6 +536,11 @@ hipError_t rdm_sort(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* ko
     return hipcub::DeviceRadixSort::SortPairs(temp, bytes, kin, kout, vin, vout, n, 0, keybits, s);
 }
 
+hipError_t class_sort(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* kout,
+                      const int32_t* vin, int32_t* vout, int64_t n, hipStream_t s) {
+    return hipcub::DeviceRadixSort::SortPairs(temp, bytes, kin, kout, vin, vout, n, 0, 5, s);
+}
+
 hipError_t rdm_esum(const uint32_t* keys, const double* vals, int64_t n, int64_t nel,
                     int64_t* start, double* tw, hipStream_t s) {
     hipLaunchKernelGGL(k_rdm_bounds, dim3(nb64(nel + 1, 256)), dim3(256), 0, s, keys, n, nel, start);

@@ -205,6 +205,14 @@ hipError_t launch_enumerate(const EnumRow* rows, int nrows, const PairInfo* P,
                             hipStream_t s);
 hipError_t launch_unpack_quartets(const uint64_t* in, int64_t n, int32_t* out,
                                   hipStream_t s);
+// LL class key (and position) of every quartet, the class counts, the largest
+// exponent-tuple count per class and the signature combinations in use
+hipError_t launch_classify(const EnumRow* rows, int nrows, const int8_t* gL, const int8_t* gsig,
+                           int S, uint32_t* key, int32_t* pos, unsigned long long* cnt, int* tmax,
+                           uint8_t* used, hipStream_t s);
+// stable radix sort of (class key, position) pairs on the 5 key bits
+hipError_t class_sort(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* kout,
+                      const int32_t* vin, int32_t* vout, int64_t n, hipStream_t s);
 // k_contract<LL>: h(beta) of one LL class (series.hip)
 size_t contract_lds_bytes(int LL, int Tmax);
 hipError_t set_contract_lds_limit(size_t bytes);
