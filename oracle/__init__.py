@@ -69,9 +69,7 @@ def _declare(L):
                                       _p, _p, _i, _d, _d, _d, _p]
     L.orc_nuclei_electron.restype = _i
     L.orc_nuclear_nn.argtypes = [_i, _p, _p, _p, _i, _p]
-    L.orc_quartet_series.argtypes = [_p, _p, _p, _p, _p, _i, _i, _p, _p, _p,
-                                     _i, _i, _i, _i, _d, _d, _d, _d, _p, _d,
-                                     _d, _p, _i, _i, _p, _p]
+    L.orc_build_z12.argtypes = [_p, _i, _i, _p, _p, _i, _p, _p]
     L.orc_dphrec.argtypes = [_p, _p, _i, _i, _i]
     L.orc_bessels0rr.argtypes = [_p, _i, _p, _i, _d, _p]
     L.orc_unique_total.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
@@ -328,6 +326,21 @@ def quartet_list_elastic_j2(px, py, pz, cutoffcentre):
     out = np.zeros((Q, 6), dtype=np.int32)
     lib().orc_quartet_list_elastic_j2(ng, _ptr(px), _ptr(py), _ptr(pz), cutoffcentre, _ptr(out))
     return out
+
+
+def build_z12(mmod, mat, total, ngto):
+    """z11, z22 of variables_total, each (nmat, ngto, ngto) (column-major
+    z(k,a,b) indexing, returned as numpy arrays indexed [k, a, b])."""
+    mmod = _D(mmod, "F")
+    mat = _I(mat, "F")
+    total = _D(total)
+    nmat = len(total)
+    z1 = np.zeros(nmat * ngto * ngto)
+    z2 = np.zeros(nmat * ngto * ngto)
+    lib().orc_build_z12(_ptr(mmod), mmod.shape[0], ngto, _ptr(mat), _ptr(total), nmat,
+                        _ptr(z1), _ptr(z2))
+    sh = (nmat, ngto, ngto)
+    return z1.reshape(sh, order="F"), z2.reshape(sh, order="F")
 
 
 def dphrec(x, lmax1, leps1=100):

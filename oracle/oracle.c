@@ -1019,19 +1019,6 @@ static void ijkr_pzero(const quartet_ctx *C, int gi, int gj, int gk, int gr,
     }
 }
 
-void orc_quartet_series(const int *l, const int *m, const int *n,
-                        const int *gstart, const int *gcount, int maxl, int ng,
-                        const double *ddx, const double *ddy, const double *ddz,
-                        int gi, int gj, int gk, int gr, double hx, double hy,
-                        double hz, double h, const double *ztot,
-                        double cutoffz, double cutoffmd, const double *q,
-                        int nq, int elastic, double *h_saved, double *F)
-{
-    quartet_ctx C = {l, m, n, gstart, gcount, maxl, ng, ddx, ddy, ddz};
-    k_ijkr(&C, gi, gj, gk, gr, hx, hy, hz, h, ztot, cutoffz, cutoffmd, q, nq,
-           elastic, h_saved, F);
-}
-
 /* ---------------------------------------------------------------- drivers */
 typedef struct {
     int ngto, ng, maxl;
@@ -1128,6 +1115,12 @@ static void build_z12(const double *mmod, int nmo, int ngto, const int *mat,
             }
         }
 #undef MM
+}
+
+void orc_build_z12(const double *mmod, int nmo, int ngto, const int *mat,
+                   const double *total, int nmat, double *z1, double *z2)
+{
+    build_z12(mmod, nmo, ngto, mat, total, nmat, z1, z2);
 }
 
 static int total_j0_impl(int ngto, int ng, int maxl, const int *l, const int *m,

@@ -147,16 +147,12 @@ int orc_elastic_j2(int ngto, int ng, int maxl, const int *l, const int *m,
                    double cutoffmd, double cutoffcentre, int nthreads,
                    double *tsi);
 
-/* Per-quartet pieces for targeted parity checks. */
-/* tot_integral_k_ijkr with an explicit ztot block ztot[((a*sj+b)*sk+c)*sr+d]
- * (a in gi, b in gj, c in gk, d in gr): writes h_saved[20] and F[nq]. */
-void orc_quartet_series(const int *l, const int *m, const int *n,
-                        const int *gstart, const int *gcount, int maxl, int ng,
-                        const double *ddx, const double *ddy, const double *ddz,
-                        int gi, int gj, int gk, int gr, double hx, double hy,
-                        double hz, double h, const double *ztot,
-                        double cutoffz, double cutoffmd, const double *q,
-                        int nq, int elastic, double *h_saved, double *F);
+/* z11/z22 of variables_total (src/Variables.f90:1041-1061) for 2-RDM rows
+ * mat(nmat,4) with values total: z1(k,a,b) = total_k (C_{p a} C_{s b} +
+ * C_{p b} C_{s a}), z2(k,a,b) = C_{q a} C_{r b} + C_{q b} C_{r a}, (p,s) and
+ * (q,r) each in increasing order; both (nmat, ngto, ngto) column-major. */
+void orc_build_z12(const double *mmod, int nmo, int ngto, const int *mat,
+                   const double *total, int nmat, double *z1, double *z2);
 
 /* dphrec, src/Integrals.f90:7147-7184: rbes(leps1,nq) col-major. */
 void orc_dphrec(const double *x, double *rbes, int leps1, int lmax1, int nq);

@@ -145,6 +145,53 @@ def save(name, inp, q, ref, cutoffs=(1e-15, 1e-20, 0.001)):
 
 
 REF_SRC = "/root/reference/src"
+REF_VARS = os.path.join(ROOT, "oracle", "_ref", "ref_vars")
+VAR_CASES = ["spd2", "f2", "h2"]
+
+
+def run_vars(name):
+    """The reference's own variables_total (oracle/_ref/ref_vars, built from
+    a copy of src/Variables.f90 with its one non-compiling token read as
+    .eqv.) on a case's basis, orbitals and 2-RDM rows."""
+    w, inp = case_inputs(name)
+    q = w.q
+    mat, tot = oracle.createtwordm(inp.confs, inp.civs)
+    ngto, ng, nq = inp.ngto, inp.ng, len(q)
+    maxl = inp.maxl
+    nmo, nmat = inp.mmod.shape[0], len(tot)
+    with tempfile.TemporaryDirectory() as d:
+        rin, rout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(rin, "wb") as f:
+            f.write(np.array([ngto, ng, nq, maxl, nmo, nmat], dtype=np.int32).tobytes())
+            for a in (inp.l, inp.m, inp.n, inp.group):
+                f.write(np.asarray(a, dtype=np.int32).tobytes())
+            for a in (inp.ga, inp.xx, inp.yy, inp.zz):
+                f.write(np.asarray(a, dtype=np.float64).tobytes())
+            f.write(np.asfortranarray(inp.mmod).tobytes(order="F"))
+            f.write(np.asfortranarray(mat.astype(np.int32)).tobytes(order="F"))
+            f.write(np.asarray(tot, dtype=np.float64).tobytes())
+            f.write(np.asarray(q, dtype=np.float64).tobytes())
+        subprocess.run([REF_VARS, rin, rout], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        T1, L1 = 2 * maxl + 1, maxl + 1
+        with open(rout, "rb") as f:
+            rd = lambda n: np.frombuffer(f.read(8 * n), dtype=np.float64)  # noqa: E731
+            P = [rd(ng * ng).reshape(ng, ng, order="F") for _ in range(3)]
+            dd = [rd(T1 * L1 * L1 * ng * ng).reshape(T1, L1, L1, ng, ng, order="F")
+                  for _ in range(3)]
+            e12 = rd(nq * ng * ng).reshape(nq, ng, ng, order="F")
+            z11 = rd(nmat * ngto * ngto).reshape(nmat, ngto, ngto, order="F")
+            z22 = rd(nmat * ngto * ngto).reshape(nmat, ngto, ngto, order="F")
+    return dict(px=P[0], py=P[1], pz=P[2], ddx=dd[0], ddy=dd[1], ddz=dd[2], e12=e12,
+                z11=z11, z22=z22)
+
+
+def save_vars(name):
+    ref = run_vars(name)
+    path = os.path.join(HERE, f"vars_{name}.npz")
+    np.savez_compressed(path, **ref)
+    print(f"vars_{name}: " + ", ".join(f"{k}{tuple(v.shape)}" for k, v in ref.items())
+          + f" size={os.path.getsize(path)}B", flush=True)
 
 
 def save_reference_rdm_files():
@@ -167,6 +214,10 @@ def save_reference_rdm_files():
 
 
 def main(names):
+    if "vars" in names:
+        for v in VAR_CASES:
+            save_vars(v)
+        names = [n for n in names if n != "vars"]
     if "ref_rdm_files" in names:
         save_reference_rdm_files()
         names = [n for n in names if n != "ref_rdm_files"]

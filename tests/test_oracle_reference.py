@@ -139,3 +139,34 @@ def test_forward_limits(case):
     assert g["q"][0] <= 1e-10
     np.testing.assert_allclose(g["S_total"][0], N * (N - 1) * f, rtol=1e-9)
     np.testing.assert_allclose(g["S_elastic"][0], N * N * f, rtol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["spd2", "f2", "h2"])
+def test_variables_total_tables(golden, name):
+    """fill_md_table's repacked Hermite tables, the pair centres, e12 (with
+    the single-precision pi) and the z11/z22 2-RDM factors of the reference's
+    own variables_total (src/Variables.f90:948-1153, run through
+    oracle/ref_vars; fixtures vars_*.npz) against the oracle's pair tables and
+    z arrays."""
+    g = golden(name)
+    v = golden(f"vars_{name}")
+    l, m, n, group = g["l"], g["m"], g["n"], g["group"]
+    dd, P, e12 = oracle.pair_tables(l, m, n, group, g["ga"], g["xx"], g["yy"], g["zz"],
+                                    g["q"])
+    for k, a in enumerate(("px", "py", "pz")):
+        np.testing.assert_allclose(P[k], v[a], rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(e12, np.transpose(v["e12"], (1, 2, 0)), rtol=1e-14, atol=0)
+    # Ddx(t, l_ii+1, l_jj+1, group(jj), group(ii)) is set for the exponents
+    # that occur (src/Variables.f90:1132-1150); compare exactly those
+    for ax, (key, ex) in enumerate((("ddx", l), ("ddy", m), ("ddz", n))):
+        ref = np.transpose(v[key], (4, 3, 1, 2, 0))          # [gi, gj, la, lb, t]
+        seen = np.zeros(ref.shape, dtype=bool)
+        for ii in range(l.size):
+            for jj in range(l.size):
+                gi, gj = group[ii] - 1, group[jj] - 1
+                seen[gi, gj, ex[ii], ex[jj], :ex[ii] + ex[jj] + 1] = True
+        np.testing.assert_allclose(dd[ax][seen], ref[seen], rtol=1e-13, atol=1e-15)
+    mat, tot = oracle.createtwordm(g["confs"], g["civs"])
+    z1, z2 = oracle.build_z12(g["mmod"], mat, tot, l.size)
+    np.testing.assert_array_equal(z1, v["z11"])
+    np.testing.assert_array_equal(z2, v["z22"])
