@@ -83,10 +83,16 @@ typedef struct ThistleBasis {
                                      THISTLE_CUTOFFMD_MAX                  */
 } ThistleBasis;
 
+/* The 2-RDM builders select |Gamma| >= cutoff with cutoff = 1E-30, a
+ * default-real literal (src/twordms.f90:514, src/one_rdm.f90:1312): the
+ * double nearest the float nearest 1e-30. */
+#define THISTLE_RDM_CUTOFF 1.0000000031710769e-30
+
 typedef struct ThistleOpts {
     int32_t precision;            /* THISTLE_FP64 or THISTLE_MIXED        */
     double rdm_cutoff;            /* |Gamma| selection threshold of
-                                     createtwordm; the reference uses 1e-30 */
+                                     createtwordm; <= 0 selects
+                                     THISTLE_RDM_CUTOFF */
     void *stream;                 /* hipStream_t, or NULL for the library's */
     int32_t device;               /* HIP device ordinal                    */
     int32_t waves;                /* persistent waves for the quartet
@@ -187,6 +193,57 @@ int32_t thistle_onerdm(const int32_t *confs, int32_t nconf, int32_t lconfs,
 int32_t thistle_createonerdm(const int32_t *confs, int32_t nconf, int32_t lconfs,
                              const double *civecs, int32_t nstates, int32_t state1,
                              int32_t state2, const ThistleOpts *opts, double *onerdm);
+
+/* ---- bit-string determinants (the file route) --------------------------
+ * A determinant is two non-negative 64-bit bit strings, alpha and beta,
+ * orbital k on bit k-1 (the reference's integers of the "i c1 c2 alpha
+ * beta" files; its 32-bit reads hold orbitals 1-31).  Every determinant must
+ * carry the same alpha count and the same beta count (THISTLE_EINVAL
+ * otherwise: the reference's excitation bookkeeping assumes it).  These
+ * entries build on the GPU with the reference's floating-point results. */
+
+/* mcci_to_bit (src/one_rdm.f90:1901-1984): c*_out = sign * c*, the sign of
+ * reordering the determinant's slots from orbital order (alpha before beta
+ * on each orbital) to alpha-then-beta order.  text_round != 0 also applies
+ * the E20.10 write and list-directed read that hand the coefficients to
+ * createtwordm_bit (10 significant digits).  Needs equal alpha and beta
+ * counts. */
+int32_t thistle_mcci_to_bit(const int64_t *alpha, const int64_t *beta,
+                            const double *c1, const double *c2, int64_t n,
+                            int32_t text_round, const ThistleOpts *opts,
+                            double *c1_out, double *c2_out);
+/* createtwordm_bit (src/one_rdm.f90:931-1353): rows (p,s,q,r) and values
+ * SpinFree2RDM(p,q,r,s) with |value| >= cutoff (<= 0: THISTLE_RDM_CUTOFF,
+ * the reference's), p,q,r,s loop order, nbft = highest occupied orbital;
+ * each element sums its contributions in the reference's order (the
+ * no-difference pass over all determinants, then ordered pairs).  mat is
+ * (cap,4) column-major.  Returns the row count or -status; rows are written
+ * only when cap >= count (cap = 0 counts). */
+int64_t thistle_createtwordm_bit(const int64_t *alpha, const int64_t *beta,
+                                 const double *c1, const double *c2, int64_t n,
+                                 double cutoff, const ThistleOpts *opts,
+                                 int32_t *mat, double *total, int64_t cap);
+/* one_rdm_bit (src/one_rdm.f90:1388-1537): onerdm (maxnmo x maxnmo,
+ * column-major), maxnmo = highest occupied orbital, written to *norb_out.
+ * civs is the file's third column (the reference reads both coefficient
+ * columns into one array, the third last).  onerdm = NULL only reports
+ * maxnmo; maxnmo > cap_norb returns THISTLE_EINVAL. */
+int32_t thistle_one_rdm_bit(const int64_t *alpha, const int64_t *beta,
+                            const double *civs, int64_t n, const ThistleOpts *opts,
+                            int32_t cap_norb, int32_t *norb_out, double *onerdm);
+/* total_scattering_calculation_2 (src/main_calculation.f90:501-849) on the
+ * bit-string route (readrdm false, fci false): type 1 total j0 and 7 total
+ * j2 from createtwordm_bit, 2 elastic j0 and 8 elastic j2 from one_rdm_bit.
+ * As in the reference, types 1 and 8 read the mcci_to_bit conversion, type
+ * 7 reads it only when mcci != 0 and type 2 reads the original
+ * coefficients; types 1, 7 and 8 need equal alpha and beta counts.
+ * q_abs_out (may be NULL) is zeroed as the reference leaves it. */
+int32_t thistle_total_scattering_calculation_2(int32_t type, const ThistleBasis *basis,
+                                               int64_t n, const int64_t *alpha,
+                                               const int64_t *beta, const double *c1,
+                                               const double *c2, int32_t mcci,
+                                               const ThistleOpts *opts,
+                                               double *q_abs_out, double *result_out);
 
 /* ---- device stages, exposed for parity checks ------------------------- */
 

@@ -74,6 +74,10 @@ def _declare(L):
     L.orc_bessels0rr.argtypes = [_p, _i, _p, _i, _d, _p]
     L.orc_unique_total.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
     L.orc_unique_total.restype = _i
+    _l = C.c_int64
+    L.orc_mcci_to_bit.argtypes = [_p, _p, _p, _p, _l, _i, _p, _p]
+    L.orc_createtwordm_bit.argtypes = [_p, _p, _p, _p, _l, _p, _i]
+    L.orc_one_rdm_bit.argtypes = [_p, _p, _p, _l, _p, _i]
 
 
 def _ptr(a):
@@ -373,3 +377,50 @@ def unique_total(matrix, total):
     lib().orc_unique_total(_ptr(matrix), nrow, ncol, _ptr(total), _ptr(rm), None,
                            None, None)
     return np.ascontiguousarray(rm), rtot[:cnt].copy(), iuni[:cnt].copy(), irec
+
+
+def _L(a):
+    return np.require(np.asarray(a), dtype=np.int64, requirements=["C"])
+
+
+def mcci_to_bit(alpha, beta, c1, c2, text_round=True):
+    """mcci_to_bit (src/one_rdm.f90:1901-1984): signed (and, with
+    text_round, E20.10-rounded) coefficients."""
+    a, b, c1, c2 = _L(alpha), _L(beta), _D(c1), _D(c2)
+    o1, o2 = np.zeros(len(a)), np.zeros(len(a))
+    rc = lib().orc_mcci_to_bit(_ptr(a), _ptr(b), _ptr(c1), _ptr(c2), len(a),
+                               1 if text_round else 0, _ptr(o1), _ptr(o2))
+    if rc != 0:
+        raise ValueError("mcci_to_bit: alpha/beta electron counts differ")
+    return o1, o2
+
+
+def twordm_bit_dense(alpha, beta, c1, c2):
+    """createtwordm_bit's SpinFree2RDM (src/one_rdm.f90:931-1300)."""
+    a, b, c1, c2 = _L(alpha), _L(beta), _D(c1), _D(c2)
+    N = lib().orc_createtwordm_bit(_ptr(a), _ptr(b), _ptr(c1), _ptr(c2), len(a), None, 0)
+    if N < 0:
+        raise ValueError("createtwordm_bit: electron counts vary")
+    tw = np.zeros((N,) * 4, order="F")
+    lib().orc_createtwordm_bit(_ptr(a), _ptr(b), _ptr(c1), _ptr(c2), len(a), _ptr(tw), N)
+    return tw
+
+
+# cutoff = 1E-30 in createtwordm_bit is a default-real literal
+BIT_CUTOFF = float(np.float32(1e-30))
+
+
+def createtwordm_bit(alpha, beta, c1, c2):
+    """createtwordm_bit rows (p,s,q,r) and values (src/one_rdm.f90:931-1353)."""
+    return twordm_select(twordm_bit_dense(alpha, beta, c1, c2), BIT_CUTOFF)
+
+
+def one_rdm_bit(alpha, beta, civs):
+    """one_rdm_bit (src/one_rdm.f90:1388-1516): maxnmo x maxnmo."""
+    a, b, c = _L(alpha), _L(beta), _D(civs)
+    N = lib().orc_one_rdm_bit(_ptr(a), _ptr(b), _ptr(c), len(a), None, 0)
+    if N < 0:
+        raise ValueError("one_rdm_bit: electron counts vary")
+    g = np.zeros((N, N), order="F")
+    lib().orc_one_rdm_bit(_ptr(a), _ptr(b), _ptr(c), len(a), _ptr(g), N)
+    return g

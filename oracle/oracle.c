@@ -10,6 +10,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1666,4 +1667,279 @@ int orc_unique_total(const int *matrix, int nrow, int ncol, const double *total,
     free(first);
     free(tot);
     return cnt;
+}
+
+/* ---- bit-string determinant builders (src/one_rdm.f90) ----------------- */
+
+/* Highest orbital of the largest bit string: integer2binary_orbs of
+ * maxval(icij) (src/twordms.f90:1036-1061, used at src/one_rdm.f90:985,
+ * :1432, :1921). */
+static int bit_nbft(const int64_t *a, const int64_t *b, int64_t n)
+{
+    int64_t mx = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (a[i] > mx) mx = a[i];
+        if (b[i] > mx) mx = b[i];
+    }
+    int nb = 0;
+    while (mx >= 1) {
+        ++nb;
+        mx /= 2;
+    }
+    return nb;
+}
+
+/* Orbitals strictly between lo and hi (1-based), the reference's
+ * IAND(ibset(0,hi-1)-1, ibclr(-1,lo)+1) as 64-bit words. */
+static uint64_t bit_between(int lo, int hi)
+{
+    uint64_t below_hi = (((uint64_t)1) << (hi - 1)) - 1;
+    uint64_t from_lo = (~(((uint64_t)1) << lo)) + 1;
+    return below_hi & from_lo;
+}
+
+static int bit_popc(uint64_t x) { return __builtin_popcountll(x); }
+static int bit_ctz(uint64_t x) { return __builtin_ctzll(x); }
+
+/* The same number of alpha and of beta electrons in every determinant. */
+static int bit_counts_ok(const int64_t *a, const int64_t *b, int64_t n)
+{
+    for (int64_t i = 1; i < n; ++i)
+        if (bit_popc((uint64_t)a[i]) != bit_popc((uint64_t)a[0]) ||
+            bit_popc((uint64_t)b[i]) != bit_popc((uint64_t)b[0]))
+            return 0;
+    return 1;
+}
+
+/* The E20.10 write of mcci_to_bit followed by the list-directed read of
+ * createtwordm_bit: the nearest double to x rounded to 10 significant
+ * decimal digits. */
+static double bit_text_round(double x)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.9E", x);
+    return strtod(buf, NULL);
+}
+
+int orc_mcci_to_bit(const int64_t *alpha, const int64_t *beta, const double *c1,
+                    const double *c2, int64_t n, int text_round, double *c1o,
+                    double *c2o)
+{
+    /* src/one_rdm.f90:1901-1984: the slot list in orbital order (alpha
+     * before beta on each orbital; beta slot j+1+nbft) and in alpha-then-beta
+     * order; the number of exchanges that turn the second into the first
+     * sets the sign of both coefficients. */
+    if (n <= 0) return -1;
+    const int nbft = bit_nbft(alpha, beta, n);
+    const int ne = bit_popc((uint64_t)alpha[0]);
+    const int ntotal = 2 * ne;
+    int64_t *l1 = malloc(sizeof(int64_t) * (ntotal + 1));
+    int64_t *l2 = malloc(sizeof(int64_t) * (ntotal + 1));
+    int rc = 0;
+    for (int64_t ici = 0; ici < n && rc == 0; ++ici) {
+        if (bit_popc((uint64_t)alpha[ici]) != ne || bit_popc((uint64_t)beta[ici]) != ne) {
+            rc = -1;
+            break;
+        }
+        int k = 0;
+        for (int j = 0; j < nbft; ++j) {
+            if ((alpha[ici] >> j) & 1) l1[k++] = j + 1;
+            if ((beta[ici] >> j) & 1) l1[k++] = j + 1 + nbft;
+        }
+        k = 0;
+        for (int j = 0; j < nbft; ++j)
+            if ((alpha[ici] >> j) & 1) l2[k++] = j + 1;
+        for (int j = 0; j < nbft; ++j)
+            if ((beta[ici] >> j) & 1) l2[k++] = j + 1 + nbft;
+        int iperm = 0;
+        for (int j = 0; j < ntotal - 1; ++j) {
+            if (l1[j] == l2[j]) continue;
+            for (int kk = j + 1; kk < ntotal; ++kk)
+                if (l1[j] == l2[kk]) {
+                    ++iperm;
+                    int64_t t = l2[j];
+                    l2[j] = l2[kk];
+                    l2[kk] = t;
+                    break;
+                }
+        }
+        const double phase = (iperm & 1) ? -1.0 : 1.0;
+        double x1 = phase * c1[ici], x2 = phase * c2[ici];
+        if (text_round) {
+            x1 = bit_text_round(x1);
+            x2 = bit_text_round(x2);
+        }
+        c1o[ici] = x1;
+        c2o[ici] = x2;
+    }
+    free(l1);
+    free(l2);
+    return rc;
+}
+
+int orc_createtwordm_bit(const int64_t *alpha, const int64_t *beta, const double *c1,
+                         const double *c2, int64_t n, double *tw, int cap_norb)
+{
+    /* src/one_rdm.f90:931-1300.  Returns nbft; writes the dense
+     * SpinFree2RDM (nbft^4, column-major) when tw != NULL and
+     * nbft <= cap_norb.  -1: inconsistent electron counts. */
+    if (n <= 0 || !bit_counts_ok(alpha, beta, n)) return -1;
+    const int N = bit_nbft(alpha, beta, n);
+    if (!tw || N > cap_norb) return N;
+    const size_t NN = (size_t)N;
+    memset(tw, 0, sizeof(double) * NN * NN * NN * NN);
+#define TWB(p, q, r, s) \
+    tw[(size_t)((p)-1) + NN * ((size_t)((q)-1) + NN * ((size_t)((r)-1) + NN * (size_t)((s)-1)))]
+    const int64_t *ic[2] = {alpha, beta};
+    /* no differences (src/one_rdm.f90:1009-1046) */
+    for (int64_t i = 0; i < n; ++i) {
+        const double c = c1[i] * c2[i];
+        for (int sp = 0; sp < 2; ++sp) {
+            uint64_t b1 = (uint64_t)ic[sp][i];
+            while (b1) {
+                const int so = bit_ctz(b1) + 1;
+                b1 &= b1 - 1;
+                for (int sp2 = 0; sp2 < 2; ++sp2) {
+                    uint64_t b2 = (uint64_t)ic[sp2][i];
+                    while (b2) {
+                        const int qo = bit_ctz(b2) + 1;
+                        b2 &= b2 - 1;
+                        if (qo == so && sp == sp2) continue;
+                        if (sp == sp2) TWB(so, qo, so, qo) = TWB(so, qo, so, qo) - c;
+                        TWB(qo, so, so, qo) = TWB(qo, so, so, qo) + c;
+                    }
+                }
+            }
+        }
+    }
+    /* every ordered pair (src/one_rdm.f90:1054-1300) */
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j) {
+            const int nd = bit_popc((uint64_t)(alpha[i] ^ alpha[j])) +
+                           bit_popc((uint64_t)(beta[i] ^ beta[j]));
+            if (nd > 4) continue;
+            if (nd == 4) {
+                int hol[2][2] = {{0}}, par[2][2] = {{0}}, nh[2] = {0, 0};
+                int nperm = 0, same = -1;
+                for (int sp = 0; sp < 2; ++sp) {
+                    const uint64_t x = (uint64_t)(ic[sp][i] ^ ic[sp][j]);
+                    uint64_t h = x & (uint64_t)ic[sp][i], pt = x & (uint64_t)ic[sp][j];
+                    int np_ = 0;
+                    while (pt) {
+                        par[sp][np_++] = bit_ctz(pt) + 1;
+                        pt &= pt - 1;
+                    }
+                    while (h) {
+                        hol[sp][nh[sp]++] = bit_ctz(h) + 1;
+                        h &= h - 1;
+                    }
+                    for (int k = 0; k < nh[sp]; ++k) {
+                        const int lo = hol[sp][k] < par[sp][k] ? hol[sp][k] : par[sp][k];
+                        const int hi = hol[sp][k] < par[sp][k] ? par[sp][k] : hol[sp][k];
+                        nperm += bit_popc((uint64_t)ic[sp][i] & bit_between(lo, hi));
+                    }
+                    if (nh[sp] == 2) {
+                        same = sp;
+                        const int a = hol[sp][0] < par[sp][0] ? hol[sp][0] : par[sp][0];
+                        const int b = hol[sp][0] < par[sp][0] ? par[sp][0] : hol[sp][0];
+                        const int cc = hol[sp][1] < par[sp][1] ? hol[sp][1] : par[sp][1];
+                        const int d = hol[sp][1] < par[sp][1] ? par[sp][1] : hol[sp][1];
+                        if (cc > a && cc < b && d > b) ++nperm;
+                        break;
+                    }
+                }
+                const double c = c1[i] * c2[j] * ((nperm & 1) ? -1.0 : 1.0);
+                if (same < 0) {
+                    const int so = par[0][0], qo = par[1][0], po = hol[1][0], ro = hol[0][0];
+                    TWB(po, ro, so, qo) = TWB(po, ro, so, qo) + c;
+                    TWB(ro, po, qo, so) = TWB(ro, po, qo, so) + c;
+                } else {
+                    const int so = par[same][0], qo = par[same][1], po = hol[same][1],
+                              ro = hol[same][0];
+                    TWB(po, ro, so, qo) = TWB(po, ro, so, qo) + c;
+                    TWB(ro, po, so, qo) = TWB(ro, po, so, qo) - c;
+                    TWB(po, ro, qo, so) = TWB(po, ro, qo, so) - c;
+                    TWB(ro, po, qo, so) = TWB(ro, po, qo, so) + c;
+                }
+                continue;
+            }
+            if (nd == 2) {
+                int sp = 0, tz = 0, tz2 = 0;
+                for (sp = 0; sp < 2; ++sp) {
+                    const uint64_t x = (uint64_t)(ic[sp][i] ^ ic[sp][j]);
+                    const uint64_t h = x & (uint64_t)ic[sp][i], pt = x & (uint64_t)ic[sp][j];
+                    if (h) {
+                        tz = 1 + bit_ctz(h);
+                        tz2 = 1 + bit_ctz(pt);
+                        break;
+                    }
+                }
+                const int lo = tz < tz2 ? tz : tz2, hi = tz < tz2 ? tz2 : tz;
+                const int nperm = bit_popc((uint64_t)ic[sp][i] & bit_between(lo, hi));
+                const double c = c1[i] * c2[j] * ((nperm & 1) ? -1.0 : 1.0);
+                const int po = tz, qo = tz2;
+                uint64_t b = (uint64_t)ic[sp][i] & ~(((uint64_t)1) << (po - 1));
+                while (b) {
+                    const int ro = bit_ctz(b) + 1;
+                    b &= b - 1;
+                    TWB(po, ro, ro, qo) = TWB(po, ro, ro, qo) + c;
+                    TWB(ro, po, qo, ro) = TWB(ro, po, qo, ro) + c;
+                    TWB(ro, po, ro, qo) = TWB(ro, po, ro, qo) - c;
+                    TWB(po, ro, qo, ro) = TWB(po, ro, qo, ro) - c;
+                }
+                b = (uint64_t)ic[1 - sp][i];
+                while (b) {
+                    const int ro = bit_ctz(b) + 1;
+                    b &= b - 1;
+                    TWB(po, ro, ro, qo) = TWB(po, ro, ro, qo) + c;
+                    TWB(ro, po, qo, ro) = TWB(ro, po, qo, ro) + c;
+                }
+            }
+        }
+#undef TWB
+    return N;
+}
+
+int orc_one_rdm_bit(const int64_t *alpha, const int64_t *beta, const double *civs,
+                    int64_t n, double *g, int cap_norb)
+{
+    /* src/one_rdm.f90:1388-1516.  Returns maxnmo; writes onerdm (maxnmo^2,
+     * column-major) when g != NULL and maxnmo <= cap_norb.  civs is the
+     * file's third column (the read at :1428 stores both coefficient
+     * columns into civs(i), the second one last). */
+    if (n <= 0 || !bit_counts_ok(alpha, beta, n)) return -1;
+    const int N = bit_nbft(alpha, beta, n);
+    if (!g || N > cap_norb) return N;
+    const size_t NN = (size_t)N;
+    memset(g, 0, sizeof(double) * NN * NN);
+#define G1(p, q) g[(size_t)((p)-1) + NN * (size_t)((q)-1)]
+    const int64_t *ic[2] = {alpha, beta};
+    for (int64_t i = 0; i < n; ++i)
+        for (int sp = 0; sp < 2; ++sp) {
+            uint64_t b = (uint64_t)ic[sp][i];
+            while (b) {
+                const int j = bit_ctz(b);
+                G1(j + 1, j + 1) = G1(j + 1, j + 1) + civs[i] * civs[i];
+                b &= b - 1;
+            }
+        }
+    for (int64_t i = 0; i + 1 < n; ++i)
+        for (int64_t j = i + 1; j < n; ++j) {
+            const int d1 = bit_popc((uint64_t)(alpha[i] ^ alpha[j]));
+            const int d2 = bit_popc((uint64_t)(beta[i] ^ beta[j]));
+            if (d1 + d2 != 2) continue;
+            for (int sp = 0; sp < 2; ++sp) {
+                if ((sp == 0 ? d1 : d2) != 2) continue;
+                const uint64_t x = (uint64_t)(ic[sp][i] ^ ic[sp][j]);
+                const int po = bit_ctz((uint64_t)ic[sp][i] & x) + 1;
+                const int qo = bit_ctz((uint64_t)ic[sp][j] & x) + 1;
+                const int lo = po < qo ? po : qo, hi = po < qo ? qo : po;
+                const int ep = bit_popc((uint64_t)ic[sp][i] & bit_between(lo, hi));
+                const double ph = (ep & 1) ? -1.0 : 1.0;
+                G1(po, qo) = G1(po, qo) + civs[i] * civs[j] * ph;
+                G1(qo, po) = G1(qo, po) + civs[i] * civs[j] * ph;
+            }
+        }
+#undef G1
+    return N;
 }

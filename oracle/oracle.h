@@ -167,6 +167,27 @@ void orc_bessels0rr(double *hsum, int order, const double *mu, int nq,
 int orc_unique_total(const int *matrix, int nrow, int ncol, const double *total,
                      int *rmat, double *rtot, int *iuni, int *irec);
 
+/* mcci_to_bit, src/one_rdm.f90:1901-1984: each coefficient times the sign
+ * of the alpha-then-beta reordering of its slots; text_round != 0 also
+ * applies the E20.10 write and list-directed read that hand the result to
+ * createtwordm_bit.  Returns -1 when a determinant's alpha or beta count
+ * differs from the first alpha count (the reference's slot lists assume
+ * it). */
+int orc_mcci_to_bit(const int64_t *alpha, const int64_t *beta, const double *c1,
+                    const double *c2, int64_t n, int text_round, double *c1o,
+                    double *c2o);
+/* createtwordm_bit, src/one_rdm.f90:931-1300: dense SpinFree2RDM(p,q,r,s)
+ * (nbft^4, column-major) from alpha/beta bit strings (orbital k on bit
+ * k-1).  Returns nbft (writes only when tw != NULL and nbft <= cap_norb),
+ * or -1 when the alpha or beta counts vary.  Select rows with
+ * orc_twordm_select (the loop of src/one_rdm.f90:1312-1348 is the same). */
+int orc_createtwordm_bit(const int64_t *alpha, const int64_t *beta, const double *c1,
+                         const double *c2, int64_t n, double *tw, int cap_norb);
+/* one_rdm_bit, src/one_rdm.f90:1388-1516: onerdm (maxnmo^2, column-major);
+ * civs is the third column of the bit file.  Same return convention. */
+int orc_one_rdm_bit(const int64_t *alpha, const int64_t *beta, const double *civs,
+                    int64_t n, double *g, int cap_norb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -213,7 +213,109 @@ def save_reference_rdm_files():
           f"size={os.path.getsize(path)}B")
 
 
+# ---- bit-string determinant builders ---------------------------------------
+
+REF_BIT = os.path.join(ROOT, "oracle", "_ref", "ref_bit")
+
+
+def bit_dets(seed, norb, ne, n):
+    """n distinct determinants, ne alpha and ne beta electrons in norb
+    orbitals (orbital k on bit k-1), random normalised coefficients; c2 is
+    c1 plus a 1e-3 perturbation, so the two coefficient columns differ."""
+    rng = np.random.default_rng(seed)
+    seen, out = set(), []
+    while len(out) < n:
+        a = sum(1 << int(k) for k in rng.choice(norb, ne, replace=False))
+        b = sum(1 << int(k) for k in rng.choice(norb, ne, replace=False))
+        if (a, b) in seen:
+            continue
+        seen.add((a, b))
+        out.append((a, b))
+    al = np.array([x[0] for x in out], dtype=np.int64)
+    be = np.array([x[1] for x in out], dtype=np.int64)
+    c = rng.standard_normal(n)
+    c /= np.linalg.norm(c)
+    return al, be, c, c + 1e-3 * rng.standard_normal(n)
+
+
+def mrci_dets():
+    """src/MRCI_CI.dat (the reference's MRCI vector, 10 orbitals, 1368
+    rows) as bit strings: '2' on both spins, '/' alpha, '\\' beta; the
+    printed coefficient on both columns."""
+    al, be, c = [], [], []
+    with open(os.path.join(REF_SRC, "MRCI_CI.dat")) as f:
+        next(f)
+        for line in f:
+            occ, val = line.split()
+            a = sum(1 << k for k, ch in enumerate(occ) if ch in "2/")
+            b = sum(1 << k for k, ch in enumerate(occ) if ch in "2\\")
+            al.append(a)
+            be.append(b)
+            c.append(float(val))
+    c = np.array(c)
+    return np.array(al, dtype=np.int64), np.array(be, dtype=np.int64), c, c.copy()
+
+
+BIT_CASES = {
+    # 6 orbitals, 3+3 electrons, 40 determinants
+    "small": lambda: bit_dets(21, 6, 3, 40),
+    # 12 orbitals, 5+5 electrons, 800 determinants
+    "mid": lambda: bit_dets(22, 12, 5, 800),
+    # 31 orbitals (the highest the reference's 32-bit strings hold), 3+3
+    # electrons, 300 determinants: phase masks up to bit 30
+    "wide": lambda: bit_dets(23, 31, 3, 300),
+    # the reference's MRCI vector
+    "mrci": mrci_dets,
+}
+
+
+def run_bit(mode, al, be, c1, c2):
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "in.dat"), "w") as f:
+            for i in range(len(al)):
+                f.write(f"{i + 1} {float(c1[i])!r} {float(c2[i])!r} {al[i]} {be[i]}\n")
+        subprocess.run([REF_BIT, mode, str(len(al)), "out.bin"], cwd=d, check=True,
+                       stdout=subprocess.DEVNULL)
+        with open(os.path.join(d, "out.bin"), "rb") as f:
+            raw = f.read()
+    if mode == "conv":
+        return np.frombuffer(raw, np.float64).reshape(2, len(al)).copy()
+    if mode == "one":
+        N = int(np.frombuffer(raw[:4], np.int32)[0])
+        return np.frombuffer(raw[4:], np.float64).reshape(N, N, order="F").copy()
+    nm = int(np.frombuffer(raw[:4], np.int32)[0])
+    mat = np.frombuffer(raw[4:4 + 16 * nm], np.int32).reshape(4, nm).T.copy()
+    return mat, np.frombuffer(raw[4 + 16 * nm:], np.float64).copy()
+
+
+def save_bit(name):
+    """mcci_to_bit, createtwordm_bit (raw and after mcci_to_bit) and
+    one_rdm_bit of the reference on one case; checked against the oracle."""
+    al, be, c1, c2 = BIT_CASES[name]()
+    conv = run_bit("conv", al, be, c1, c2)
+    two_mat, two_tot = run_bit("two", al, be, c1, c2)
+    route_mat, route_tot = run_bit("route", al, be, c1, c2)
+    g = run_bit("one", al, be, c1, c2)
+    o1, o2 = oracle.mcci_to_bit(al, be, c1, c2)
+    om, ot = oracle.createtwordm_bit(al, be, c1, c2)
+    rm, rt = oracle.createtwordm_bit(al, be, o1, o2)
+    og = oracle.one_rdm_bit(al, be, c2)
+    path = os.path.join(HERE, f"bit_{name}.npz")
+    np.savez_compressed(path, alpha=al, beta=be, c1=c1, c2=c2, conv_c1=conv[0],
+                        conv_c2=conv[1], two_mat=two_mat.astype(np.int8), two_total=two_tot,
+                        route_mat=route_mat.astype(np.int8), route_total=route_tot, onerdm=g)
+    print(f"bit_{name}: n={len(al)} rows={len(two_tot)}/{len(route_tot)} "
+          f"conv-equal={np.array_equal(conv[0], o1) and np.array_equal(conv[1], o2)} "
+          f"two-equal={np.array_equal(two_mat, om) and np.array_equal(two_tot, ot)} "
+          f"route-equal={np.array_equal(route_mat, rm) and np.array_equal(route_tot, rt)} "
+          f"one-equal={np.array_equal(g, og)} size={os.path.getsize(path)}B", flush=True)
+
+
 def main(names):
+    if "bit" in names:
+        for b in BIT_CASES:
+            save_bit(b)
+        names = [n for n in names if n != "bit"]
     if "vars" in names:
         for v in VAR_CASES:
             save_vars(v)

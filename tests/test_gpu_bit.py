@@ -1,0 +1,157 @@
+"""The bit-string route on the GPU (thistle_mcci_to_bit,
+thistle_createtwordm_bit, thistle_one_rdm_bit,
+thistle_total_scattering_calculation_2) against the reference's own
+routines (flang build of src/one_rdm.f90, the bit_* golden fixtures) and the
+oracle, bit for bit: same rows in the same order, identical doubles.  Covers
+single, double (same- and opposite-spin) and no-difference pairs, the
+reference's MRCI vector, orbitals up to 31 (the reference's 32-bit strings)
+and beyond, unequal alpha / beta counts, duplicate determinants and the
+four calculation types of the file route."""
+import numpy as np
+import pytest
+
+import oracle
+from thistle_amd import readers
+from thistle_amd import scattering as sc
+
+pytestmark = pytest.mark.gpu
+
+BIT_CASES = ["small", "mid", "wide", "mrci"]
+
+
+def rand_dets(seed, norb, na, nb, n, dup=0):
+    rng = np.random.default_rng(seed)
+    seen, out = set(), []
+    while len(out) < n:
+        a = sum(1 << int(k) for k in rng.choice(norb, na, replace=False))
+        b = sum(1 << int(k) for k in rng.choice(norb, nb, replace=False))
+        if (a, b) in seen:
+            continue
+        seen.add((a, b))
+        out.append((a, b))
+    out += [out[int(k)] for k in rng.integers(0, n, dup)]
+    al = np.array([x[0] for x in out], dtype=np.int64)
+    be = np.array([x[1] for x in out], dtype=np.int64)
+    c = rng.standard_normal(len(out))
+    c /= np.linalg.norm(c)
+    return al, be, c, c + 1e-3 * rng.standard_normal(len(out))
+
+
+def same_rows(got, ref):
+    return np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+@pytest.mark.parametrize("name", BIT_CASES)
+def test_reference_goldens(golden, name):
+    g = golden(f"bit_{name}")
+    print(f"bit case {name}: n={g['alpha'].size}", flush=True)
+    a, b, c1, c2 = g["alpha"], g["beta"], g["c1"], g["c2"]
+    o1, o2 = sc.mcci_to_bit(a, b, c1, c2)
+    assert np.array_equal(o1, g["conv_c1"]) and np.array_equal(o2, g["conv_c2"])
+    assert same_rows(sc.createtwordm_bit(a, b, c1, c2),
+                     (g["two_mat"].astype(np.int32), g["two_total"]))
+    assert same_rows(sc.createtwordm_bit(a, b, o1, o2),
+                     (g["route_mat"].astype(np.int32), g["route_total"]))
+    assert np.array_equal(sc.one_rdm_bit(a, b, c2), g["onerdm"])
+
+
+@pytest.mark.parametrize("seed,norb,na,nb,n,dup", [
+    (31, 16, 6, 6, 3000, 0),   # 3000 determinants, many connected pairs
+    (32, 48, 4, 4, 600, 0),    # orbitals beyond the reference's 31
+    (33, 10, 4, 2, 400, 0),    # unequal alpha and beta counts
+    (34, 8, 3, 3, 200, 25),    # duplicate determinants
+])
+def test_vs_oracle(seed, norb, na, nb, n, dup):
+    a, b, c1, c2 = rand_dets(seed, norb, na, nb, n, dup)
+    print(f"bit oracle seed {seed}: n={a.size} norb={norb}", flush=True)
+    assert same_rows(sc.createtwordm_bit(a, b, c1, c2), oracle.createtwordm_bit(a, b, c1, c2))
+    assert np.array_equal(sc.one_rdm_bit(a, b, c2), oracle.one_rdm_bit(a, b, c2))
+    if na == nb:
+        o = sc.mcci_to_bit(a, b, c1, c2)
+        r = oracle.mcci_to_bit(a, b, c1, c2)
+        assert np.array_equal(o[0], r[0]) and np.array_equal(o[1], r[1])
+        o = sc.mcci_to_bit(a, b, c1, c2, text_round=False)
+        r = oracle.mcci_to_bit(a, b, c1, c2, text_round=False)
+        assert np.array_equal(o[0], r[0]) and np.array_equal(o[1], r[1])
+    else:
+        with pytest.raises(Exception):
+            sc.mcci_to_bit(a, b, c1, c2)
+
+
+def test_cutoff_argument():
+    a, b, c1, c2 = rand_dets(35, 9, 3, 3, 150)
+    tw = oracle.twordm_bit_dense(a, b, c1, c2)
+    for cut in (1e-6, 1e-3):
+        assert same_rows(sc.createtwordm_bit(a, b, c1, c2, cutoff=cut),
+                         oracle.twordm_select(tw, cut))
+
+
+def _bits_of(confs):
+    """Canonical punch rows (1 on slot 2k, 2 on slot 2k+1) as bit strings."""
+    a = np.zeros(confs.shape[0], dtype=np.int64)
+    b = np.zeros(confs.shape[0], dtype=np.int64)
+    for k in range(confs.shape[1] // 2):
+        a |= (confs[:, 2 * k] != 0).astype(np.int64) << k
+        b |= (confs[:, 2 * k + 1] != 0).astype(np.int64) << k
+    return a, b
+
+
+def _close(S, ref, rtol=1e-12, ntol=1e-14):
+    return np.all(np.abs(S - ref) <= rtol * np.abs(ref) + ntol * np.max(np.abs(ref)))
+
+
+def test_route_types(golden, tmp_path):
+    """total_scattering_calculation_2 (file route) for types 1, 2, 7, 8 and
+    both mcci settings against the oracle fed the reference's conversions:
+    1 and 8 read mcci_to_bit's output, 7 only with mcci, 2 never."""
+    g = golden("xyx_cas44")
+    a, b = _bits_of(g["confs"])
+    rng = np.random.default_rng(36)
+    c1 = g["civs"][:, 0]
+    c2 = c1 + 1e-3 * rng.standard_normal(c1.size)
+    o1, o2 = oracle.mcci_to_bit(a, b, c1, c2)
+    cz, cm, cc = g["cutoffs"]
+    geo = (g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"], g["yy"], g["zz"])
+    kw = dict(cutoffz=cz, cutoffmd=cm, cutoffcentre=cc)
+    path = str(tmp_path / "bits.dat")
+    readers.write_bit_file(path, a, b, c1, c2)
+    l = g["l"]
+
+    def call(typ, mcci):
+        print(f"route type {typ} mcci {mcci}", flush=True)
+        return sc.total_scattering_calculation_2(
+            typ, np.ones(3), np.zeros((3, 3)), 1, 1, int(l.max()), l.size,
+            int(g["group"].max()), g["ga"], l, g["m"], g["n"], g["xx"], g["yy"], g["zz"],
+            g["mmod"], g["q"], g["q"].size, g["group"], cz, cm, cc, path, "", a.size,
+            None, None, None, None, None, False, None, None, None, None, False, mcci)
+
+    m1, t1 = oracle.createtwordm_bit(a, b, o1, o2)
+    ref1 = oracle.total_j0(*geo, g["mmod"], m1, t1, g["q"], **kw)
+    for mcci in (False, True):
+        q_abs, S = call(1, mcci)
+        assert np.all(q_abs == 0.0)
+        assert _close(S, ref1)
+    m0, t0 = oracle.createtwordm_bit(a, b, c1, c2)
+    S = call(7, False)[1]
+    assert _close(S, oracle.total_j2(*geo, g["mmod"], m0, t0, g["q"], **kw), 1e-11)
+    S = call(7, True)[1]
+    assert _close(S, oracle.total_j2(*geo, g["mmod"], m1, t1, g["q"], **kw), 1e-11)
+    Z2 = oracle.elastic_z(oracle.one_rdm_bit(a, b, c2), g["mmod"])
+    assert _close(call(2, False)[1], oracle.elastic_j0(*geo, Z2, g["q"], **kw))
+    Z8 = oracle.elastic_z(oracle.one_rdm_bit(a, b, o2), g["mmod"])
+    assert _close(call(8, True)[1], oracle.elastic_j2(*geo, Z8, g["q"], **kw), 1e-11)
+
+
+def test_route_readrdm_and_fci(golden, tmp_path):
+    g = golden("h2")
+    cz, cm, cc = g["cutoffs"]
+    l = g["l"]
+    p2 = str(tmp_path / "twordm.dat")
+    readers.write_twordm(p2, g["mat"], g["total"])
+    args = (1, np.ones(2), np.zeros((2, 3)), 1, 1, int(l.max()), l.size, int(g["group"].max()),
+            g["ga"], l, g["m"], g["n"], g["xx"], g["yy"], g["zz"], g["mmod"], g["q"],
+            g["q"].size, g["group"], cz, cm, cc, "", p2, 0, None, None, None, None, None)
+    S = sc.total_scattering_calculation_2(*args, False, None, None, None, None, True, False)[1]
+    assert np.max(np.abs(S - g["S_total"]) / np.abs(g["S_total"])) < 1e-10
+    with pytest.raises(NotImplementedError):
+        sc.total_scattering_calculation_2(*args, True, None, None, None, None, False, False)

@@ -170,3 +170,25 @@ def test_variables_total_tables(golden, name):
     z1, z2 = oracle.build_z12(g["mmod"], mat, tot, l.size)
     np.testing.assert_array_equal(z1, v["z11"])
     np.testing.assert_array_equal(z2, v["z22"])
+
+
+BIT_CASES = ["small", "mid", "wide", "mrci"]
+
+
+@pytest.mark.parametrize("name", BIT_CASES)
+def test_bit_builders(golden, name):
+    """mcci_to_bit (E20.10 round trip included), createtwordm_bit on the raw
+    and on the converted coefficients, and one_rdm_bit, against the
+    reference's own routines (flang build of src/one_rdm.f90): identical
+    doubles and rows."""
+    g = golden(f"bit_{name}")
+    a, b, c1, c2 = g["alpha"], g["beta"], g["c1"], g["c2"]
+    o1, o2 = oracle.mcci_to_bit(a, b, c1, c2)
+    assert np.array_equal(o1, g["conv_c1"]) and np.array_equal(o2, g["conv_c2"])
+    m, t = oracle.createtwordm_bit(a, b, c1, c2)
+    assert np.array_equal(m, g["two_mat"].astype(np.int32))
+    assert np.array_equal(t, g["two_total"])
+    m, t = oracle.createtwordm_bit(a, b, o1, o2)
+    assert np.array_equal(m, g["route_mat"].astype(np.int32))
+    assert np.array_equal(t, g["route_total"])
+    assert np.array_equal(oracle.one_rdm_bit(a, b, c2), g["onerdm"])

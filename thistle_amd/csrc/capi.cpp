@@ -72,7 +72,8 @@ int dalloc(DevBuf& b, size_t bytes) {
     return THISTLE_OK;
 }
 
-const ThistleOpts kDefaultOpts = {THISTLE_FP64, 1e-30, nullptr, 0, 0};
+const ThistleOpts kDefaultOpts = {THISTLE_FP64, THISTLE_RDM_CUTOFF, nullptr, 0, 0};
+static_assert(THISTLE_RDM_CUTOFF == (double)1e-30f, "1E-30 as a default real");
 
 // Mixed precision: a q point is re-evaluated in FP64 when |S(q)| <
 // kMixedKappa * sqrt(A2(q)), A2(q) = sum over the FP32-evaluated terms of
@@ -1516,135 +1517,7 @@ int64_t thistle_twordm_select(const double* twordm, int32_t norb, double cutoff,
 
 namespace {
 
-// the device accumulation of createtwordm (rdm.hip); count of selected rows,
-// rows written to mat / total when cap >= count
-int64_t createtwordm_device(const int32_t* confs, int nconf, int lconfs, const double* c1v_h,
-                            const double* c2v_h, double cutoff, int device, int32_t* mat,
-                            double* total, int64_t cap) {
-#define RCHECK(x)                                                               \
-    do {                                                                        \
-        hipError_t e_ = (x);                                                    \
-        if (e_ != hipSuccess) {                                                 \
-            std::fprintf(stderr, "thistle: %s failed: %s (%s:%d)\n", #x,        \
-                         hipGetErrorString(e_), __FILE__, __LINE__);            \
-            return -THISTLE_EDEVICE;                                            \
-        }                                                                       \
-    } while (0)
-#define RALLOC(b, n)                                        \
-    do {                                                    \
-        if (dalloc(b, n)) return -THISTLE_ENOMEM;           \
-    } while (0)
-    RCHECK(hipSetDevice(device));
-    hipStream_t st;
-    RCHECK(hipStreamCreate(&st));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{st};
-    const int N = lconfs / 2, nw = (lconfs + 63) / 64;
-    const int64_t n4 = (int64_t)N * N * N * N;
-    DevBuf dconfs, masks, civ, cnt, off, c1s, c2s, nds, ccnt, coff, keys, vals, keys2, vals2,
-        temp, start, tw, flags, pos, rows, outv;
-    RALLOC(dconfs, sizeof(int32_t) * (size_t)nconf * lconfs);
-    RALLOC(masks, sizeof(uint64_t) * (size_t)nconf * nw);
-    RALLOC(civ, sizeof(double) * 2 * (size_t)nconf);
-    RCHECK(hipMemcpyAsync(dconfs.p, confs, dconfs.bytes, hipMemcpyHostToDevice, st));
-    RCHECK(hipMemcpyAsync(civ.p, c1v_h, sizeof(double) * nconf, hipMemcpyHostToDevice, st));
-    RCHECK(hipMemcpyAsync(civ.as<double>() + nconf, c2v_h, sizeof(double) * nconf,
-                          hipMemcpyHostToDevice, st));
-    RCHECK(rdm_pack(dconfs.as<int32_t>(), nconf, lconfs, nw, masks.as<uint64_t>(), st));
-    // 1. connected pairs in (c1, c2) order
-    RALLOC(cnt, sizeof(int64_t) * ((size_t)nconf + 1));
-    RALLOC(off, sizeof(int64_t) * ((size_t)nconf + 1));
-    RCHECK(launch_fill(reinterpret_cast<double*>(cnt.p), nconf + 1, 0.0, st));
-    RCHECK(rdm_pairs(masks.as<uint64_t>(), nconf, nw, 4, cnt.as<int64_t>(), nullptr, nullptr,
-                     nullptr, nullptr, st));
-    size_t tb = 0;
-    RCHECK(rdm_scan_i64(nullptr, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
-    RALLOC(temp, tb);
-    RCHECK(rdm_scan_i64(temp.p, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
-    int64_t np = 0;
-    RCHECK(hipMemcpyAsync(&np, off.as<int64_t>() + nconf, sizeof(int64_t), hipMemcpyDeviceToHost,
-                          st));
-    RCHECK(hipStreamSynchronize(st));
-    RALLOC(c1s, sizeof(int32_t) * (size_t)std::max<int64_t>(np, 1));
-    RALLOC(c2s, sizeof(int32_t) * (size_t)std::max<int64_t>(np, 1));
-    RALLOC(nds, (size_t)std::max<int64_t>(np, 1));
-    RCHECK(rdm_pairs(masks.as<uint64_t>(), nconf, nw, 4, nullptr, off.as<int64_t>(),
-                     c1s.as<int32_t>(), c2s.as<int32_t>(), nds.as<int8_t>(), st));
-    // 2.-3. contributions at their positions in the reference's order
-    RALLOC(ccnt, sizeof(int64_t) * (size_t)(np + 1));
-    RALLOC(coff, sizeof(int64_t) * (size_t)(np + 1));
-    RCHECK(launch_fill(reinterpret_cast<double*>(ccnt.p), np + 1, 0.0, st));
-    RCHECK(rdm_ccount(masks.as<uint64_t>(), nw, c1s.as<int32_t>(), c2s.as<int32_t>(),
-                      nds.as<int8_t>(), np, ccnt.as<int64_t>(), st));
-    tb = 0;
-    RCHECK(rdm_scan_i64(nullptr, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), np + 1, st));
-    if (tb > temp.bytes) RALLOC(temp, tb);
-    RCHECK(rdm_scan_i64(temp.p, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), np + 1, st));
-    int64_t nc = 0;
-    RCHECK(hipMemcpyAsync(&nc, coff.as<int64_t>() + np, sizeof(int64_t), hipMemcpyDeviceToHost,
-                          st));
-    RCHECK(hipStreamSynchronize(st));
-    const size_t ncs = (size_t)std::max<int64_t>(nc, 1);
-    RALLOC(keys, sizeof(uint32_t) * ncs);
-    RALLOC(vals, sizeof(double) * ncs);
-    RALLOC(keys2, sizeof(uint32_t) * ncs);
-    RALLOC(vals2, sizeof(double) * ncs);
-    RCHECK(rdm_cwrite(masks.as<uint64_t>(), nw, c1s.as<int32_t>(), c2s.as<int32_t>(),
-                      nds.as<int8_t>(), np, civ.as<double>(), civ.as<double>() + nconf,
-                      coff.as<int64_t>(), N, keys.as<uint32_t>(), vals.as<double>(), st));
-    // 4. stable sort by element
-    int keybits = 1;
-    while (((int64_t)1 << keybits) < n4) ++keybits;
-    tb = 0;
-    RCHECK(rdm_sort(nullptr, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
-                    vals2.as<double>(), nc, keybits, st));
-    if (tb > temp.bytes) RALLOC(temp, tb);
-    if (nc > 0)
-        RCHECK(rdm_sort(temp.p, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
-                        vals2.as<double>(), nc, keybits, st));
-    // 5. ordered sums per element
-    RALLOC(start, sizeof(int64_t) * (size_t)(n4 + 1));
-    RALLOC(tw, sizeof(double) * (size_t)n4);
-    RCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, n4, start.as<int64_t>(),
-                    tw.as<double>(), st));
-    // 6. selection in p, q, r, s order
-    const int64_t nb = (n4 + 2047) / 2048 + 1;
-    RALLOC(flags, (size_t)n4);
-    RALLOC(pos, sizeof(int64_t) * (size_t)(n4 + nb + 1));
-    RCHECK(rdm_flags(tw.as<double>(), N, cutoff, flags.as<uint8_t>(), st));
-    RCHECK(launch_exclusive_scan_u8(flags.as<uint8_t>(), n4, pos.as<int64_t>(),
-                                    pos.as<int64_t>() + n4 + nb, st));
-    int64_t nsel = 0;
-    RCHECK(hipMemcpyAsync(&nsel, pos.as<int64_t>() + n4 + nb, sizeof(int64_t),
-                          hipMemcpyDeviceToHost, st));
-    RCHECK(hipStreamSynchronize(st));
-    if (mat && total && cap >= nsel && nsel > 0) {
-        RALLOC(rows, sizeof(int32_t) * 4 * (size_t)nsel);
-        RALLOC(outv, sizeof(double) * (size_t)nsel);
-        RCHECK(rdm_emit(tw.as<double>(), N, flags.as<uint8_t>(), pos.as<int64_t>(),
-                        rows.as<int32_t>(), outv.as<double>(), st));
-        std::vector<int32_t> r4((size_t)4 * nsel);
-        RCHECK(hipMemcpyAsync(r4.data(), rows.p, sizeof(int32_t) * r4.size(),
-                              hipMemcpyDeviceToHost, st));
-        RCHECK(hipMemcpyAsync(total, outv.p, sizeof(double) * nsel, hipMemcpyDeviceToHost, st));
-        RCHECK(hipStreamSynchronize(st));
-        for (int64_t i = 0; i < nsel; ++i)
-            for (int k = 0; k < 4; ++k) mat[i + k * cap] = r4[(size_t)4 * i + k];
-    }
-    return nsel;
-#undef RALLOC
-#undef RCHECK
-}
-
-// createonerdm on the GPU: the ndiff <= 2 pairs, each pair's contributions at
-// their positions in the reference's (c1, c2, slot) order, a stable sort by
-// element and one ordered sum per element, so that every element is the
-// reference's double.
-int createonerdm_device(const int32_t* confs, int nconf, int lconfs, const double* c1v_h,
-                        const double* c2v_h, int device, double* onerdm) {
-#define RCHECK(x)                                                               \
+#define DCHECK(x)                                                               \
     do {                                                                        \
         hipError_t e_ = (x);                                                    \
         if (e_ != hipSuccess) {                                                 \
@@ -1653,89 +1526,219 @@ int createonerdm_device(const int32_t* confs, int nconf, int lconfs, const doubl
             return THISTLE_EDEVICE;                                             \
         }                                                                       \
     } while (0)
-#define RALLOC(b, n)                                        \
+#define DALLOC(b, n)                                        \
     do {                                                    \
         if (dalloc(b, n)) return THISTLE_ENOMEM;            \
     } while (0)
-    RCHECK(hipSetDevice(device));
-    hipStream_t st;
-    RCHECK(hipStreamCreate(&st));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{st};
-    const int N = lconfs / 2, nw = (lconfs + 63) / 64;
-    const int64_t n2 = (int64_t)N * N;
-    DevBuf dconfs, masks, civ, cnt, off, c1s, c2s, nds, ccnt, coff, keys, vals, keys2, vals2,
-        temp, start, g;
-    RALLOC(dconfs, sizeof(int32_t) * (size_t)nconf * lconfs);
-    RALLOC(masks, sizeof(uint64_t) * (size_t)nconf * nw);
-    RALLOC(civ, sizeof(double) * 2 * (size_t)nconf);
-    RCHECK(hipMemcpyAsync(dconfs.p, confs, dconfs.bytes, hipMemcpyHostToDevice, st));
-    RCHECK(hipMemcpyAsync(civ.p, c1v_h, sizeof(double) * nconf, hipMemcpyHostToDevice, st));
-    RCHECK(hipMemcpyAsync(civ.as<double>() + nconf, c2v_h, sizeof(double) * nconf,
-                          hipMemcpyHostToDevice, st));
-    RCHECK(rdm_pack(dconfs.as<int32_t>(), nconf, lconfs, nw, masks.as<uint64_t>(), st));
-    // 1. pairs with ndiff <= 2 in (c1, c2) order
-    RALLOC(cnt, sizeof(int64_t) * ((size_t)nconf + 1));
-    RALLOC(off, sizeof(int64_t) * ((size_t)nconf + 1));
-    RCHECK(launch_fill(reinterpret_cast<double*>(cnt.p), nconf + 1, 0.0, st));
-    RCHECK(rdm_pairs(masks.as<uint64_t>(), nconf, nw, 2, cnt.as<int64_t>(), nullptr, nullptr,
-                     nullptr, nullptr, st));
+
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    ~StreamGuard() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+// The determinant builders of rdm.hip share one ordered accumulation: an
+// item list in the reference's loop order ((c1, c2) pairs with ndiff <=
+// maxdiff, after ndiag leading no-difference items for the bit builders),
+// each item's contribution count, scanned into global positions, every
+// (element, value) written at its position, a stable radix sort by element
+// and one ordered sum per element from 0.  dense receives the nel sums.
+enum class RdmKind { Twordm, Onerdm, Bit2, Bit1 };
+
+int rdm_accumulate(RdmKind kind, const uint64_t* masks, int nconf, int nw, int N,
+                   const double* c1v, const double* c2v, DevBuf& dense, hipStream_t st) {
+    const bool bit = kind == RdmKind::Bit2 || kind == RdmKind::Bit1;
+    const int maxdiff = (kind == RdmKind::Twordm || kind == RdmKind::Bit2) ? 4 : 2;
+    const int64_t ndiag = bit ? nconf : 0;
+    const int64_t nel = (kind == RdmKind::Twordm || kind == RdmKind::Bit2)
+                            ? (int64_t)N * N * N * N
+                            : (int64_t)N * N;
+    DevBuf cnt, off, c1s, c2s, nds, ccnt, coff, keys, vals, keys2, vals2, temp, start;
+    // 1. connected pairs per c1, c2 increasing
+    DALLOC(cnt, sizeof(int64_t) * ((size_t)nconf + 1));
+    DALLOC(off, sizeof(int64_t) * ((size_t)nconf + 1));
+    DCHECK(launch_fill(reinterpret_cast<double*>(cnt.p), nconf + 1, 0.0, st));
+    DCHECK(rdm_pairs(masks, nconf, nw, maxdiff, cnt.as<int64_t>(), nullptr, nullptr, nullptr,
+                     nullptr, st));
     size_t tb = 0;
-    RCHECK(rdm_scan_i64(nullptr, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
-    RALLOC(temp, tb);
-    RCHECK(rdm_scan_i64(temp.p, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
+    DCHECK(rdm_scan_i64(nullptr, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
+    DALLOC(temp, tb);
+    DCHECK(rdm_scan_i64(temp.p, tb, cnt.as<int64_t>(), off.as<int64_t>(), nconf + 1, st));
     int64_t np = 0;
-    RCHECK(hipMemcpyAsync(&np, off.as<int64_t>() + nconf, sizeof(int64_t), hipMemcpyDeviceToHost,
+    DCHECK(hipMemcpyAsync(&np, off.as<int64_t>() + nconf, sizeof(int64_t), hipMemcpyDeviceToHost,
                           st));
-    RCHECK(hipStreamSynchronize(st));
-    RALLOC(c1s, sizeof(int32_t) * (size_t)std::max<int64_t>(np, 1));
-    RALLOC(c2s, sizeof(int32_t) * (size_t)std::max<int64_t>(np, 1));
-    RALLOC(nds, (size_t)std::max<int64_t>(np, 1));
-    RCHECK(rdm_pairs(masks.as<uint64_t>(), nconf, nw, 2, nullptr, off.as<int64_t>(),
-                     c1s.as<int32_t>(), c2s.as<int32_t>(), nds.as<int8_t>(), st));
-    // 2.-3. contributions at their positions
-    RALLOC(ccnt, sizeof(int64_t) * (size_t)(np + 1));
-    RALLOC(coff, sizeof(int64_t) * (size_t)(np + 1));
-    RCHECK(launch_fill(reinterpret_cast<double*>(ccnt.p), np + 1, 0.0, st));
-    RCHECK(rdm1_ccount(masks.as<uint64_t>(), nw, c1s.as<int32_t>(), nds.as<int8_t>(), np,
-                       ccnt.as<int64_t>(), st));
+    DCHECK(hipStreamSynchronize(st));
+    const int64_t ni = ndiag + np;
+    DALLOC(c1s, sizeof(int32_t) * (size_t)std::max<int64_t>(ni, 1));
+    DALLOC(c2s, sizeof(int32_t) * (size_t)std::max<int64_t>(ni, 1));
+    DALLOC(nds, (size_t)std::max<int64_t>(ni, 1));
+    if (ndiag) DCHECK(bit_diag(nconf, c1s.as<int32_t>(), c2s.as<int32_t>(), nds.as<int8_t>(), st));
+    DCHECK(rdm_pairs(masks, nconf, nw, maxdiff, nullptr, off.as<int64_t>(),
+                     c1s.as<int32_t>() + ndiag, c2s.as<int32_t>() + ndiag,
+                     nds.as<int8_t>() + ndiag, st));
+    // 2. contributions per item, scanned into positions
+    DALLOC(ccnt, sizeof(int64_t) * (size_t)(ni + 1));
+    DALLOC(coff, sizeof(int64_t) * (size_t)(ni + 1));
+    DCHECK(launch_fill(reinterpret_cast<double*>(ccnt.p), ni + 1, 0.0, st));
+    const int32_t *i1 = c1s.as<int32_t>(), *i2 = c2s.as<int32_t>();
+    const int8_t* id = nds.as<int8_t>();
+    switch (kind) {
+        case RdmKind::Twordm:
+            DCHECK(rdm_ccount(masks, nw, i1, i2, id, ni, ccnt.as<int64_t>(), st));
+            break;
+        case RdmKind::Onerdm:
+            DCHECK(rdm1_ccount(masks, nw, i1, id, ni, ccnt.as<int64_t>(), st));
+            break;
+        case RdmKind::Bit2:
+            DCHECK(bit2_ccount(masks, i1, i2, id, ni, (int)ndiag, ccnt.as<int64_t>(), st));
+            break;
+        case RdmKind::Bit1:
+            DCHECK(bit1_ccount(masks, i1, i2, id, ni, (int)ndiag, ccnt.as<int64_t>(), st));
+            break;
+    }
     tb = 0;
-    RCHECK(rdm_scan_i64(nullptr, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), np + 1, st));
-    if (tb > temp.bytes) RALLOC(temp, tb);
-    RCHECK(rdm_scan_i64(temp.p, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), np + 1, st));
+    DCHECK(rdm_scan_i64(nullptr, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), ni + 1, st));
+    if (tb > temp.bytes) DALLOC(temp, tb);
+    DCHECK(rdm_scan_i64(temp.p, tb, ccnt.as<int64_t>(), coff.as<int64_t>(), ni + 1, st));
     int64_t nc = 0;
-    RCHECK(hipMemcpyAsync(&nc, coff.as<int64_t>() + np, sizeof(int64_t), hipMemcpyDeviceToHost,
+    DCHECK(hipMemcpyAsync(&nc, coff.as<int64_t>() + ni, sizeof(int64_t), hipMemcpyDeviceToHost,
                           st));
-    RCHECK(hipStreamSynchronize(st));
+    DCHECK(hipStreamSynchronize(st));
+    // 3. every contribution at its position
     const size_t ncs = (size_t)std::max<int64_t>(nc, 1);
-    RALLOC(keys, sizeof(uint32_t) * ncs);
-    RALLOC(vals, sizeof(double) * ncs);
-    RALLOC(keys2, sizeof(uint32_t) * ncs);
-    RALLOC(vals2, sizeof(double) * ncs);
-    RCHECK(rdm1_cwrite(masks.as<uint64_t>(), nw, c1s.as<int32_t>(), c2s.as<int32_t>(),
-                       nds.as<int8_t>(), np, civ.as<double>(), civ.as<double>() + nconf,
-                       coff.as<int64_t>(), N, keys.as<uint32_t>(), vals.as<double>(), st));
-    // 4. stable sort by element, 5. ordered sums
+    DALLOC(keys, sizeof(uint32_t) * ncs);
+    DALLOC(vals, sizeof(double) * ncs);
+    DALLOC(keys2, sizeof(uint32_t) * ncs);
+    DALLOC(vals2, sizeof(double) * ncs);
+    uint32_t* k = keys.as<uint32_t>();
+    double* v = vals.as<double>();
+    const int64_t* co = coff.as<int64_t>();
+    switch (kind) {
+        case RdmKind::Twordm:
+            DCHECK(rdm_cwrite(masks, nw, i1, i2, id, ni, c1v, c2v, co, N, k, v, st));
+            break;
+        case RdmKind::Onerdm:
+            DCHECK(rdm1_cwrite(masks, nw, i1, i2, id, ni, c1v, c2v, co, N, k, v, st));
+            break;
+        case RdmKind::Bit2:
+            DCHECK(bit2_cwrite(masks, i1, i2, id, ni, (int)ndiag, c1v, c2v, co, N, k, v, st));
+            break;
+        case RdmKind::Bit1:
+            DCHECK(bit1_cwrite(masks, i1, i2, id, ni, (int)ndiag, c1v, co, N, k, v, st));
+            break;
+    }
+    // 4. stable sort by element
     int keybits = 1;
-    while (((int64_t)1 << keybits) < n2) ++keybits;
+    while (((int64_t)1 << keybits) < nel) ++keybits;
     tb = 0;
-    RCHECK(rdm_sort(nullptr, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
-                    vals2.as<double>(), nc, keybits, st));
-    if (tb > temp.bytes) RALLOC(temp, tb);
+    DCHECK(rdm_sort(nullptr, tb, k, keys2.as<uint32_t>(), v, vals2.as<double>(), nc, keybits, st));
+    if (tb > temp.bytes) DALLOC(temp, tb);
     if (nc > 0)
-        RCHECK(rdm_sort(temp.p, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
-                        vals2.as<double>(), nc, keybits, st));
-    RALLOC(start, sizeof(int64_t) * (size_t)(n2 + 1));
-    RALLOC(g, sizeof(double) * (size_t)n2);
-    RCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, n2, start.as<int64_t>(),
-                    g.as<double>(), st));
-    RCHECK(hipMemcpyAsync(onerdm, g.p, sizeof(double) * (size_t)n2, hipMemcpyDeviceToHost, st));
-    RCHECK(hipStreamSynchronize(st));
+        DCHECK(rdm_sort(temp.p, tb, k, keys2.as<uint32_t>(), v, vals2.as<double>(), nc, keybits,
+                        st));
+    // 5. ordered sums per element
+    DALLOC(start, sizeof(int64_t) * (size_t)(nel + 1));
+    DALLOC(dense, sizeof(double) * (size_t)nel);
+    DCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, nel, start.as<int64_t>(),
+                    dense.as<double>(), st));
     return THISTLE_OK;
-#undef RALLOC
-#undef RCHECK
+}
+
+// 6. the selection loop (p, q, r, s) of a dense N^4 twordm: count of rows,
+// written to mat (cap, 4) / total when cap >= count
+int64_t twordm_select_device(const DevBuf& tw, int N, double cutoff, int32_t* mat, double* total,
+                             int64_t cap, hipStream_t st) {
+#define SCHECK(x)                                   \
+    do {                                            \
+        if ((x) != hipSuccess) return -THISTLE_EDEVICE; \
+    } while (0)
+#define SALLOC(b, n)                                \
+    do {                                            \
+        if (dalloc(b, n)) return -THISTLE_ENOMEM;   \
+    } while (0)
+    const int64_t n4 = (int64_t)N * N * N * N;
+    const int64_t nb = (n4 + 2047) / 2048 + 1;
+    DevBuf flags, pos, rows, outv;
+    SALLOC(flags, (size_t)n4);
+    SALLOC(pos, sizeof(int64_t) * (size_t)(n4 + nb + 1));
+    SCHECK(rdm_flags(tw.as<double>(), N, cutoff, flags.as<uint8_t>(), st));
+    SCHECK(launch_exclusive_scan_u8(flags.as<uint8_t>(), n4, pos.as<int64_t>(),
+                                    pos.as<int64_t>() + n4 + nb, st));
+    int64_t nsel = 0;
+    SCHECK(hipMemcpyAsync(&nsel, pos.as<int64_t>() + n4 + nb, sizeof(int64_t),
+                          hipMemcpyDeviceToHost, st));
+    SCHECK(hipStreamSynchronize(st));
+    if (mat && total && cap >= nsel && nsel > 0) {
+        SALLOC(rows, sizeof(int32_t) * 4 * (size_t)nsel);
+        SALLOC(outv, sizeof(double) * (size_t)nsel);
+        SCHECK(rdm_emit(tw.as<double>(), N, flags.as<uint8_t>(), pos.as<int64_t>(),
+                        rows.as<int32_t>(), outv.as<double>(), st));
+        std::vector<int32_t> r4((size_t)4 * nsel);
+        SCHECK(hipMemcpyAsync(r4.data(), rows.p, sizeof(int32_t) * r4.size(),
+                              hipMemcpyDeviceToHost, st));
+        SCHECK(hipMemcpyAsync(total, outv.p, sizeof(double) * nsel, hipMemcpyDeviceToHost, st));
+        SCHECK(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < nsel; ++i)
+            for (int k = 0; k < 4; ++k) mat[i + k * cap] = r4[(size_t)4 * i + k];
+    }
+    return nsel;
+#undef SALLOC
+#undef SCHECK
+}
+
+// device, stream, masks and the two coefficient vectors of a canonical
+// punch-format determinant list
+int confs_to_device(const int32_t* confs, int nconf, int lconfs, const double* c1v_h,
+                    const double* c2v_h, int device, StreamGuard& sg, DevBuf& masks,
+                    DevBuf& civ) {
+    DCHECK(hipSetDevice(device));
+    DCHECK(hipStreamCreate(&sg.s));
+    const int nw = (lconfs + 63) / 64;
+    DevBuf dconfs;
+    DALLOC(dconfs, sizeof(int32_t) * (size_t)nconf * lconfs);
+    DALLOC(masks, sizeof(uint64_t) * (size_t)nconf * nw);
+    DALLOC(civ, sizeof(double) * 2 * (size_t)nconf);
+    DCHECK(hipMemcpyAsync(dconfs.p, confs, dconfs.bytes, hipMemcpyHostToDevice, sg.s));
+    DCHECK(hipMemcpyAsync(civ.p, c1v_h, sizeof(double) * nconf, hipMemcpyHostToDevice, sg.s));
+    DCHECK(hipMemcpyAsync(civ.as<double>() + nconf, c2v_h, sizeof(double) * nconf,
+                          hipMemcpyHostToDevice, sg.s));
+    DCHECK(rdm_pack(dconfs.as<int32_t>(), nconf, lconfs, nw, masks.as<uint64_t>(), sg.s));
+    DCHECK(hipStreamSynchronize(sg.s));
+    return THISTLE_OK;
+}
+
+// createtwordm on the device (rdm.hip): count of selected rows, rows written
+// to mat / total when cap >= count; -status on failure
+int64_t createtwordm_device(const int32_t* confs, int nconf, int lconfs, const double* c1v_h,
+                            const double* c2v_h, double cutoff, int device, int32_t* mat,
+                            double* total, int64_t cap) {
+    StreamGuard sg;
+    DevBuf masks, civ, tw;
+    int rc = confs_to_device(confs, nconf, lconfs, c1v_h, c2v_h, device, sg, masks, civ);
+    if (rc) return -rc;
+    const int N = lconfs / 2;
+    rc = rdm_accumulate(RdmKind::Twordm, masks.as<uint64_t>(), nconf, (lconfs + 63) / 64, N,
+                        civ.as<double>(), civ.as<double>() + nconf, tw, sg.s);
+    if (rc) return -rc;
+    return twordm_select_device(tw, N, cutoff, mat, total, cap, sg.s);
+}
+
+// createonerdm on the device (rdm.hip): the dense norb x norb onerdm
+int createonerdm_device(const int32_t* confs, int nconf, int lconfs, const double* c1v_h,
+                        const double* c2v_h, int device, double* onerdm) {
+    StreamGuard sg;
+    DevBuf masks, civ, g;
+    int rc = confs_to_device(confs, nconf, lconfs, c1v_h, c2v_h, device, sg, masks, civ);
+    if (rc) return rc;
+    const int N = lconfs / 2;
+    rc = rdm_accumulate(RdmKind::Onerdm, masks.as<uint64_t>(), nconf, (lconfs + 63) / 64, N,
+                        civ.as<double>(), civ.as<double>() + nconf, g, sg.s);
+    if (rc) return rc;
+    DCHECK(hipMemcpyAsync(onerdm, g.p, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost,
+                          sg.s));
+    DCHECK(hipStreamSynchronize(sg.s));
+    return THISTLE_OK;
 }
 
 bool canonical_confs(const int32_t* confs, int nconf, int lconfs) {
@@ -1758,6 +1761,116 @@ int onerdm_any(const int32_t* confs, int nconf, int lconfs, const double* civecs
     return createonerdm_device(confs, nconf, lconfs, civecs + (size_t)nconf * (state1 - 1),
                                civecs + (size_t)nconf * (state2 - 1), device, onerdm);
 }
+
+// ---- bit-string determinants (src/one_rdm.f90) ----------------------------
+
+// Bit strings must be non-negative, with one alpha count and one beta count
+// throughout (the excitation bookkeeping of src/one_rdm.f90:1080-1217 and
+// :1468-1512 reads holes and particles of one spin as pairs); mcci_to_bit
+// also needs the two counts equal (its slot lists have 2 x the first alpha
+// count entries, src/one_rdm.f90:1923,1940).  nbft: the highest occupied
+// orbital (src/one_rdm.f90:985).
+int bit_check(const int64_t* alpha, const int64_t* beta, int64_t n, bool equal_spins,
+              int* nbft) {
+    if (!alpha || !beta || n <= 0 || n > (int64_t)INT32_MAX) return THISTLE_EINVAL;
+    const int na = __builtin_popcountll((uint64_t)alpha[0]);
+    const int nb = __builtin_popcountll((uint64_t)beta[0]);
+    if (equal_spins && na != nb) return THISTLE_EINVAL;
+    int64_t mx = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (alpha[i] < 0 || beta[i] < 0) return THISTLE_EINVAL;
+        if (__builtin_popcountll((uint64_t)alpha[i]) != na ||
+            __builtin_popcountll((uint64_t)beta[i]) != nb)
+            return THISTLE_EINVAL;
+        mx = std::max(mx, std::max(alpha[i], beta[i]));
+    }
+    int w = 0;
+    while (mx) {
+        ++w;
+        mx >>= 1;
+    }
+    if (w == 0) return THISTLE_EINVAL;
+    *nbft = w;
+    return THISTLE_OK;
+}
+
+// mcci_to_bit writes each signed coefficient with E20.10 and
+// createtwordm_bit reads it back (src/one_rdm.f90:1975, :979): the double
+// nearest the value rounded to 10 significant digits.  The rounding is
+// symmetric, so it is applied before the sign.
+double e20_10(double x) {
+    char buf[48];
+    std::snprintf(buf, sizeof buf, "%.9E", x);
+    return std::strtod(buf, nullptr);
+}
+
+// stream, (alpha, beta) masks and the two coefficient vectors on the device
+int bits_to_device(const int64_t* alpha, const int64_t* beta, int64_t n, const double* c1,
+                   const double* c2, int device, StreamGuard& sg, DevBuf& masks, DevBuf& civ) {
+    DCHECK(hipSetDevice(device));
+    DCHECK(hipStreamCreate(&sg.s));
+    DevBuf ab;
+    DALLOC(ab, sizeof(int64_t) * 2 * (size_t)n);
+    DALLOC(masks, sizeof(uint64_t) * 2 * (size_t)n);
+    DALLOC(civ, sizeof(double) * 2 * (size_t)n);
+    DCHECK(hipMemcpyAsync(ab.p, alpha, sizeof(int64_t) * n, hipMemcpyHostToDevice, sg.s));
+    DCHECK(hipMemcpyAsync(ab.as<int64_t>() + n, beta, sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                          sg.s));
+    DCHECK(hipMemcpyAsync(civ.p, c1, sizeof(double) * n, hipMemcpyHostToDevice, sg.s));
+    DCHECK(hipMemcpyAsync(civ.as<double>() + n, c2, sizeof(double) * n, hipMemcpyHostToDevice,
+                          sg.s));
+    DCHECK(bit_pack(ab.as<int64_t>(), ab.as<int64_t>() + n, (int)n, masks.as<uint64_t>(), sg.s));
+    DCHECK(hipStreamSynchronize(sg.s));
+    return THISTLE_OK;
+}
+
+// mcci_to_bit in place on the device coefficients: text rounding on the way
+// in (host copies c1r / c2r), the reordering sign on the device
+int mcci_in_place(const DevBuf& masks, int64_t n, const double* c1, const double* c2,
+                  bool text_round, DevBuf& civ, hipStream_t st) {
+    std::vector<double> r((size_t)2 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        r[i] = text_round ? e20_10(c1[i]) : c1[i];
+        r[n + i] = text_round ? e20_10(c2[i]) : c2[i];
+    }
+    DevBuf in;
+    DALLOC(in, sizeof(double) * r.size());
+    DCHECK(hipMemcpyAsync(in.p, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, st));
+    DCHECK(mcci_phase(masks.as<uint64_t>(), (int)n, in.as<double>(), in.as<double>() + n,
+                      civ.as<double>(), civ.as<double>() + n, st));
+    DCHECK(hipStreamSynchronize(st));
+    return THISTLE_OK;
+}
+
+// createtwordm_bit's rows and values (count first, then the rows)
+int bit2_rows(const DevBuf& masks, int64_t n, int N, const DevBuf& civ, double cutoff,
+              hipStream_t st, std::vector<int32_t>& mat, std::vector<double>& tot,
+              int64_t& cnt) {
+    DevBuf tw;
+    int rc = rdm_accumulate(RdmKind::Bit2, masks.as<uint64_t>(), (int)n, 2, N, civ.as<double>(),
+                            civ.as<double>() + n, tw, st);
+    if (rc) return rc;
+    cnt = twordm_select_device(tw, N, cutoff, nullptr, nullptr, 0, st);
+    if (cnt < 0) return (int)-cnt;
+    mat.assign((size_t)4 * std::max<int64_t>(cnt, 1), 0);
+    tot.assign((size_t)std::max<int64_t>(cnt, 1), 0.0);
+    const int64_t c2 = twordm_select_device(tw, N, cutoff, mat.data(), tot.data(), cnt, st);
+    return c2 < 0 ? (int)-c2 : THISTLE_OK;
+}
+
+// one_rdm_bit's dense N x N onerdm from the device coefficients civ
+int bit1_dense(const DevBuf& masks, int64_t n, int N, const double* civ, hipStream_t st,
+               double* onerdm) {
+    DevBuf g;
+    int rc = rdm_accumulate(RdmKind::Bit1, masks.as<uint64_t>(), (int)n, 2, N, civ, civ, g, st);
+    if (rc) return rc;
+    DCHECK(hipMemcpyAsync(onerdm, g.p, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost, st));
+    DCHECK(hipStreamSynchronize(st));
+    return THISTLE_OK;
+}
+
+#undef DALLOC
+#undef DCHECK
 
 }  // namespace
 
@@ -1848,7 +1961,7 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
     if (type == 1 || type == 7) {
         // createtwordm on the device; at most norb^4 rows, so one pass when
         // that bound is small
-        const double cut = o->rdm_cutoff > 0 ? o->rdm_cutoff : 1e-30;
+        const double cut = o->rdm_cutoff > 0 ? o->rdm_cutoff : THISTLE_RDM_CUTOFF;
         const int64_t n4 = (int64_t)norb * norb * norb * norb;
         int64_t cap = n4 <= ((int64_t)1 << 22) ? n4 : 0;
         std::vector<int32_t> m4((size_t)4 * std::max<int64_t>(cap, 1));
@@ -1886,6 +1999,116 @@ int32_t thistle_total_scattering_calculation(int32_t type, int32_t natom, const 
     // type 8: elastic j2 (src/main_calculation.f90:780-816)
     ThistlePlan* pl = nullptr;
     rc = thistle_plan_create(8, basis, 0, nullptr, nullptr, norb, g.data(), o, &pl);
+    if (rc) return rc;
+    rc = thistle_plan_execute(pl, result_out);
+    thistle_plan_destroy(pl);
+    return rc;
+}
+
+int32_t thistle_mcci_to_bit(const int64_t* alpha, const int64_t* beta, const double* c1,
+                            const double* c2, int64_t n, int32_t text_round,
+                            const ThistleOpts* opts, double* c1_out, double* c2_out) {
+    if (!c1 || !c2 || !c1_out || !c2_out) return THISTLE_EINVAL;
+    int N = 0;
+    int rc = bit_check(alpha, beta, n, true, &N);
+    if (rc) return rc;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    StreamGuard sg;
+    DevBuf masks, civ;
+    if ((rc = bits_to_device(alpha, beta, n, c1, c2, o->device, sg, masks, civ))) return rc;
+    if ((rc = mcci_in_place(masks, n, c1, c2, text_round != 0, civ, sg.s))) return rc;
+    if (hipMemcpyAsync(c1_out, civ.p, sizeof(double) * n, hipMemcpyDeviceToHost, sg.s) !=
+            hipSuccess ||
+        hipMemcpyAsync(c2_out, civ.as<double>() + n, sizeof(double) * n, hipMemcpyDeviceToHost,
+                       sg.s) != hipSuccess ||
+        hipStreamSynchronize(sg.s) != hipSuccess)
+        return THISTLE_EDEVICE;
+    return THISTLE_OK;
+}
+
+int64_t thistle_createtwordm_bit(const int64_t* alpha, const int64_t* beta, const double* c1,
+                                 const double* c2, int64_t n, double cutoff,
+                                 const ThistleOpts* opts, int32_t* mat, double* total,
+                                 int64_t cap) {
+    if (!c1 || !c2) return -THISTLE_EINVAL;
+    if (cap > 0 && (!mat || !total)) return -THISTLE_EINVAL;
+    int N = 0;
+    int rc = bit_check(alpha, beta, n, false, &N);
+    if (rc) return -rc;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    StreamGuard sg;
+    DevBuf masks, civ, tw;
+    if ((rc = bits_to_device(alpha, beta, n, c1, c2, o->device, sg, masks, civ))) return -rc;
+    rc = rdm_accumulate(RdmKind::Bit2, masks.as<uint64_t>(), (int)n, 2, N, civ.as<double>(),
+                        civ.as<double>() + n, tw, sg.s);
+    if (rc) return -rc;
+    return twordm_select_device(tw, N, cutoff > 0 ? cutoff : THISTLE_RDM_CUTOFF, cap > 0 ? mat : nullptr,
+                                cap > 0 ? total : nullptr, cap, sg.s);
+}
+
+int32_t thistle_one_rdm_bit(const int64_t* alpha, const int64_t* beta, const double* civs,
+                            int64_t n, const ThistleOpts* opts, int32_t cap_norb,
+                            int32_t* norb_out, double* onerdm) {
+    if (!civs || !norb_out) return THISTLE_EINVAL;
+    int N = 0;
+    int rc = bit_check(alpha, beta, n, false, &N);
+    if (rc) return rc;
+    *norb_out = N;
+    if (!onerdm) return THISTLE_OK;
+    if (N > cap_norb) return THISTLE_EINVAL;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    StreamGuard sg;
+    DevBuf masks, civ;
+    if ((rc = bits_to_device(alpha, beta, n, civs, civs, o->device, sg, masks, civ))) return rc;
+    return bit1_dense(masks, n, N, civ.as<double>(), sg.s, onerdm);
+}
+
+int32_t thistle_total_scattering_calculation_2(int32_t type, const ThistleBasis* basis,
+                                               int64_t n, const int64_t* alpha,
+                                               const int64_t* beta, const double* c1,
+                                               const double* c2, int32_t mcci,
+                                               const ThistleOpts* opts, double* q_abs_out,
+                                               double* result_out) {
+    if (type != 1 && type != 2 && type != 7 && type != 8) return THISTLE_ETYPE;
+    if (!basis || !result_out || !c1 || !c2) return THISTLE_EINVAL;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    if (q_abs_out)
+        for (int i = 0; i < basis->nq; ++i) q_abs_out[i] = 0.0;
+    // every type but 2 passes the file through mcci_to_bit first
+    // (src/main_calculation.f90:586-604, 730-748, 786-787)
+    int N = 0;
+    int rc = bit_check(alpha, beta, n, type != 2, &N);
+    if (rc) return rc;
+    StreamGuard sg;
+    DevBuf masks, civ;
+    if ((rc = bits_to_device(alpha, beta, n, c1, c2, o->device, sg, masks, civ))) return rc;
+    // type 1 always reads the converted file; type 7 only with mcci (its
+    // default branch converts, then reads the original file,
+    // src/main_calculation.f90:740-744); type 8 converts; type 2 reads the
+    // original
+    const bool converted = type == 1 || type == 8 || (type == 7 && mcci != 0);
+    if (converted && (rc = mcci_in_place(masks, n, c1, c2, true, civ, sg.s))) return rc;
+    if (type == 2 || type == 8) {
+        // one_rdm_bit keeps the third column (src/one_rdm.f90:1428)
+        std::vector<double> g((size_t)N * N);
+        if ((rc = bit1_dense(masks, n, N, civ.as<double>() + n, sg.s, g.data()))) return rc;
+        if (type == 2) return thistle_elastic_from_onerdm(basis, N, g.data(), o, result_out);
+        ThistlePlan* pl = nullptr;
+        rc = thistle_plan_create(8, basis, 0, nullptr, nullptr, N, g.data(), o, &pl);
+        if (rc) return rc;
+        rc = thistle_plan_execute(pl, result_out);
+        thistle_plan_destroy(pl);
+        return rc;
+    }
+    std::vector<int32_t> m4;
+    std::vector<double> tot;
+    int64_t cnt = 0;
+    if ((rc = bit2_rows(masks, n, N, civ, o->rdm_cutoff > 0 ? o->rdm_cutoff : THISTLE_RDM_CUTOFF, sg.s,
+                        m4, tot, cnt)))
+        return rc;
+    if (type == 1) return thistle_total_from_twordm(basis, cnt, m4.data(), tot.data(), o, result_out);
+    ThistlePlan* pl = nullptr;
+    rc = thistle_plan_create(7, basis, cnt, m4.data(), tot.data(), 0, nullptr, o, &pl);
     if (rc) return rc;
     rc = thistle_plan_execute(pl, result_out);
     thistle_plan_destroy(pl);

@@ -429,6 +429,251 @@ __global__ void k_rdm_emit(const double* __restrict__ tw, int N, const uint8_t* 
     out[o] = tw[p + (int64_t)N * (q + (int64_t)N * (r + (int64_t)N * s))];
 }
 
+// ---- bit-string builders (src/one_rdm.f90) -------------------------------
+// A determinant is two 64-bit words, w[0] alpha and w[1] beta, orbital k on
+// bit k-1 (the reference's integer bit strings).  The item list is the n
+// determinants themselves (the no-difference pass, run first for every
+// determinant) followed by the connected ordered pairs of k_rdm_pairs, so
+// scanning per-item counts gives every contribution its place in the
+// reference's global order.
+
+// orbitals strictly between 1-based orbitals x and y
+__device__ __forceinline__ uint64_t bit_between(int x, int y) {
+    const int lo = x < y ? x : y, hi = x < y ? y : x;
+    return below(hi - 1) & ~below(lo);
+}
+
+__global__ void k_bit_pack(const int64_t* __restrict__ alpha, const int64_t* __restrict__ beta,
+                           int n, uint64_t* __restrict__ masks) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    masks[2 * (size_t)i] = (uint64_t)alpha[i];
+    masks[2 * (size_t)i + 1] = (uint64_t)beta[i];
+}
+
+// the determinants' own items (c1 = c2 = i) at the head of the list
+__global__ void k_bit_diag(int n, int32_t* __restrict__ c1s, int32_t* __restrict__ c2s,
+                           int8_t* __restrict__ nds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    c1s[i] = i;
+    c2s[i] = i;
+    nds[i] = 0;
+}
+
+// createtwordm_bit's contributions of item i (src/one_rdm.f90:1009-1300),
+// in statement order; emit(key, value)
+template <class F>
+__device__ __forceinline__ void bit2_contribs(const DMask<2>& m1, const DMask<2>& m2, int nd,
+                                              bool diag, double c1i, double c2j, int N,
+                                              F&& emit) {
+    auto TW = [N](int p, int q, int r, int s) -> uint32_t {
+        return (uint32_t)((p - 1) + N * ((q - 1) + N * ((r - 1) + N * (s - 1))));
+    };
+    if (diag) {
+        // no differences: every ordered choice of two occupied slots
+        const double c = c1i * c2j;
+        for (int sp = 0; sp < 2; ++sp) {
+            uint64_t b1 = m1.w[sp];
+            while (b1) {
+                const int so = __builtin_ctzll(b1) + 1;
+                b1 &= b1 - 1;
+                for (int sp2 = 0; sp2 < 2; ++sp2) {
+                    uint64_t b2 = m1.w[sp2];
+                    while (b2) {
+                        const int qo = __builtin_ctzll(b2) + 1;
+                        b2 &= b2 - 1;
+                        if (qo == so && sp == sp2) continue;
+                        if (sp == sp2) emit(TW(so, qo, so, qo), -c);
+                        emit(TW(qo, so, so, qo), c);
+                    }
+                }
+            }
+        }
+        return;
+    }
+    if (nd == 4) {
+        int hol[2][2] = {{0, 0}, {0, 0}}, par[2][2] = {{0, 0}, {0, 0}};
+        int nperm = 0, same = -1;
+        for (int sp = 0; sp < 2; ++sp) {
+            const uint64_t x = m1.w[sp] ^ m2.w[sp];
+            uint64_t h = x & m1.w[sp], pt = x & m2.w[sp];
+            int nh = 0, np_ = 0;
+            while (pt && np_ < 2) {
+                par[sp][np_++] = __builtin_ctzll(pt) + 1;
+                pt &= pt - 1;
+            }
+            while (h && nh < 2) {
+                hol[sp][nh++] = __builtin_ctzll(h) + 1;
+                h &= h - 1;
+            }
+            for (int k = 0; k < nh; ++k)
+                nperm += __popcll(m1.w[sp] & bit_between(hol[sp][k], par[sp][k]));
+            if (nh == 2) {
+                same = sp;
+                const int a = min(hol[sp][0], par[sp][0]), b = max(hol[sp][0], par[sp][0]);
+                const int c = min(hol[sp][1], par[sp][1]), d = max(hol[sp][1], par[sp][1]);
+                if (c > a && c < b && d > b) ++nperm;
+                break;
+            }
+        }
+        const double c = c1i * c2j * ((nperm & 1) ? -1.0 : 1.0);
+        if (same < 0) {
+            const int so = par[0][0], qo = par[1][0], po = hol[1][0], ro = hol[0][0];
+            emit(TW(po, ro, so, qo), c);
+            emit(TW(ro, po, qo, so), c);
+        } else {
+            const int so = par[same][0], qo = par[same][1], po = hol[same][1], ro = hol[same][0];
+            emit(TW(po, ro, so, qo), c);
+            emit(TW(ro, po, so, qo), -c);
+            emit(TW(po, ro, qo, so), -c);
+            emit(TW(ro, po, qo, so), c);
+        }
+    } else if (nd == 2) {
+        const int sp = (m1.w[0] ^ m2.w[0]) ? 0 : 1;
+        const uint64_t x = m1.w[sp] ^ m2.w[sp];
+        const int po = __builtin_ctzll(x & m1.w[sp]) + 1, qo = __builtin_ctzll(x & m2.w[sp]) + 1;
+        const int nperm = __popcll(m1.w[sp] & bit_between(po, qo));
+        const double c = c1i * c2j * ((nperm & 1) ? -1.0 : 1.0);
+        uint64_t b = m1.w[sp] & ~(1ull << (po - 1));
+        while (b) {
+            const int ro = __builtin_ctzll(b) + 1;
+            b &= b - 1;
+            emit(TW(po, ro, ro, qo), c);
+            emit(TW(ro, po, qo, ro), c);
+            emit(TW(ro, po, ro, qo), -c);
+            emit(TW(po, ro, qo, ro), -c);
+        }
+        b = m1.w[1 - sp];
+        while (b) {
+            const int ro = __builtin_ctzll(b) + 1;
+            b &= b - 1;
+            emit(TW(po, ro, ro, qo), c);
+            emit(TW(ro, po, qo, ro), c);
+        }
+    }
+}
+
+__device__ __forceinline__ int64_t bit2_count(const DMask<2>& m1, const DMask<2>& m2, int nd,
+                                              bool diag) {
+    const int64_t na = __popcll(m1.w[0]), nb = __popcll(m1.w[1]);
+    if (diag) return 2 * (na * (na - 1) + nb * (nb - 1)) + 2 * na * nb;
+    if (nd == 4) return (__popcll(m1.w[0] ^ m2.w[0]) == 4 || __popcll(m1.w[1] ^ m2.w[1]) == 4) ? 4 : 2;
+    if (nd == 2) {
+        const int sp = (m1.w[0] ^ m2.w[0]) ? 0 : 1;
+        return 4 * (__popcll(m1.w[sp]) - 1) + 2 * (int64_t)__popcll(m1.w[1 - sp]);
+    }
+    return 0;
+}
+
+__global__ void k_bit2_ccount(const uint64_t* __restrict__ masks, const int32_t* __restrict__ c1s,
+                              const int32_t* __restrict__ c2s, const int8_t* __restrict__ nds,
+                              int64_t nitem, int ndiag, int64_t* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitem) return;
+    const DMask<2> m1 = load_mask<2>(masks, c1s[i]), m2 = load_mask<2>(masks, c2s[i]);
+    cnt[i] = bit2_count(m1, m2, nds[i], i < ndiag);
+}
+
+__global__ void k_bit2_cwrite(const uint64_t* __restrict__ masks, const int32_t* __restrict__ c1s,
+                              const int32_t* __restrict__ c2s, const int8_t* __restrict__ nds,
+                              int64_t nitem, int ndiag, const double* __restrict__ c1v,
+                              const double* __restrict__ c2v, const int64_t* __restrict__ coff,
+                              int N, uint32_t* __restrict__ keys, double* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitem) return;
+    const int c1 = c1s[i], c2 = c2s[i];
+    const DMask<2> m1 = load_mask<2>(masks, c1), m2 = load_mask<2>(masks, c2);
+    int64_t pos = coff[i];
+    const int64_t end = coff[i + 1];  // k_bit2_ccount's count for this item
+    bit2_contribs(m1, m2, nds[i], i < ndiag, c1v[c1], c2v[c2], N, [&](uint32_t k, double v) {
+        if (pos < end) {
+            keys[pos] = k;
+            vals[pos] = v;
+        }
+        ++pos;
+    });
+}
+
+// one_rdm_bit (src/one_rdm.f90:1440-1516): a determinant adds civs^2 on the
+// diagonal of each occupied orbital, alpha then beta; a pair c1 < c2 with one
+// difference adds civs(c1) civs(c2) ep at (p, q) and at (q, p).
+__global__ void k_bit1_ccount(const uint64_t* __restrict__ masks, const int32_t* __restrict__ c1s,
+                              const int32_t* __restrict__ c2s, const int8_t* __restrict__ nds,
+                              int64_t nitem, int ndiag, int64_t* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitem) return;
+    int64_t n = 0;
+    if (i < ndiag) {
+        const DMask<2> m1 = load_mask<2>(masks, c1s[i]);
+        n = __popcll(m1.w[0]) + __popcll(m1.w[1]);
+    } else if (nds[i] == 2 && c2s[i] > c1s[i]) {
+        n = 2;
+    }
+    cnt[i] = n;
+}
+
+__global__ void k_bit1_cwrite(const uint64_t* __restrict__ masks, const int32_t* __restrict__ c1s,
+                              const int32_t* __restrict__ c2s, const int8_t* __restrict__ nds,
+                              int64_t nitem, int ndiag, const double* __restrict__ civ,
+                              const int64_t* __restrict__ coff, int N,
+                              uint32_t* __restrict__ keys, double* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitem) return;
+    int64_t pos = coff[i];
+    const int64_t end = coff[i + 1];  // k_bit1_ccount's count for this item
+    const int c1 = c1s[i], c2 = c2s[i];
+    const DMask<2> m1 = load_mask<2>(masks, c1);
+    if (i < ndiag) {
+        const double v = civ[c1] * civ[c1];
+        for (int sp = 0; sp < 2; ++sp) {
+            uint64_t b = m1.w[sp];
+            while (b) {
+                const int j = __builtin_ctzll(b);
+                b &= b - 1;
+                if (pos < end) {
+                    keys[pos] = (uint32_t)(j + N * j);
+                    vals[pos] = v;
+                }
+                ++pos;
+            }
+        }
+    } else if (nds[i] == 2 && c2 > c1 && pos + 2 <= end) {
+        const DMask<2> m2 = load_mask<2>(masks, c2);
+        const int sp = (m1.w[0] ^ m2.w[0]) ? 0 : 1;
+        const uint64_t x = m1.w[sp] ^ m2.w[sp];
+        const int po = __builtin_ctzll(m1.w[sp] & x) + 1, qo = __builtin_ctzll(m2.w[sp] & x) + 1;
+        const int ep = __popcll(m1.w[sp] & bit_between(po, qo));
+        const double v = civ[c1] * civ[c2] * ((ep & 1) ? -1.0 : 1.0);
+        keys[pos] = (uint32_t)((po - 1) + N * (qo - 1));
+        vals[pos] = v;
+        keys[pos + 1] = (uint32_t)((qo - 1) + N * (po - 1));
+        vals[pos + 1] = v;
+    }
+}
+
+// mcci_to_bit's sign (src/one_rdm.f90:1920-1972): the parity of the
+// reordering of a determinant's slots from orbital order (alpha before beta
+// on each orbital) to alpha-then-beta order, i.e. of the number of
+// (beta j, alpha k) pairs with k > j.  Coefficients are multiplied by it.
+__global__ void k_mcci_phase(const uint64_t* __restrict__ masks, int n,
+                             const double* __restrict__ c1in, const double* __restrict__ c2in,
+                             double* __restrict__ c1out, double* __restrict__ c2out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t a = masks[2 * (size_t)i], b0 = masks[2 * (size_t)i + 1];
+    int inv = 0;
+    uint64_t b = b0;
+    while (b) {
+        const int j = __builtin_ctzll(b);
+        b &= b - 1;
+        inv += __popcll(a & ~below(j + 1));
+    }
+    const double ph = (inv & 1) ? -1.0 : 1.0;
+    c1out[i] = ph * c1in[i];
+    c2out[i] = ph * c2in[i];
+}
+
 inline unsigned nb64(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -559,5 +804,58 @@ hipError_t rdm_emit(const double* tw, int N, const uint8_t* flags, const int64_t
     const int64_t n4 = (int64_t)N * N * N * N;
     hipLaunchKernelGGL(k_rdm_emit, dim3(nb64(n4, 256)), dim3(256), 0, s, tw, N, flags, pos, rows,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t bit_pack(const int64_t* alpha, const int64_t* beta, int n, uint64_t* masks,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_bit_pack, dim3(nb64(n, 256)), dim3(256), 0, s, alpha, beta, n, masks);
+    return hipGetLastError();
+}
+
+hipError_t bit_diag(int n, int32_t* c1s, int32_t* c2s, int8_t* nds, hipStream_t s) {
+    hipLaunchKernelGGL(k_bit_diag, dim3(nb64(n, 256)), dim3(256), 0, s, n, c1s, c2s, nds);
+    return hipGetLastError();
+}
+
+hipError_t bit2_ccount(const uint64_t* masks, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t nitem, int ndiag, int64_t* cnt, hipStream_t s) {
+    if (nitem <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bit2_ccount, dim3(nb64(nitem, 256)), dim3(256), 0, s, masks, c1s, c2s, nds,
+                       nitem, ndiag, cnt);
+    return hipGetLastError();
+}
+
+hipError_t bit2_cwrite(const uint64_t* masks, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t nitem, int ndiag, const double* c1v,
+                       const double* c2v, const int64_t* coff, int N, uint32_t* keys, double* vals,
+                       hipStream_t s) {
+    if (nitem <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bit2_cwrite, dim3(nb64(nitem, 256)), dim3(256), 0, s, masks, c1s, c2s, nds,
+                       nitem, ndiag, c1v, c2v, coff, N, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t bit1_ccount(const uint64_t* masks, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t nitem, int ndiag, int64_t* cnt, hipStream_t s) {
+    if (nitem <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bit1_ccount, dim3(nb64(nitem, 256)), dim3(256), 0, s, masks, c1s, c2s, nds,
+                       nitem, ndiag, cnt);
+    return hipGetLastError();
+}
+
+hipError_t bit1_cwrite(const uint64_t* masks, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t nitem, int ndiag, const double* civ,
+                       const int64_t* coff, int N, uint32_t* keys, double* vals, hipStream_t s) {
+    if (nitem <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bit1_cwrite, dim3(nb64(nitem, 256)), dim3(256), 0, s, masks, c1s, c2s, nds,
+                       nitem, ndiag, civ, coff, N, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t mcci_phase(const uint64_t* masks, int n, const double* c1in, const double* c2in,
+                      double* c1out, double* c2out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mcci_phase, dim3(nb64(n, 256)), dim3(256), 0, s, masks, n, c1in, c2in,
+                       c1out, c2out);
     return hipGetLastError();
 }

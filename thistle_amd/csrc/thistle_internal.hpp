@@ -119,6 +119,26 @@ hipError_t rdm_flags(const double* tw, int N, double cutoff, uint8_t* flags, hip
 hipError_t rdm_emit(const double* tw, int N, const uint8_t* flags, const int64_t* pos,
                     int32_t* rows, double* out, hipStream_t s);
 
+// Bit-string builders (createtwordm_bit, one_rdm_bit, mcci_to_bit): masks
+// hold (alpha, beta) per determinant; items [0, ndiag) are the determinants'
+// no-difference passes, the rest the connected pairs of rdm_pairs (nw = 2).
+hipError_t bit_pack(const int64_t* alpha, const int64_t* beta, int n, uint64_t* masks,
+                    hipStream_t s);
+hipError_t bit_diag(int n, int32_t* c1s, int32_t* c2s, int8_t* nds, hipStream_t s);
+hipError_t bit2_ccount(const uint64_t* masks, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t nitem, int ndiag, int64_t* cnt, hipStream_t s);
+hipError_t bit2_cwrite(const uint64_t* masks, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t nitem, int ndiag, const double* c1v,
+                       const double* c2v, const int64_t* coff, int N, uint32_t* keys, double* vals,
+                       hipStream_t s);
+hipError_t bit1_ccount(const uint64_t* masks, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t nitem, int ndiag, int64_t* cnt, hipStream_t s);
+hipError_t bit1_cwrite(const uint64_t* masks, const int32_t* c1s, const int32_t* c2s,
+                       const int8_t* nds, int64_t nitem, int ndiag, const double* civ,
+                       const int64_t* coff, int N, uint32_t* keys, double* vals, hipStream_t s);
+hipError_t mcci_phase(const uint64_t* masks, int n, const double* c1in, const double* c2in,
+                      double* c1out, double* c2out, hipStream_t s);
+
 // Host helpers (rdm_host.cpp)
 int thistle_twordm_dense_host(const int32_t* confs, int nconf, int lconfs,
                               const double* civs, int nstates, int state1,

@@ -22,6 +22,9 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "a GTO has l, m or n > 4",
           8: "cutoffmd above THISTLE_CUTOFFMD_MAX (1e-18): the factorised contraction keeps "
              "every MD product"}
 CUTOFFMD_MAX = 1e-18
+# 1E-30 as a default real: the 2-RDM selection threshold of the reference's
+# builders (src/twordms.f90:514, src/one_rdm.f90:1312)
+RDM_CUTOFF = 1.0000000031710769e-30
 FP64, MIXED = 0, 1
 
 EXPORTS = [
@@ -29,6 +32,8 @@ EXPORTS = [
     "thistle_total_scattering_adjoint",
     "thistle_elastic_from_onerdm", "thistle_twordm_dense", "thistle_twordm_select",
     "thistle_createtwordm", "thistle_nuclei_electron",
+    "thistle_mcci_to_bit", "thistle_createtwordm_bit", "thistle_one_rdm_bit",
+    "thistle_total_scattering_calculation_2",
     "thistle_onerdm", "thistle_createonerdm", "thistle_quartet_list",
     "thistle_quartet_list_screened",
     "thistle_unique_total", "thistle_unique_total_noalloc", "thistle_pair_tables", "thistle_plan_create",
@@ -94,6 +99,14 @@ def _declare(L):
     L.thistle_twordm_select.restype = i64
     L.thistle_createtwordm.argtypes = [p, i32, i32, p, i32, i32, i32, d, p, p, p, i64]
     L.thistle_createtwordm.restype = i64
+    L.thistle_mcci_to_bit.argtypes = [p, p, p, p, i64, i32, p, p, p]
+    L.thistle_mcci_to_bit.restype = i32
+    L.thistle_createtwordm_bit.argtypes = [p, p, p, p, i64, d, p, p, p, i64]
+    L.thistle_createtwordm_bit.restype = i64
+    L.thistle_one_rdm_bit.argtypes = [p, p, p, i64, p, i32, p, p]
+    L.thistle_one_rdm_bit.restype = i32
+    L.thistle_total_scattering_calculation_2.argtypes = [i32, p, i64, p, p, p, p, i32, p, p, p]
+    L.thistle_total_scattering_calculation_2.restype = i32
     L.thistle_nuclei_electron.argtypes = [p, i32, p, i32, p, p, p, p]
     L.thistle_nuclei_electron.restype = i32
     L.thistle_onerdm.argtypes = [p, i32, i32, p, i32, i32, i32, p]
@@ -184,7 +197,7 @@ def make_basis(ga, l, m, n, xx, yy, zz, group, mmod, q, cutoffz, cutoffmd, cutof
     return b
 
 
-def make_opts(precision=FP64, rdm_cutoff=1e-30, stream=None, device=0, waves=0):
+def make_opts(precision=FP64, rdm_cutoff=RDM_CUTOFF, stream=None, device=0, waves=0):
     o = Opts()
     o.precision = int(precision)
     o.rdm_cutoff = float(rdm_cutoff)

@@ -478,6 +478,54 @@ def read_twordm(filename):
     return np.array(mats, dtype=np.int32).reshape(-1, 4), np.array(tots, dtype=np.float64)
 
 
+def read_onerdm(filename):
+    """``i j value`` rows (src/one_rdm.f90:2072-2108): maxnmo is the first
+    index of the last row; unlisted elements are 0."""
+    rows = []
+    with open(filename, "r") as fh:
+        for line in fh:
+            tok = line.split()
+            if tok:
+                rows.append((int(tok[0]), int(tok[1]), float(tok[2].replace("D", "E"))))
+    nmo = rows[-1][0] if rows else 0
+    g = np.zeros((nmo, nmo), order="F")
+    for i, j, v in rows:
+        g[i - 1, j - 1] = v
+    return g
+
+
+# -------------------------------------------------- bit-string CI files
+def read_bit_file(filename, numberlines=None):
+    """``i c1 c2 alpha beta`` lines, the layout mcci_to_bit and
+    createtwordm_bit read (src/one_rdm.f90:979, :1914): int64 alpha and beta
+    bit strings (orbital k on bit k-1) and float64 coefficient columns.
+    numberlines, when given, reads only that many lines, as the reference
+    does."""
+    al, be, c1, c2 = [], [], [], []
+    with open(filename, "r") as fh:
+        for line in fh:
+            if numberlines is not None and len(al) >= numberlines:
+                break
+            tok = line.replace(",", " ").split()
+            if not tok:
+                continue
+            c1.append(float(tok[1].replace("D", "E")))
+            c2.append(float(tok[2].replace("D", "E")))
+            al.append(int(tok[3]))
+            be.append(int(tok[4]))
+    if numberlines is not None and len(al) < numberlines:
+        raise ValueError(f"{filename}: {len(al)} lines, {numberlines} expected")
+    return (np.array(al, dtype=np.int64), np.array(be, dtype=np.int64),
+            np.array(c1, dtype=np.float64), np.array(c2, dtype=np.float64))
+
+
+def write_bit_file(filename, alpha, beta, c1, c2):
+    """The same layout, coefficients written to round-trip exactly."""
+    with open(filename, "w") as fh:
+        for i, (a, b, x, y) in enumerate(zip(alpha, beta, c1, c2)):
+            fh.write(f"{i + 1} {float(x)!r} {float(y)!r} {int(a)} {int(b)}\n")
+
+
 def write_twordm(filename, mat, total):
     """Same column layout as twordm_fortran.dat (src/twordms.f90:521-539)."""
     with open(filename, "w") as fh:

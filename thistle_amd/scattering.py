@@ -31,7 +31,7 @@ def _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcen
 def total_scattering_calculation(type, Zn, geom, state1, state2, state3, maxl, ngto, ng,
                                  ga, l, m, n, xx, yy, zz, mmod, q, nq, group, cutoffz,
                                  cutoffmd, cutoffcentre, confs, civecs, *, precision=L.FP64,
-                                 rdm_cutoff=1e-30, waves=0):
+                                 rdm_cutoff=L.RDM_CUTOFF, waves=0):
     """Drop-in for main_calculation_mod::total_scattering_calculation (types 1, 2,
     5, 6, 7 and 8)."""
     lib = L.load()
@@ -79,11 +79,12 @@ def total_from_twordm(ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total, cutof
 
 
 def total_j2_from_twordm(ga, l, m, n, xx, yy, zz, mmod, q, group, mat, total, cutoffz=1e-15,
-                         cutoffmd=1e-20, cutoffcentre=0.001, *, precision=L.FP64, waves=0):
+                         cutoffmd=1e-20, cutoffcentre=0.001, *, precision=L.FP64, waves=0,
+                         maxl=None):
     """Total j2 S(q) (type 7, tot_integration_j2) from 2-RDM rows (p,s,q,r) + values."""
     plan = Plan(7, ga, l, m, n, xx, yy, zz, mmod, q, group, mat=mat, total=total,
                 cutoffz=cutoffz, cutoffmd=cutoffmd, cutoffcentre=cutoffcentre,
-                precision=precision, waves=waves)
+                precision=precision, waves=waves, maxl=maxl)
     try:
         return plan.execute()
     finally:
@@ -172,7 +173,7 @@ def elastic_j2_from_onerdm(ga, l, m, n, xx, yy, zz, mmod, q, group, onerdm, cuto
 
 
 # --------------------------------------------------- 2-RDM / 1-RDM
-def createtwordm(confs, civs, state1=1, state2=1, cutoff=1e-30, device=None):
+def createtwordm(confs, civs, state1=1, state2=1, cutoff=L.RDM_CUTOFF, device=None):
     """maxcoincidence + createtwordm: rows (p,s,q,r) and values (src/twordms.f90).
 
     device=None builds it on the GPU when one is present (thistle_createtwordm)
@@ -238,6 +239,113 @@ def onerdm(confs, civs, state1=1, state2=1, device=None):
         L.check(lib.thistle_onerdm(L._ptr(confs), confs.shape[0], confs.shape[1], L._ptr(civs),
                                    civs.shape[1], state1, state2, L._ptr(g)), "onerdm")
     return g
+
+
+# ------------------------------------------------- bit-string determinants
+def _bits(alpha, beta, *coefs):
+    a = np.require(np.asarray(alpha), dtype=np.int64, requirements=["C"])
+    b = np.require(np.asarray(beta), dtype=np.int64, requirements=["C"])
+    cs = [np.require(np.asarray(c, dtype=np.float64), requirements=["C"]) for c in coefs]
+    if a.ndim != 1 or a.shape != b.shape or any(c.shape != a.shape for c in cs):
+        raise ValueError("alpha, beta and the coefficients must be 1-D of one length")
+    return (a, b, *cs)
+
+
+def mcci_to_bit(alpha, beta, c1, c2, text_round=True):
+    """mcci_to_bit (src/one_rdm.f90:1901-1984) on the GPU: each coefficient
+    times the sign of reordering its determinant's slots from orbital order
+    to alpha-then-beta order; text_round applies the E20.10 write / read that
+    hands them to createtwordm_bit."""
+    lib = L.load()
+    a, b, c1, c2 = _bits(alpha, beta, c1, c2)
+    o1, o2 = np.zeros(a.size), np.zeros(a.size)
+    L.check(lib.thistle_mcci_to_bit(L._ptr(a), L._ptr(b), L._ptr(c1), L._ptr(c2), a.size,
+                                    1 if text_round else 0, None, L._ptr(o1), L._ptr(o2)),
+            "mcci_to_bit")
+    return o1, o2
+
+
+def createtwordm_bit(alpha, beta, c1, c2, cutoff=L.RDM_CUTOFF):
+    """createtwordm_bit (src/one_rdm.f90:931-1353) on the GPU: rows (p,s,q,r)
+    and values, bit-identical to the reference's."""
+    lib = L.load()
+    a, b, c1, c2 = _bits(alpha, beta, c1, c2)
+    args = (L._ptr(a), L._ptr(b), L._ptr(c1), L._ptr(c2), a.size, cutoff, None)
+    cnt = lib.thistle_createtwordm_bit(*args, None, None, 0)
+    if cnt < 0:
+        L.check(-cnt, "createtwordm_bit")
+    mat = np.zeros((max(cnt, 1), 4), dtype=np.int32, order="F")
+    tot = np.zeros(max(cnt, 1))
+    c2n = lib.thistle_createtwordm_bit(*args, L._ptr(mat), L._ptr(tot), cnt)
+    if c2n < 0:
+        L.check(-c2n, "createtwordm_bit")
+    return np.ascontiguousarray(mat[:cnt]), tot[:cnt].copy()
+
+
+def one_rdm_bit(alpha, beta, civs):
+    """one_rdm_bit (src/one_rdm.f90:1388-1537) on the GPU: the maxnmo x maxnmo
+    1-RDM; civs is the file's third column."""
+    lib = L.load()
+    a, b, c = _bits(alpha, beta, civs)
+    norb = C.c_int32(0)
+    L.check(lib.thistle_one_rdm_bit(L._ptr(a), L._ptr(b), L._ptr(c), a.size, None, 0,
+                                    C.byref(norb), None), "one_rdm_bit")
+    g = np.zeros((norb.value, norb.value), order="F")
+    L.check(lib.thistle_one_rdm_bit(L._ptr(a), L._ptr(b), L._ptr(c), a.size, None, norb.value,
+                                    C.byref(norb), L._ptr(g)), "one_rdm_bit")
+    return g
+
+
+def total_scattering_calculation_2(type, Zn, geom, state1, state2, maxl, ngto, ng, ga, l, m,
+                                   n, xx, yy, zz, mmod, q, nq, group, cutoffz, cutoffmd,
+                                   cutoffcentre, fileJ, fileJ_2, numberlines, newdat, start,
+                                   end, start_2, end_2, fci, irep1, irep2, ordering1, ordering2,
+                                   readrdm, mcci, *, precision=L.FP64, waves=0):
+    """Drop-in for main_calculation_mod::total_scattering_calculation_2
+    (src/main_calculation.f90:501-849), types 1, 2, 7 and 8.
+
+    fileJ is the ``i c1 c2 alpha beta`` file (numberlines lines); with readrdm
+    types 1 / 7 read the 2-RDM file fileJ_2 and types 2 / 8 the 1-RDM file
+    fileJ instead.  newdat, start, end, start_2, end_2, irep1, irep2 and the
+    orderings feed only createtwordm_bit_fci, which both of the reference's
+    callers switch off (fci = False at src/scattering_main_routine.py:251 and
+    src/Main.f90:142,179); fci=True raises.  Returns (q_abs, result)."""
+    from . import readers
+    lib = L.load()
+    q = np.asarray(q, dtype=np.float64)
+    if int(nq) != q.size or int(ngto) != np.asarray(l).size:
+        raise ValueError("nq / ngto do not match the array sizes")
+    if int(type) not in (1, 2, 7, 8):
+        raise NotImplementedError(
+            f"type {type}: only 1 (total j0), 2 (elastic j0), 7 (total j2) and 8 (elastic j2) "
+            "are on this path")
+    q_abs = np.zeros(q.size)
+    kw = dict(cutoffz=cutoffz, cutoffmd=cutoffmd, cutoffcentre=cutoffcentre,
+              precision=precision, waves=waves, maxl=maxl)
+    basis = (ga, l, m, n, xx, yy, zz, mmod, q, group)
+    if readrdm:
+        if int(type) in (1, 7):
+            mat, tot = readers.read_twordm(fileJ_2)
+            f = total_from_twordm if int(type) == 1 else total_j2_from_twordm
+            return q_abs, f(*basis, mat, tot, **kw)
+        g = readers.read_onerdm(fileJ)
+        f = elastic_from_onerdm if int(type) == 2 else elastic_j2_from_onerdm
+        return q_abs, f(*basis, g, **kw)
+    if fci:
+        raise NotImplementedError(
+            "createtwordm_bit_fci: the reference's callers never set fci "
+            "(src/scattering_main_routine.py:251, src/Main.f90:142,179)")
+    a, b, c1, c2 = readers.read_bit_file(fileJ, int(numberlines))
+    a, b, c1, c2 = _bits(a, b, c1, c2)
+    bb = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
+                maxl=maxl, ng=ng)
+    o = L.make_opts(precision, waves=waves)
+    res = np.zeros(q.size)
+    rc = lib.thistle_total_scattering_calculation_2(
+        int(type), C.byref(bb), a.size, L._ptr(a), L._ptr(b), L._ptr(c1), L._ptr(c2),
+        1 if mcci else 0, C.byref(o), L._ptr(q_abs), L._ptr(res))
+    L.check(rc, "total_scattering_calculation_2")
+    return q_abs, res
 
 
 # ----------------------------------------------------------- device stages
