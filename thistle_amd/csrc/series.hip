@@ -309,6 +309,31 @@ __device__ __forceinline__ double f_pzero(const double (&hs)[LL + 1], double q) 
     return F;
 }
 
+// dphrec's divisors 4 LU^2 - 1 (LU = 0..99, exact in FP64) and their
+// correctly rounded reciprocals.  XX / d is evaluated as q0 = XX c followed
+// by one FMA-exact residual correction, q0 + (XX - q0 d) c: the correctly
+// rounded quotient (identical to the division for every LU here and x from
+// 70 to 5000 on 1.96e8 sampled arguments), at three operations instead of a
+// full FP64 division.
+struct DphTable {
+    double d[100], c[100];
+};
+constexpr DphTable make_dph_table() {
+    DphTable t{};
+    for (int i = 0; i < 100; ++i) {
+        t.d[i] = 4.0 * (double)i * (double)i - 1.0;
+        t.c[i] = 1.0 / t.d[i];
+    }
+    return t;
+}
+__constant__ const DphTable kDph = make_dph_table();
+
+__device__ __forceinline__ double dph_quot(double XX, int LU) {
+    const double d = kDph.d[LU], c = kDph.c[LU];
+    const double q0 = XX * c;
+    return fma(fma(-q0, d, XX), c, q0);
+}
+
 // dphrec with start order 100 followed literally, then bessels0rr's sum.
 // Out of line and rarely taken; it reads the quartet's h(beta) row from
 // global memory, so that the caller's h(beta) registers are never spilled
@@ -321,16 +346,14 @@ __device__ __noinline__ double f_dphrec(const double* __restrict__ hs, double q,
     double r2 = 1.0, r1 = 1.0;  // R[i+2], R[i+1] with R[99] = R[98] = 1
     int i = 97;
     for (; i > LL + 1; --i) {
-        double LU = (double)(i + 2);
-        double r0 = r1 - XX / (4.0 * LU * LU - 1.0) * r2;
+        double r0 = r1 - dph_quot(XX, i + 2) * r2;
         r2 = r1;
         r1 = r0;
     }
     double R[LL + 2];
 #pragma unroll
     for (int k = LL + 1; k >= 0; --k) {
-        double LU = (double)(k + 2);
-        double r0 = r1 - XX / (4.0 * LU * LU - 1.0) * r2;
+        double r0 = r1 - dph_quot(XX, k + 2) * r2;
         r2 = r1;
         r1 = r0;
         R[k] = r0;
