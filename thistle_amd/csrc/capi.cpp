@@ -648,6 +648,8 @@ SeriesArgs series_args(const ThistlePlan* pl) {
     a.comboff = pl->comboff.as<int64_t>();
     a.nsig = pl->nsig;
     a.nqc = std::min(512, pl->nq);
+    a.qs0 = 0;
+    a.qs1 = a.nqc;
     a.qlin = pl->qlin;
     a.q0off = pl->q0off;
     a.qa = pl->qa;
@@ -803,7 +805,7 @@ int set_quartets(ThistlePlan* pl, const int64_t* pos, int64_t n) {
 // Every device stage of one execution, enqueued on the plan's stream with no
 // host synchronisation, memset or (unless timing) event: safe under stream
 // capture.  S_dst (device, optional) receives a copy of S(q).
-int enqueue_execute(ThistlePlan* pl, bool timing, double* S_dst) {
+int enqueue_execute(ThistlePlan* pl, bool timing, double* S_dst, bool quartets = true) {
     if (timing) HCHECK(hipEventRecord(pl->ev[0], pl->stream));
     int rc = run_tables_and_enum(pl, timing);
     if (rc) return rc;
@@ -815,6 +817,7 @@ int enqueue_execute(ThistlePlan* pl, bool timing, double* S_dst) {
                           pl->ngto, pl->Z.as<double>(), pl->stream));
     }
     if (timing) HCHECK(hipEventRecord(pl->ev[3], pl->stream));
+    if (!quartets) return THISTLE_OK;  // a batch runs the quartet stage across frames
     rc = run_series(pl, timing);
     if (rc) return rc;
     if (S_dst) HCHECK(launch_copy(pl->S.as<double>(), S_dst, pl->nq, pl->stream));
@@ -1161,12 +1164,34 @@ struct ThistleBatch {
     hipGraphExec_t exec = nullptr;
     bool tuned = false, graph_ok = false;
     float ms = 0.f;
+    // Cross-frame quartet stage (FP64, every q grid within one 512-point
+    // chunk): per LL class one contraction launch and at most three series
+    // and three far-regime launches over all frames, driven by work tables;
+    // one zeroing and one reduction launch for every frame's partial rows.
+    bool merged = false;
+    int rows = 0;  // partial rows (series chunks) per frame, tuned
+    struct SLaunch {
+        int LL, sq;
+        bool far;
+        int64_t off, n, blocks;
+    };
+    DevBuf dargs, dcw, dsw, dfill, dred;
+    int64_t cw_off[18] = {}, cw_blocks[17] = {};
+    int cls_tmax[17] = {};
+    std::vector<SLaunch> slaunch;
+    int nfill = 0, nred = 0;
+    int64_t fill_blocks = 0, red_blocks = 0;
+    hipEvent_t cev[17] = {}, fill_ev = nullptr, ser_ev = nullptr;
     ~ThistleBatch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         plans.clear();  // plans use the batch streams
         for (auto& e : join)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : cev)
+            if (e) (void)hipEventDestroy(e);
+        if (fill_ev) (void)hipEventDestroy(fill_ev);
+        if (ser_ev) (void)hipEventDestroy(ser_ev);
         if (fork) (void)hipEventDestroy(fork);
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
@@ -1184,7 +1209,145 @@ void batch_set_waves(ThistleBatch* b, int per_cu) {
             1, std::min<int64_t>({(int64_t)per_cu * b->cus, p->waves_alloc, p->nquart}));
 }
 
+// Work tables of the cross-frame quartet stage for `rows` partial rows per
+// frame (a frame's class of n records is split into min(rows, n) chunks,
+// as a single-frame execution with waves = rows splits it).
+int batch_build_tables(ThistleBatch* b, int rows) {
+    b->rows = rows;
+    const int nf = (int)b->plans.size();
+    std::vector<SeriesArgs> args((size_t)nf);
+    std::vector<ClassWork> cw;
+    std::vector<SeriesWork> sw;
+    std::vector<FillWork> fw;
+    std::vector<RedWork> rw;
+    int64_t fb = 0, rb = 0;
+    const int rq = reduce_q_per_block();
+    for (int f = 0; f < nf; ++f) {
+        ThistlePlan* p = b->plans[f].get();
+        p->waves = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)rows, p->waves_alloc,
+                                                                p->nquart}));
+        args[f] = series_args(p);
+        args[f].q0 = 0;
+        args[f].nqc = p->nq;
+        const int64_t len = (int64_t)p->waves * p->nq;
+        fw.push_back({fb, p->part.as<double>(), len});
+        fb += (len + 255) / 256;
+        fw.push_back({fb, p->partc.as<double>(), len});
+        fb += (len + 255) / 256;
+        rw.push_back({rb, p->part.as<double>(), p->partc.as<double>(),
+                      b->S.as<double>() + b->soff[f], p->waves, p->nq});
+        rb += (p->nq + rq - 1) / rq;
+    }
+    b->slaunch.clear();
+    for (int LL = 0; LL <= 16; ++LL) {
+        b->cw_off[LL] = (int64_t)cw.size();
+        const int ng = contract_groups_per_block(LL);
+        int64_t blocks = 0;
+        int tmax = 0;
+        for (int f = 0; f < nf; ++f) {
+            ThistlePlan* p = b->plans[f].get();
+            const int64_t n = p->cls_off[LL + 1] - p->cls_off[LL];
+            if (n <= 0) continue;
+            cw.push_back({blocks, n, p->cls.as<int32_t>() + p->cls_off[LL],
+                          p->hs.as<double>() + p->cls_hoff[LL], f});
+            blocks += (n + ng - 1) / ng;
+            tmax = std::max(tmax, p->cls_tmax[LL]);
+        }
+        b->cw_blocks[LL] = blocks;
+        b->cls_tmax[LL] = tmax;
+        // series launches in the order a single frame issues them: the
+        // 256-point groups (q slices of 4), the tail, then the far regime
+        for (int far = 0; far < 2; ++far)
+            for (int sq : {4, 2, 1}) {
+                const int64_t off = (int64_t)sw.size();
+                int64_t sb = 0;
+                for (int f = 0; f < nf; ++f) {
+                    ThistlePlan* p = b->plans[f].get();
+                    const int64_t n = p->cls_off[LL + 1] - p->cls_off[LL];
+                    if (n <= 0 || (far && !p->far)) continue;
+                    int qs0[2], qs1[2], sqs[2];
+                    const int np = series_q_split(p->nq, false, qs0, qs1, sqs);
+                    for (int k = 0; k < np; ++k) {
+                        if (sqs[k] != sq) continue;
+                        const int nchunk = (int)std::min<int64_t>(p->waves, n);
+                        const int ny = (qs1[k] - qs0[k] + sq * 64 - 1) / (sq * 64);
+                        sw.push_back({sb, n, p->hs.as<double>() + p->cls_hoff[LL], f, nchunk, ny,
+                                      qs0[k], qs1[k], 0});
+                        sb += (int64_t)nchunk * ny;
+                    }
+                }
+                if (sb > 0)
+                    b->slaunch.push_back({LL, sq, far != 0, off, (int64_t)sw.size() - off, sb});
+            }
+    }
+    b->cw_off[17] = (int64_t)cw.size();
+    b->nfill = (int)fw.size();
+    b->fill_blocks = fb;
+    b->nred = (int)rw.size();
+    b->red_blocks = rb;
+    int rc;
+    if ((rc = dalloc(b->dargs, sizeof(SeriesArgs) * std::max<size_t>(args.size(), 1))) ||
+        (rc = dalloc(b->dcw, sizeof(ClassWork) * std::max<size_t>(cw.size(), 1))) ||
+        (rc = dalloc(b->dsw, sizeof(SeriesWork) * std::max<size_t>(sw.size(), 1))) ||
+        (rc = dalloc(b->dfill, sizeof(FillWork) * std::max<size_t>(fw.size(), 1))) ||
+        (rc = dalloc(b->dred, sizeof(RedWork) * std::max<size_t>(rw.size(), 1))))
+        return rc;
+    HCHECK(hipMemcpy(b->dargs.p, args.data(), sizeof(SeriesArgs) * args.size(),
+                     hipMemcpyHostToDevice));
+    if (!cw.empty())
+        HCHECK(hipMemcpy(b->dcw.p, cw.data(), sizeof(ClassWork) * cw.size(),
+                         hipMemcpyHostToDevice));
+    if (!sw.empty())
+        HCHECK(hipMemcpy(b->dsw.p, sw.data(), sizeof(SeriesWork) * sw.size(),
+                         hipMemcpyHostToDevice));
+    HCHECK(hipMemcpy(b->dfill.p, fw.data(), sizeof(FillWork) * fw.size(), hipMemcpyHostToDevice));
+    HCHECK(hipMemcpy(b->dred.p, rw.data(), sizeof(RedWork) * rw.size(), hipMemcpyHostToDevice));
+    return THISTLE_OK;
+}
+
+// Per-frame set-up stages on the batch streams, then the quartet stage of
+// every frame in a few launches: contractions on stream 0, series (behind
+// each class's contraction) and the reduction on stream 1.
+int batch_enqueue_merged(ThistleBatch* b) {
+    hipStream_t s0 = b->streams[0], s1 = b->streams[1];
+    HCHECK(hipEventRecord(b->fork, s0));
+    for (int s = 1; s < ThistleBatch::NS; ++s) HCHECK(hipStreamWaitEvent(b->streams[s], b->fork, 0));
+    for (size_t f = 0; f < b->plans.size(); ++f) {
+        int rc = enqueue_execute(b->plans[f].get(), false, nullptr, false);
+        if (rc) return rc;
+    }
+    for (int s = 1; s < ThistleBatch::NS; ++s) {
+        HCHECK(hipEventRecord(b->join[s], b->streams[s]));
+        HCHECK(hipStreamWaitEvent(s0, b->join[s], 0));
+    }
+    HCHECK(launch_fill_multi(b->dfill.as<FillWork>(), b->nfill, b->fill_blocks, s0));
+    HCHECK(hipEventRecord(b->fill_ev, s0));
+    HCHECK(hipStreamWaitEvent(s1, b->fill_ev, 0));
+    const SeriesArgs* args = b->dargs.as<SeriesArgs>();
+    for (int LL = 0; LL <= 16; ++LL) {
+        const int nw = (int)(b->cw_off[LL + 1] - b->cw_off[LL]);
+        if (nw <= 0) continue;
+        HCHECK(launch_contract_multi(args, b->dcw.as<ClassWork>() + b->cw_off[LL], nw,
+                                     b->cw_blocks[LL], LL, b->cls_tmax[LL], false, s0));
+        HCHECK(hipEventRecord(b->cev[LL], s0));
+    }
+    int waited = -1;
+    for (const auto& L : b->slaunch) {
+        if (L.LL != waited) {
+            HCHECK(hipStreamWaitEvent(s1, b->cev[L.LL], 0));
+            waited = L.LL;
+        }
+        HCHECK(launch_series_multi(args, b->dsw.as<SeriesWork>() + L.off, (int)L.n, L.blocks,
+                                   L.LL, L.sq, false, L.far, s1));
+    }
+    HCHECK(launch_reduce_multi(b->dred.as<RedWork>(), b->nred, b->red_blocks, s1));
+    HCHECK(hipEventRecord(b->ser_ev, s1));
+    HCHECK(hipStreamWaitEvent(s0, b->ser_ev, 0));
+    return THISTLE_OK;
+}
+
 int batch_enqueue(ThistleBatch* b) {
+    if (b->merged) return batch_enqueue_merged(b);
     HCHECK(hipEventRecord(b->fork, b->streams[0]));
     for (int s = 1; s < ThistleBatch::NS; ++s) HCHECK(hipStreamWaitEvent(b->streams[s], b->fork, 0));
     for (size_t f = 0; f < b->plans.size(); ++f) {
@@ -1208,13 +1371,22 @@ int batch_timed(ThistleBatch* b, float& ms) {
     return THISTLE_OK;
 }
 
-// autotune on eager executions, then capture the chosen configuration
+// Autotune on eager executions, then capture the chosen configuration.
+// Per-frame launches: series waves per CU.  Cross-frame launches: partial
+// rows per frame, i.e. records per series wave (fewer rows, longer chunks
+// and a shorter reduction; more rows, more waves in flight).
 int batch_prepare(ThistleBatch* b) {
-    const int cand[3] = {4, 8, 12};
+    const std::vector<int> cand = b->merged ? std::vector<int>{32, 128, 512, 2048}
+                                            : std::vector<int>{4, 8, 12};
     float best = 0.f;
     int pick = cand[0];
     for (int c : cand) {
-        batch_set_waves(b, c);
+        if (b->merged) {
+            int rc = batch_build_tables(b, c);
+            if (rc) return rc;
+        } else {
+            batch_set_waves(b, c);
+        }
         float ms = 0.f;
         int rc = batch_timed(b, ms);  // the first candidate also warms up rocBLAS
         if (rc) return rc;
@@ -1225,7 +1397,12 @@ int batch_prepare(ThistleBatch* b) {
             pick = c;
         }
     }
-    batch_set_waves(b, pick);
+    if (b->merged) {
+        int rc = batch_build_tables(b, pick);
+        if (rc) return rc;
+    } else {
+        batch_set_waves(b, pick);
+    }
     b->tuned = true;
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(b->streams[0], hipStreamCaptureModeGlobal) != hipSuccess)
@@ -1287,6 +1464,15 @@ int32_t thistle_batch_create(int32_t nframes, const ThistleBasis* bases, const i
     }
     int rc = dalloc(b->S, sizeof(double) * b->nqtot);
     if (rc) return rc;
+    // the cross-frame quartet stage: FP64 frames whose q grids fit one chunk
+    b->merged = o->precision == THISTLE_FP64;
+    for (int f = 0; f < nframes; ++f) b->merged = b->merged && bases[f].nq <= 512;
+    if (b->merged) {
+        for (auto& e : b->cev) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HCHECK(hipEventCreateWithFlags(&b->fill_ev, hipEventDisableTiming));
+        HCHECK(hipEventCreateWithFlags(&b->ser_ev, hipEventDisableTiming));
+        if ((rc = batch_build_tables(b.get(), 512))) return rc;
+    }
     *out = b.release();
     return THISTLE_OK;
 }
@@ -1319,7 +1505,7 @@ int32_t thistle_batch_info(ThistleBatch* b, double* ms_last, int64_t* units, int
     if (!b) return THISTLE_EINVAL;
     if (ms_last) *ms_last = b->ms;
     if (units) *units = b->units;
-    if (waves) *waves = b->waves_per_cu * b->cus;
+    if (waves) *waves = b->merged ? b->rows : b->waves_per_cu * b->cus;
     if (graph) *graph = b->graph_ok ? 1 : 0;
     return THISTLE_OK;
 }

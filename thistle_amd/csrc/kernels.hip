@@ -811,13 +811,12 @@ __global__ void k_to_float(const double* __restrict__ in, float* __restrict__ ou
 // thread, and the segment sums are combined in segment order.
 #define RED_SEG 32
 #define RED_Q 8
-__global__ void __launch_bounds__(RED_SEG * RED_Q) k_reduce(const double* __restrict__ part,
-                                                           const double* __restrict__ partc,
-                                                           int waves, int nqc,
-                                                           double* __restrict__ out) {
+__device__ __forceinline__ void reduce_block(const double* __restrict__ part,
+                                             const double* __restrict__ partc, int waves,
+                                             int nqc, double* __restrict__ out, int64_t qb) {
     __shared__ double ss[RED_SEG][RED_Q], sc[RED_SEG][RED_Q];
     const int ql = threadIdx.x % RED_Q, seg = threadIdx.x / RED_Q;
-    const int iq = blockIdx.x * RED_Q + ql;
+    const int iq = (int)(qb * RED_Q) + ql;
     double s = 0.0, c = 0.0;
     if (iq < nqc) {
         const int w0 = (int)((int64_t)seg * waves / RED_SEG),
@@ -838,6 +837,38 @@ __global__ void __launch_bounds__(RED_SEG * RED_Q) k_reduce(const double* __rest
         }
         out[iq] = S;
     }
+}
+
+__global__ void __launch_bounds__(RED_SEG * RED_Q) k_reduce(const double* __restrict__ part,
+                                                           const double* __restrict__ partc,
+                                                           int waves, int nqc,
+                                                           double* __restrict__ out) {
+    reduce_block(part, partc, waves, nqc, out, blockIdx.x);
+}
+
+template <class W>
+__device__ __forceinline__ const W& find_entry(const W* __restrict__ work, int nwork, int64_t b) {
+    int lo = 0, hi = nwork - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (work[mid].blk0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return work[lo];
+}
+
+// k_reduce of many frames in one launch (the same order of additions)
+__global__ void __launch_bounds__(RED_SEG * RED_Q)
+    k_reduce_multi(const RedWork* __restrict__ work, int nwork) {
+    const RedWork& W = find_entry(work, nwork, blockIdx.x);
+    reduce_block(W.part, W.partc, W.waves, W.nqc, W.out, (int64_t)blockIdx.x - W.blk0);
+}
+
+// zero many buffers in one launch (256 doubles per block)
+__global__ void k_fill_multi(const FillWork* __restrict__ work, int nwork) {
+    const FillWork& W = find_entry(work, nwork, blockIdx.x);
+    const int64_t i = ((int64_t)blockIdx.x - W.blk0) * 256 + threadIdx.x;
+    if (i < W.n) W.p[i] = 0.0;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1000,5 +1031,20 @@ hipError_t launch_copy(const double* in, double* out, int64_t n, hipStream_t s) 
 
 hipError_t launch_to_float(const double* in, float* out, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_to_float, dim3(nblk(n, 256)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_multi(const RedWork* work, int nwork, int64_t nblocks, hipStream_t s) {
+    if (nwork <= 0 || nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_multi, dim3((unsigned)nblocks), dim3(RED_SEG * RED_Q), 0, s, work,
+                       nwork);
+    return hipGetLastError();
+}
+
+int reduce_q_per_block() { return RED_Q; }
+
+hipError_t launch_fill_multi(const FillWork* work, int nwork, int64_t nblocks, hipStream_t s) {
+    if (nwork <= 0 || nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_multi, dim3((unsigned)nblocks), dim3(256), 0, s, work, nwork);
     return hipGetLastError();
 }

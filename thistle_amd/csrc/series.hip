@@ -33,6 +33,7 @@
 #include "series_math.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 
 // ------------------------------------------------------------ contraction
@@ -43,9 +44,12 @@
 //           factorised like the j0 case: the rrdj2b weight 2 ceil((l+p)/2)
 //           is 2 m per axis, and -2 on z)
 //   h = 0:  c0(b) = sqrt(5/pi)/2 (b h(b) - 3 g(b)),  b <= LL  (P0_j2, :3034-3170)
+// Quartet groups g0, g0 + gstep, ... (NG quartets each) of a class list idx
+// (nidx entries) whose records go to hsb.
 template <int LL, bool J2>
-__global__ void __launch_bounds__(ContractShape<LL>::CT)
-    k_contract(SeriesArgs A, const int32_t* __restrict__ idx, int64_t nidx, int tmax) {
+__device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __restrict__ hsb,
+                                                const int32_t* __restrict__ idx, int64_t nidx,
+                                                int tmax, int64_t g0, int64_t gstep) {
     constexpr int D = LL + 1;
     constexpr int NA = J2 ? 2 : 1;  // accumulated vectors: h, and g for j2
     constexpr int CT = ContractShape<LL>::CT, G = ContractShape<LL>::G, NG = ContractShape<LL>::NG;
@@ -62,7 +66,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
     // grid-stride over the class list: neighbouring quartets go to
     // neighbouring workgroups, hence to different XCDs (measured faster on
     // XY6 than contiguous runs per workgroup or per XCD)
-    for (int64_t base = (int64_t)blockIdx.x * NG; base < nidx; base += (int64_t)gridDim.x * NG) {
+    for (int64_t base = g0 * NG; base < nidx; base += gstep * NG) {
         const int64_t kq = base + grp;
         const bool valid = kq < nidx;
         const int64_t e = idx[valid ? kq : base];
@@ -206,7 +210,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
         // the values of h(beta) (j0) or of the j2 series / h = 0 polynomial
         // coefficients (DO of them), given the reduced sums in hr / gr
         constexpr int DO = J2 ? D + 2 : D;
-        double* rec = A.hs + kq * HS_ROW(DO);
+        double* rec = hsb + kq * HS_ROW(DO);
         // the series kernel's per-quartet scalars, written by one lane
         auto tail = [&]() {
             const double hq = sqrt(rx * rx + ry * ry + rz * rz);
@@ -291,6 +295,36 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
             }
         }
     }
+}
+
+template <int LL, bool J2>
+__global__ void __launch_bounds__(ContractShape<LL>::CT)
+    k_contract(SeriesArgs A, const int32_t* __restrict__ idx, int64_t nidx, int tmax) {
+    contract_groups<LL, J2>(A, A.hs, idx, nidx, tmax, blockIdx.x, gridDim.x);
+}
+
+// The entry of a cross-frame work table that holds block b: the last one
+// whose first block is <= b (entries in increasing first block).
+template <class W>
+__device__ __forceinline__ const W& find_work(const W* __restrict__ work, int nwork, int64_t b) {
+    int lo = 0, hi = nwork - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (work[mid].blk0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return work[lo];
+}
+
+// One class of many frames in one launch: each frame's quartet groups take
+// consecutive blocks (one group per block), so a block's frame is uniform.
+template <int LL, bool J2>
+__global__ void __launch_bounds__(ContractShape<LL>::CT)
+    k_contract_multi(const SeriesArgs* __restrict__ args, const ClassWork* __restrict__ work,
+                     int nwork, int tmax) {
+    const ClassWork& W = find_work(work, nwork, blockIdx.x);
+    contract_groups<LL, J2>(args[W.frame], W.hs, W.idx, W.n, tmax,
+                            (int64_t)blockIdx.x - W.blk0, (int64_t)1 << 40);
 }
 
 // ------------------------------------------------------------------ series
@@ -458,7 +492,8 @@ __device__ __forceinline__ float f_up32(const float (&hf)[LL + 1], const float (
     return F;
 }
 
-#define SQ 4   // q slices (of 64 points) per wave
+// q slices (of 64 points) per wave: SQ = 4 on the full 256-point groups of a
+// chunk, 1 or 2 on its tail, so that short grids do not pad to 256 points
 
 // h = 0 polynomial of total j2: sum_b c(b) q^(2b) / (2b+3)!!
 template <int LL>
@@ -535,8 +570,13 @@ __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const doubl
     }
 }
 
-template <int LL, bool MIXED, bool J2>
-__global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int first) {
+// Chunk `chunk` of nchunk (the records of one wave) of a class whose records
+// are hsb (nidx of them), on q slice group ys of the points [qs0, qs1) of
+// the chunk of q; the sums go to partial row `chunk`.
+template <int LL, bool MIXED, bool J2, int SQ>
+__device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* __restrict__ hsb,
+                                             int64_t nidx, int first, int64_t chunk,
+                                             int64_t nchunk, int ys, int qs0, int qs1) {
     constexpr int R = HS_ROW(LL + 1);
     constexpr int NB = 64;  // quartet records staged in LDS per batch
     __shared__ double recs[NB * R];
@@ -546,26 +586,25 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
     double qv[SQ], qiv[SQ], S[SQ], Sc[SQ], Sa2[SQ];
 #pragma unroll
     for (int s = 0; s < SQ; ++s) {
-        const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
-        iqs[s] = A.q0 + (iql < A.nqc ? iql : 0);
+        const int iql = qs0 + (ys * SQ + s) * WAVE + lane;
+        iqs[s] = A.q0 + (iql < qs1 ? iql : qs0);
         qv[s] = A.q[iqs[s]];
         qiv[s] = qv[s] > 0.0 ? rcp64(qv[s]) : 0.0;
         S[s] = Sc[s] = Sa2[s] = 0.0;
     }
     // a contiguous chunk of the class's records per wave, staged NB at a time
     // by coalesced loads; each record then is an LDS broadcast
-    const int64_t k0 = (int64_t)blockIdx.x * nidx / gridDim.x,
-                  k1 = ((int64_t)blockIdx.x + 1) * nidx / gridDim.x;
+    const int64_t k0 = chunk * nidx / nchunk, k1 = (chunk + 1) * nidx / nchunk;
     for (int64_t kb = k0; kb < k1; kb += NB) {
         const int nb = (int)(k1 - kb < NB ? k1 - kb : NB);
-        const double* src = A.hs + kb * R;
+        const double* src = hsb + kb * R;
         __syncthreads();  // the previous batch is consumed
         for (int t = lane; t < nb * R; t += WAVE) recs[t] = src[t];
         __syncthreads();
         // the phase step sin, cos(64 dq h) between a lane's q slices on a
         // uniform grid, one lane per staged record (0 on the h = 0 branch,
         // which uses no sine); once per record and wave rather than per lane
-        if (A.qlin && lane < nb) {
+        if (SQ > 1 && A.qlin && lane < nb) {
             const double* r = recs + lane * R;
             const uint64_t m = (uint64_t)__double_as_longlong(r[LL + 3]);
             double sd = 0.0, cd = 0.0;
@@ -614,7 +653,7 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
 #pragma unroll
             for (int s = 0; s < SQ; ++s) sxs[s] = cxs[s] = 0.0;
             if (!br) {
-                if (A.qlin) {
+                if (SQ > 1 && A.qlin) {
                     const bool off = A.q0off && iqs[0] == 0;
                     double sb, cbb;
                     sincos_cw(off ? A.qa * h : qv[0] * h, sb, cbb);
@@ -654,9 +693,9 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
     }
 #pragma unroll
     for (int s = 0; s < SQ; ++s) {
-        const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
-        if (iql >= A.nqc) continue;
-        int64_t o = (int64_t)blockIdx.x * A.nqc + iql;
+        const int iql = qs0 + (ys * SQ + s) * WAVE + lane;
+        if (iql >= qs1) continue;
+        int64_t o = chunk * A.nqc + iql;
         if (MIXED) A.parta[o] = first ? Sa2[s] : A.parta[o] + Sa2[s];
         if (first) {
             A.part[o] = S[s];
@@ -671,13 +710,33 @@ __global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int
     }
 }
 
+template <int LL, bool MIXED, bool J2, int SQ>
+__global__ void __launch_bounds__(WAVE) k_series(SeriesArgs A, int64_t nidx, int first) {
+    series_chunk<LL, MIXED, J2, SQ>(A, A.hs, nidx, first, blockIdx.x, gridDim.x, blockIdx.y,
+                                    A.qs0, A.qs1);
+}
+
+// One class of many frames: each work entry (a frame's class and q range)
+// takes nchunk x ny consecutive blocks.
+template <int LL, bool J2, int SQ>
+__global__ void __launch_bounds__(WAVE)
+    k_series_multi(const SeriesArgs* __restrict__ args, const SeriesWork* __restrict__ work,
+                   int nwork) {
+    const SeriesWork& W = find_work(work, nwork, blockIdx.x);
+    const int64_t local = (int64_t)blockIdx.x - W.blk0;
+    series_chunk<LL, false, J2, SQ>(args[W.frame], W.hs, W.n, 0, local / W.ny, W.nchunk,
+                                    (int)(local % W.ny), W.qs0, W.qs1);
+}
+
 // The x = q h >= XDPH points of one class (dphrec), with k_series's launch
 // shape and class partition; each wave adds its sums to the partial rows the
 // series kernel wrote.  A wave skips every quartet whose h keeps its q points
 // below XDPH, so the kernel is a read of the records where the regime is
 // rare.
-template <int LL, bool J2>
-__global__ void __launch_bounds__(WAVE) k_series_far(SeriesArgs A, int64_t nidx) {
+template <int LL, bool J2, int SQ>
+__device__ __forceinline__ void far_chunk(const SeriesArgs& A, const double* __restrict__ hsb,
+                                          int64_t nidx, int64_t chunk, int64_t nchunk, int ys,
+                                          int qs0, int qs1) {
     constexpr int R = HS_ROW(LL + 1);
     const int lane = threadIdx.x;
     int iqs[SQ];
@@ -685,18 +744,17 @@ __global__ void __launch_bounds__(WAVE) k_series_far(SeriesArgs A, int64_t nidx)
     double qmx = 0.0;
 #pragma unroll
     for (int s = 0; s < SQ; ++s) {
-        const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
-        iqs[s] = A.q0 + (iql < A.nqc ? iql : 0);
+        const int iql = qs0 + (ys * SQ + s) * WAVE + lane;
+        iqs[s] = A.q0 + (iql < qs1 ? iql : qs0);
         qv[s] = A.q[iqs[s]];
         qmx = fmax(qmx, qv[s]);
         S[s] = Sc[s] = 0.0;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) qmx = fmax(qmx, __shfl_xor(qmx, off, WAVE));
-    const int64_t k0 = (int64_t)blockIdx.x * nidx / gridDim.x,
-                  k1 = ((int64_t)blockIdx.x + 1) * nidx / gridDim.x;
+    const int64_t k0 = chunk * nidx / nchunk, k1 = (chunk + 1) * nidx / nchunk;
     for (int64_t kq = k0; kq < k1; ++kq) {
-        const double* rc = A.hs + kq * R;
+        const double* rc = hsb + kq * R;
         const double h = rc[LL + 1];
         const uint64_t meta = (uint64_t)__double_as_longlong(rc[LL + 3]);
         if (((meta >> 44) & 1u) || qmx * h < XDPH) continue;
@@ -718,15 +776,30 @@ __global__ void __launch_bounds__(WAVE) k_series_far(SeriesArgs A, int64_t nidx)
     }
 #pragma unroll
     for (int s = 0; s < SQ; ++s) {
-        const int iql = (blockIdx.y * SQ + s) * WAVE + lane;
-        if (iql >= A.nqc) continue;
-        const int64_t o = (int64_t)blockIdx.x * A.nqc + iql;
+        const int iql = qs0 + (ys * SQ + s) * WAVE + lane;
+        if (iql >= qs1) continue;
+        const int64_t o = chunk * A.nqc + iql;
         double ps = A.part[o], pc = A.partc[o];
         kahan(ps, pc, S[s]);
         kahan(ps, pc, -Sc[s]);
         A.part[o] = ps;
         A.partc[o] = pc;
     }
+}
+
+template <int LL, bool J2, int SQ>
+__global__ void __launch_bounds__(WAVE) k_series_far(SeriesArgs A, int64_t nidx) {
+    far_chunk<LL, J2, SQ>(A, A.hs, nidx, blockIdx.x, gridDim.x, blockIdx.y, A.qs0, A.qs1);
+}
+
+template <int LL, bool J2, int SQ>
+__global__ void __launch_bounds__(WAVE)
+    k_series_far_multi(const SeriesArgs* __restrict__ args, const SeriesWork* __restrict__ work,
+                       int nwork) {
+    const SeriesWork& W = find_work(work, nwork, blockIdx.x);
+    const int64_t local = (int64_t)blockIdx.x - W.blk0;
+    far_chunk<LL, J2, SQ>(args[W.frame], W.hs, W.n, local / W.ny, W.nchunk, (int)(local % W.ny),
+                          W.qs0, W.qs1);
 }
 
 // Mixed precision, second pass: the flagged q points (A.qsel[A.pbase + k],
@@ -1034,7 +1107,44 @@ hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t*
     return hipGetLastError();
 }
 
+hipError_t launch_contract_multi(const SeriesArgs* args, const ClassWork* work, int nwork,
+                                 int64_t nblocks, int LL, int tmax, bool j2, hipStream_t s) {
+    if (nwork <= 0 || nblocks <= 0) return hipSuccess;
+    const size_t lds = contract_lds_bytes(LL, tmax);
+    switch (LL) {
+#define C_(k)                                                                                   \
+    case k:                                                                                     \
+        if (j2)                                                                                 \
+            hipLaunchKernelGGL((k_contract_multi<k, true>), dim3((unsigned)nblocks),            \
+                               dim3(ContractShape<k>::CT), lds, s, args, work, nwork, tmax);    \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_contract_multi<k, false>), dim3((unsigned)nblocks),           \
+                               dim3(ContractShape<k>::CT), lds, s, args, work, nwork, tmax);    \
+        break;
+        ALL_LL(C_)
+#undef C_
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int contract_groups_per_block(int LL) {
+    switch (LL) {
+#define C_(k) \
+    case k: return ContractShape<k>::NG;
+        ALL_LL(C_)
+#undef C_
+        default: return 1;
+    }
+}
+
+// Raises (never lowers) the dynamic LDS limit of the contraction kernels:
+// plans with different needs share the kernels.
 hipError_t set_contract_lds_limit(size_t bytes) {
+    static std::mutex mu;
+    static size_t cur = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    if (bytes <= cur) return hipSuccess;
 #define C_(k)                                                                            \
     {                                                                                    \
         hipError_t e = hipFuncSetAttribute((const void*)k_contract<k, false>,            \
@@ -1044,23 +1154,104 @@ hipError_t set_contract_lds_limit(size_t bytes) {
         e = hipFuncSetAttribute((const void*)k_contract<k, true>,                        \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes); \
         if (e != hipSuccess) return e;                                                   \
+        e = hipFuncSetAttribute((const void*)k_contract_multi<k, false>,                 \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes); \
+        if (e != hipSuccess) return e;                                                   \
+        e = hipFuncSetAttribute((const void*)k_contract_multi<k, true>,                  \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes); \
+        if (e != hipSuccess) return e;                                                   \
     }
     ALL_LL(C_)
 #undef C_
+    cur = bytes;
     return hipSuccess;
 }
 
-hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int waves,
-                             hipStream_t s) {
+namespace {
+// The sub-ranges [qs0, qs1) of a q chunk and their slices per wave: the
+// 256-point groups at SQ = 4 (one sincos and three rotations per lane and
+// record), the tail at the SQ that pads it least, so that a 64-point grid
+// does not run 256-point waves.  Mixed precision keeps SQ = 4 throughout.
+struct QSplit {
+    int n, qs0[2], qs1[2], sq[2];
+};
+QSplit q_split(int nqc, bool only4) {
+    QSplit r{};
+    const int full = only4 ? nqc : (nqc / 256) * 256;
+    int k = 0;
+    if (full > 0) {
+        r.qs0[k] = 0;
+        r.qs1[k] = full;
+        r.sq[k] = 4;
+        ++k;
+    }
+    if (full < nqc) {
+        const int tail = nqc - full;
+        r.qs0[k] = full;
+        r.qs1[k] = nqc;
+        r.sq[k] = tail <= 64 ? 1 : tail <= 128 ? 2 : tail <= 192 ? 1 : 4;
+        ++k;
+    }
+    r.n = k;
+    return r;
+}
+
+template <int LL, int SQV>
+void series_go(const SeriesArgs& a, int64_t nidx, int waves, int first, hipStream_t s) {
+    const dim3 grid(waves, (a.qs1 - a.qs0 + SQV * WAVE - 1) / (SQV * WAVE));
+    if constexpr (SQV == 4) {
+        if (a.mixed) {
+            if (a.j2)
+                hipLaunchKernelGGL((k_series<LL, true, true, 4>), grid, dim3(WAVE), 0, s, a, nidx,
+                                   first);
+            else
+                hipLaunchKernelGGL((k_series<LL, true, false, 4>), grid, dim3(WAVE), 0, s, a,
+                                   nidx, first);
+            return;
+        }
+    }
+    if (a.j2)
+        hipLaunchKernelGGL((k_series<LL, false, true, SQV>), grid, dim3(WAVE), 0, s, a, nidx,
+                           first);
+    else
+        hipLaunchKernelGGL((k_series<LL, false, false, SQV>), grid, dim3(WAVE), 0, s, a, nidx,
+                           first);
+}
+
+template <int LL, int SQV>
+void series_far_go(const SeriesArgs& a, int64_t nidx, int waves, hipStream_t s) {
+    const dim3 grid(waves, (a.qs1 - a.qs0 + SQV * WAVE - 1) / (SQV * WAVE));
+    if (a.j2)
+        hipLaunchKernelGGL((k_series_far<LL, true, SQV>), grid, dim3(WAVE), 0, s, a, nidx);
+    else
+        hipLaunchKernelGGL((k_series_far<LL, false, SQV>), grid, dim3(WAVE), 0, s, a, nidx);
+}
+
+template <int LL>
+void series_ll(SeriesArgs a, int64_t nidx, int waves, int first, bool far, hipStream_t s) {
+    const QSplit sp = q_split(a.nqc, a.mixed);
+    for (int k = 0; k < sp.n; ++k) {
+        a.qs0 = sp.qs0[k];
+        a.qs1 = sp.qs1[k];
+        if (far) {
+            if (sp.sq[k] == 1) series_far_go<LL, 1>(a, nidx, waves, s);
+            else if (sp.sq[k] == 2) series_far_go<LL, 2>(a, nidx, waves, s);
+            else series_far_go<LL, 4>(a, nidx, waves, s);
+        } else {
+            if (sp.sq[k] == 1) series_go<LL, 1>(a, nidx, waves, first, s);
+            else if (sp.sq[k] == 2) series_go<LL, 2>(a, nidx, waves, first, s);
+            else series_go<LL, 4>(a, nidx, waves, first, s);
+        }
+    }
+}
+
+hipError_t series_any(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first, bool far,
+                      hipStream_t s) {
     if (nidx <= 0) return hipSuccess;
-    dim3 grid(waves, (a.nqc + SQ * WAVE - 1) / (SQ * WAVE));
     switch (LL) {
-#define CASE(k)                                                                              \
-    case k:                                                                                  \
-        if (a.j2)                                                                            \
-            hipLaunchKernelGGL((k_series_far<k, true>), grid, dim3(WAVE), 0, s, a, nidx);      \
-        else                                                                                 \
-            hipLaunchKernelGGL((k_series_far<k, false>), grid, dim3(WAVE), 0, s, a, nidx);     \
+#define CASE(k)                                       \
+    case k:                                           \
+        series_ll<k>(a, nidx, waves, first, far, s);  \
         break;
         CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
@@ -1068,6 +1259,64 @@ hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int wave
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+}  // namespace
+
+int series_q_split(int nqc, bool only4, int* qs0, int* qs1, int* sq) {
+    const QSplit sp = q_split(nqc, only4);
+    for (int k = 0; k < sp.n; ++k) {
+        qs0[k] = sp.qs0[k];
+        qs1[k] = sp.qs1[k];
+        sq[k] = sp.sq[k];
+    }
+    return sp.n;
+}
+
+namespace {
+template <int LL, int SQV>
+void series_multi_go(const SeriesArgs* args, const SeriesWork* work, int nwork, int64_t nblocks,
+                     bool j2, bool far, hipStream_t s) {
+    const dim3 grid((unsigned)nblocks);
+    if (far) {
+        if (j2)
+            hipLaunchKernelGGL((k_series_far_multi<LL, true, SQV>), grid, dim3(WAVE), 0, s, args,
+                               work, nwork);
+        else
+            hipLaunchKernelGGL((k_series_far_multi<LL, false, SQV>), grid, dim3(WAVE), 0, s, args,
+                               work, nwork);
+    } else {
+        if (j2)
+            hipLaunchKernelGGL((k_series_multi<LL, true, SQV>), grid, dim3(WAVE), 0, s, args, work,
+                               nwork);
+        else
+            hipLaunchKernelGGL((k_series_multi<LL, false, SQV>), grid, dim3(WAVE), 0, s, args,
+                               work, nwork);
+    }
+}
+}  // namespace
+
+hipError_t launch_series_multi(const SeriesArgs* args, const SeriesWork* work, int nwork,
+                               int64_t nblocks, int LL, int sq, bool j2, bool far,
+                               hipStream_t s) {
+    if (nwork <= 0 || nblocks <= 0) return hipSuccess;
+    switch (LL) {
+#define CASE(k)                                                                      \
+    case k:                                                                          \
+        if (sq == 1) series_multi_go<k, 1>(args, work, nwork, nblocks, j2, far, s);  \
+        else if (sq == 2) series_multi_go<k, 2>(args, work, nwork, nblocks, j2, far, s); \
+        else series_multi_go<k, 4>(args, work, nwork, nblocks, j2, far, s);          \
+        break;
+        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int waves,
+                             hipStream_t s) {
+    return series_any(a, LL, nidx, waves, 0, true, s);
 }
 
 hipError_t launch_series_pts(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,
@@ -1105,28 +1354,5 @@ hipError_t launch_qscatter(const double* Sfix, const int32_t* qsel, const int32_
 
 hipError_t launch_series(const SeriesArgs& a, int LL, int64_t nidx, int waves, int first,
                          hipStream_t s) {
-    if (nidx <= 0) return hipSuccess;
-    dim3 grid(waves, (a.nqc + SQ * WAVE - 1) / (SQ * WAVE));
-    switch (LL) {
-#define CASE(k)                                                                              \
-    case k:                                                                                  \
-        if (a.j2 && a.mixed)                                                                 \
-            hipLaunchKernelGGL((k_series<k, true, true>), grid, dim3(WAVE), 0, s, a, nidx,   \
-                               first);                                                       \
-        else if (a.j2)                                                                       \
-            hipLaunchKernelGGL((k_series<k, false, true>), grid, dim3(WAVE), 0, s, a, nidx,  \
-                               first);                                                       \
-        else if (a.mixed)                                                                    \
-            hipLaunchKernelGGL((k_series<k, true, false>), grid, dim3(WAVE), 0, s, a, nidx,  \
-                               first);                                                       \
-        else                                                                                 \
-            hipLaunchKernelGGL((k_series<k, false, false>), grid, dim3(WAVE), 0, s, a, nidx, \
-                               first);                                                       \
-        break;
-        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
-#undef CASE
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return series_any(a, LL, nidx, waves, first, false, s);
 }

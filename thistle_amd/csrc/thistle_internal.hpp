@@ -160,6 +160,7 @@ struct SeriesArgs {
     const char4* lxyz;
     const double* q;
     int nq, q0, nqc;          // full grid size, chunk start, chunk length
+    int qs0, qs1;             // the launch's points [qs0, qs1) of the chunk
     int qlin;                 // q[k] = qa + k dq for k >= 1 (dq > 0)
     int q0off;                // ... and q[0] is not qa (e.g. 1e-10 set by hand)
     double qa, dq;
@@ -188,6 +189,51 @@ struct SeriesArgs {
     const int64_t* comboff;   // [nsig^4] table start per signature combination
     int nsig;
 };
+
+// Cross-frame work tables (batched frames): one entry per frame that has
+// quartets in the class, in increasing first block blk0.
+struct ClassWork {           // contraction: one block per quartet group
+    int64_t blk0;
+    int64_t n;               // quartets of the class in this frame
+    const int32_t* idx;      // the frame's class list
+    double* hs;              // the frame's class records
+    int64_t frame;           // index into the SeriesArgs array
+};
+struct SeriesWork {          // series: nchunk x ny blocks
+    int64_t blk0;
+    int64_t n;               // records of the class in this frame
+    const double* hs;
+    int32_t frame, nchunk;   // chunks = partial rows written (<= the frame's rows)
+    int32_t ny, qs0, qs1;    // q slice groups over the points [qs0, qs1)
+    int32_t pad;
+};
+
+struct RedWork {             // reduction: ceil(nqc / RED_Q) blocks per frame
+    int64_t blk0;
+    const double* part;
+    const double* partc;
+    double* out;
+    int32_t waves, nqc;
+};
+struct FillWork {            // zeroing: ceil(n / 256) blocks per buffer
+    int64_t blk0;
+    double* p;
+    int64_t n;
+};
+hipError_t launch_reduce_multi(const RedWork* work, int nwork, int64_t nblocks, hipStream_t s);
+int reduce_q_per_block();
+hipError_t launch_fill_multi(const FillWork* work, int nwork, int64_t nblocks, hipStream_t s);
+
+// Cross-frame launches of the quartet stage (series.hip, kernels.hip)
+hipError_t launch_contract_multi(const SeriesArgs* args, const ClassWork* work, int nwork,
+                                 int64_t nblocks, int LL, int tmax, bool j2, hipStream_t s);
+int contract_groups_per_block(int LL);
+// the q sub-ranges [qs0, qs1) of a chunk of nqc points and their q slices
+// per wave (1, 2 or 4); returns their number (<= 2)
+int series_q_split(int nqc, bool only4, int* qs0, int* qs1, int* sq);
+hipError_t launch_series_multi(const SeriesArgs* args, const SeriesWork* work, int nwork,
+                               int64_t nblocks, int LL, int sq, bool j2, bool far,
+                               hipStream_t s);
 
 // Adjoint of the total j0 path (adjoint.hip)
 struct AdjointArgs {
