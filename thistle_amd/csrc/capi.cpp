@@ -1165,32 +1165,46 @@ struct ThistleBatch {
     bool tuned = false, graph_ok = false;
     float ms = 0.f;
     // Cross-frame quartet stage (FP64, every q grid within one 512-point
-    // chunk): per LL class one contraction launch and at most three series
-    // and three far-regime launches over all frames, driven by work tables;
-    // one zeroing and one reduction launch for every frame's partial rows.
+    // chunk).  The frames form a few groups of about equal work; per group
+    // and LL class one contraction launch and at most three series and
+    // three far-regime launches over all its frames, driven by work tables,
+    // and one zeroing and one reduction launch.  A group's per-frame set-up
+    // stages (streams 2 and 3) overlap the previous group's quartet stage
+    // (contractions on stream 0, series and reduction on stream 1).
     bool merged = false;
-    int rows = 0;  // partial rows (series chunks) per frame, tuned
+    int rows = 0, ngroups = 0;  // partial rows (series chunks) per frame, groups: tuned
     struct SLaunch {
         int LL, sq;
         bool far;
         int64_t off, n, blocks;
     };
-    DevBuf dargs, dcw, dsw, dfill, dred;
-    int64_t cw_off[18] = {}, cw_blocks[17] = {};
-    int cls_tmax[17] = {};
-    std::vector<SLaunch> slaunch;
-    int nfill = 0, nred = 0;
-    int64_t fill_blocks = 0, red_blocks = 0;
-    hipEvent_t cev[17] = {}, fill_ev = nullptr, ser_ev = nullptr;
+    struct Group {
+        int f0 = 0, f1 = 0;
+        DevBuf dcw, dsw, dfill, dred;
+        int64_t cw_off[18] = {}, cw_blocks[17] = {};
+        int cls_tmax[17] = {};
+        std::vector<SLaunch> slaunch;
+        int nfill = 0, nred = 0;
+        int64_t fill_blocks = 0, red_blocks = 0;
+        hipEvent_t cev[17] = {}, joina[2] = {}, fill_ev = nullptr;
+        ~Group() {
+            for (auto& e : cev)
+                if (e) (void)hipEventDestroy(e);
+            for (auto& e : joina)
+                if (e) (void)hipEventDestroy(e);
+            if (fill_ev) (void)hipEventDestroy(fill_ev);
+        }
+    };
+    DevBuf dargs;
+    std::vector<std::unique_ptr<Group>> groups;
+    hipEvent_t ser_ev = nullptr;
     ~ThistleBatch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         plans.clear();  // plans use the batch streams
         for (auto& e : join)
             if (e) (void)hipEventDestroy(e);
-        for (auto& e : cev)
-            if (e) (void)hipEventDestroy(e);
-        if (fill_ev) (void)hipEventDestroy(fill_ev);
+        groups.clear();
         if (ser_ev) (void)hipEventDestroy(ser_ev);
         if (fork) (void)hipEventDestroy(fork);
         if (t0) (void)hipEventDestroy(t0);
@@ -1212,23 +1226,15 @@ void batch_set_waves(ThistleBatch* b, int per_cu) {
 // Work tables of the cross-frame quartet stage for `rows` partial rows per
 // frame (a frame's class of n records is split into min(rows, n) chunks,
 // as a single-frame execution with waves = rows splits it).
-int batch_build_tables(ThistleBatch* b, int rows) {
-    b->rows = rows;
-    const int nf = (int)b->plans.size();
-    std::vector<SeriesArgs> args((size_t)nf);
+int group_tables(ThistleBatch* b, ThistleBatch::Group* g) {
     std::vector<ClassWork> cw;
     std::vector<SeriesWork> sw;
     std::vector<FillWork> fw;
     std::vector<RedWork> rw;
     int64_t fb = 0, rb = 0;
     const int rq = reduce_q_per_block();
-    for (int f = 0; f < nf; ++f) {
+    for (int f = g->f0; f < g->f1; ++f) {
         ThistlePlan* p = b->plans[f].get();
-        p->waves = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)rows, p->waves_alloc,
-                                                                p->nquart}));
-        args[f] = series_args(p);
-        args[f].q0 = 0;
-        args[f].nqc = p->nq;
         const int64_t len = (int64_t)p->waves * p->nq;
         fw.push_back({fb, p->part.as<double>(), len});
         fb += (len + 255) / 256;
@@ -1238,30 +1244,30 @@ int batch_build_tables(ThistleBatch* b, int rows) {
                       b->S.as<double>() + b->soff[f], p->waves, p->nq});
         rb += (p->nq + rq - 1) / rq;
     }
-    b->slaunch.clear();
+    g->slaunch.clear();
     for (int LL = 0; LL <= 16; ++LL) {
-        b->cw_off[LL] = (int64_t)cw.size();
-        const int ng = contract_groups_per_block(LL);
+        g->cw_off[LL] = (int64_t)cw.size();
+        const int ngq = contract_groups_per_block(LL);
         int64_t blocks = 0;
         int tmax = 0;
-        for (int f = 0; f < nf; ++f) {
+        for (int f = g->f0; f < g->f1; ++f) {
             ThistlePlan* p = b->plans[f].get();
             const int64_t n = p->cls_off[LL + 1] - p->cls_off[LL];
             if (n <= 0) continue;
             cw.push_back({blocks, n, p->cls.as<int32_t>() + p->cls_off[LL],
                           p->hs.as<double>() + p->cls_hoff[LL], f});
-            blocks += (n + ng - 1) / ng;
+            blocks += (n + ngq - 1) / ngq;
             tmax = std::max(tmax, p->cls_tmax[LL]);
         }
-        b->cw_blocks[LL] = blocks;
-        b->cls_tmax[LL] = tmax;
+        g->cw_blocks[LL] = blocks;
+        g->cls_tmax[LL] = tmax;
         // series launches in the order a single frame issues them: the
         // 256-point groups (q slices of 4), the tail, then the far regime
         for (int far = 0; far < 2; ++far)
             for (int sq : {4, 2, 1}) {
                 const int64_t off = (int64_t)sw.size();
                 int64_t sb = 0;
-                for (int f = 0; f < nf; ++f) {
+                for (int f = g->f0; f < g->f1; ++f) {
                     ThistlePlan* p = b->plans[f].get();
                     const int64_t n = p->cls_off[LL + 1] - p->cls_off[LL];
                     if (n <= 0 || (far && !p->far)) continue;
@@ -1277,72 +1283,149 @@ int batch_build_tables(ThistleBatch* b, int rows) {
                     }
                 }
                 if (sb > 0)
-                    b->slaunch.push_back({LL, sq, far != 0, off, (int64_t)sw.size() - off, sb});
+                    g->slaunch.push_back({LL, sq, far != 0, off, (int64_t)sw.size() - off, sb});
             }
     }
-    b->cw_off[17] = (int64_t)cw.size();
-    b->nfill = (int)fw.size();
-    b->fill_blocks = fb;
-    b->nred = (int)rw.size();
-    b->red_blocks = rb;
+    g->cw_off[17] = (int64_t)cw.size();
+    g->nfill = (int)fw.size();
+    g->fill_blocks = fb;
+    g->nred = (int)rw.size();
+    g->red_blocks = rb;
     int rc;
-    if ((rc = dalloc(b->dargs, sizeof(SeriesArgs) * std::max<size_t>(args.size(), 1))) ||
-        (rc = dalloc(b->dcw, sizeof(ClassWork) * std::max<size_t>(cw.size(), 1))) ||
-        (rc = dalloc(b->dsw, sizeof(SeriesWork) * std::max<size_t>(sw.size(), 1))) ||
-        (rc = dalloc(b->dfill, sizeof(FillWork) * std::max<size_t>(fw.size(), 1))) ||
-        (rc = dalloc(b->dred, sizeof(RedWork) * std::max<size_t>(rw.size(), 1))))
+    if ((rc = dalloc(g->dcw, sizeof(ClassWork) * std::max<size_t>(cw.size(), 1))) ||
+        (rc = dalloc(g->dsw, sizeof(SeriesWork) * std::max<size_t>(sw.size(), 1))) ||
+        (rc = dalloc(g->dfill, sizeof(FillWork) * std::max<size_t>(fw.size(), 1))) ||
+        (rc = dalloc(g->dred, sizeof(RedWork) * std::max<size_t>(rw.size(), 1))))
         return rc;
-    HCHECK(hipMemcpy(b->dargs.p, args.data(), sizeof(SeriesArgs) * args.size(),
-                     hipMemcpyHostToDevice));
     if (!cw.empty())
-        HCHECK(hipMemcpy(b->dcw.p, cw.data(), sizeof(ClassWork) * cw.size(),
+        HCHECK(hipMemcpy(g->dcw.p, cw.data(), sizeof(ClassWork) * cw.size(),
                          hipMemcpyHostToDevice));
     if (!sw.empty())
-        HCHECK(hipMemcpy(b->dsw.p, sw.data(), sizeof(SeriesWork) * sw.size(),
+        HCHECK(hipMemcpy(g->dsw.p, sw.data(), sizeof(SeriesWork) * sw.size(),
                          hipMemcpyHostToDevice));
-    HCHECK(hipMemcpy(b->dfill.p, fw.data(), sizeof(FillWork) * fw.size(), hipMemcpyHostToDevice));
-    HCHECK(hipMemcpy(b->dred.p, rw.data(), sizeof(RedWork) * rw.size(), hipMemcpyHostToDevice));
+    if (!fw.empty())
+        HCHECK(hipMemcpy(g->dfill.p, fw.data(), sizeof(FillWork) * fw.size(),
+                         hipMemcpyHostToDevice));
+    if (!rw.empty())
+        HCHECK(hipMemcpy(g->dred.p, rw.data(), sizeof(RedWork) * rw.size(),
+                         hipMemcpyHostToDevice));
     return THISTLE_OK;
 }
 
-// Per-frame set-up stages on the batch streams, then the quartet stage of
-// every frame in a few launches: contractions on stream 0, series (behind
-// each class's contraction) and the reduction on stream 1.
+int batch_build_tables(ThistleBatch* b, int rows) {
+    b->rows = rows;
+    const int nf = (int)b->plans.size();
+    std::vector<SeriesArgs> args((size_t)nf);
+    for (int f = 0; f < nf; ++f) {
+        ThistlePlan* p = b->plans[f].get();
+        p->waves = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)rows, p->waves_alloc,
+                                                                p->nquart}));
+        args[f] = series_args(p);
+        args[f].q0 = 0;
+        args[f].nqc = p->nq;
+    }
+    int rc = dalloc(b->dargs, sizeof(SeriesArgs) * std::max<size_t>(args.size(), 1));
+    if (rc) return rc;
+    HCHECK(hipMemcpy(b->dargs.p, args.data(), sizeof(SeriesArgs) * args.size(),
+                     hipMemcpyHostToDevice));
+    for (auto& g : b->groups)
+        if ((rc = group_tables(b, g.get()))) return rc;
+    return THISTLE_OK;
+}
+
+// Frames split into contiguous groups of about equal work (quartets x q
+// points), with their events.
+int batch_make_groups(ThistleBatch* b, int ngroups) {
+    const int nf = (int)b->plans.size();
+    ngroups = std::max(1, std::min(ngroups, nf));
+    int64_t total = 0;
+    for (auto& p : b->plans) total += p->nquart * (int64_t)p->nq;
+    b->groups.clear();
+    int f = 0;
+    int64_t acc = 0;
+    for (int k = 0; k < ngroups; ++k) {
+        std::unique_ptr<ThistleBatch::Group> g(new ThistleBatch::Group());
+        g->f0 = f;
+        const int64_t goal = total * (k + 1) / ngroups;
+        while (f < nf && (k == ngroups - 1 || acc < goal || f == g->f0)) {
+            acc += b->plans[f]->nquart * (int64_t)b->plans[f]->nq;
+            ++f;
+        }
+        g->f1 = f;
+        if (g->f1 == g->f0) continue;
+        for (auto& e : g->cev) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto& e : g->joina) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HCHECK(hipEventCreateWithFlags(&g->fill_ev, hipEventDisableTiming));
+        b->groups.push_back(std::move(g));
+    }
+    return THISTLE_OK;
+}
+
+// Rows per frame and number of frame groups.  One group: the per-frame
+// stages spread over all four streams; several: over streams 2 and 3, so
+// that they overlap the previous group's quartet stage on streams 0 and 1.
+int batch_configure(ThistleBatch* b, int rows, int ngroups) {
+    int rc = batch_make_groups(b, ngroups);
+    if (rc) return rc;
+    const bool split = b->groups.size() > 1;
+    for (size_t f = 0; f < b->plans.size(); ++f) {
+        ThistlePlan* p = b->plans[f].get();
+        p->stream = split ? b->streams[2 + (f & 1)] : b->streams[f % ThistleBatch::NS];
+        if (p->blas) BCHECK(rocblas_set_stream(p->blas, p->stream));
+    }
+    b->ngroups = (int)b->groups.size();
+    return batch_build_tables(b, rows);
+}
+
 int batch_enqueue_merged(ThistleBatch* b) {
     hipStream_t s0 = b->streams[0], s1 = b->streams[1];
     HCHECK(hipEventRecord(b->fork, s0));
     for (int s = 1; s < ThistleBatch::NS; ++s) HCHECK(hipStreamWaitEvent(b->streams[s], b->fork, 0));
-    for (size_t f = 0; f < b->plans.size(); ++f) {
-        int rc = enqueue_execute(b->plans[f].get(), false, nullptr, false);
-        if (rc) return rc;
+    const SeriesArgs* args = b->dargs.as<SeriesArgs>();
+    for (auto& gp : b->groups) {
+        ThistleBatch::Group* g = gp.get();
+        for (int f = g->f0; f < g->f1; ++f) {
+            int rc = enqueue_execute(b->plans[f].get(), false, nullptr, false);
+            if (rc) return rc;
+        }
+        if (b->groups.size() > 1) {
+            HCHECK(hipEventRecord(g->joina[0], b->streams[2]));
+            HCHECK(hipEventRecord(g->joina[1], b->streams[3]));
+            HCHECK(hipStreamWaitEvent(s0, g->joina[0], 0));
+            HCHECK(hipStreamWaitEvent(s0, g->joina[1], 0));
+        } else {
+            for (int s = 1; s < ThistleBatch::NS; ++s) {
+                HCHECK(hipEventRecord(b->join[s], b->streams[s]));
+                HCHECK(hipStreamWaitEvent(s0, b->join[s], 0));
+            }
+        }
+        HCHECK(launch_fill_multi(g->dfill.as<FillWork>(), g->nfill, g->fill_blocks, s0));
+        HCHECK(hipEventRecord(g->fill_ev, s0));
+        HCHECK(hipStreamWaitEvent(s1, g->fill_ev, 0));
+        for (int LL = 0; LL <= 16; ++LL) {
+            const int nw = (int)(g->cw_off[LL + 1] - g->cw_off[LL]);
+            if (nw <= 0) continue;
+            HCHECK(launch_contract_multi(args, g->dcw.as<ClassWork>() + g->cw_off[LL], nw,
+                                         g->cw_blocks[LL], LL, g->cls_tmax[LL], false, s0));
+            HCHECK(hipEventRecord(g->cev[LL], s0));
+        }
+        int waited = -1;
+        for (const auto& L : g->slaunch) {
+            if (L.LL != waited) {
+                HCHECK(hipStreamWaitEvent(s1, g->cev[L.LL], 0));
+                waited = L.LL;
+            }
+            HCHECK(launch_series_multi(args, g->dsw.as<SeriesWork>() + L.off, (int)L.n, L.blocks,
+                                       L.LL, L.sq, false, L.far, s1));
+        }
+        HCHECK(launch_reduce_multi(g->dred.as<RedWork>(), g->nred, g->red_blocks, s1));
     }
-    for (int s = 1; s < ThistleBatch::NS; ++s) {
+    HCHECK(hipEventRecord(b->ser_ev, s1));
+    HCHECK(hipStreamWaitEvent(s0, b->ser_ev, 0));
+    for (int s = 2; s < ThistleBatch::NS; ++s) {
         HCHECK(hipEventRecord(b->join[s], b->streams[s]));
         HCHECK(hipStreamWaitEvent(s0, b->join[s], 0));
     }
-    HCHECK(launch_fill_multi(b->dfill.as<FillWork>(), b->nfill, b->fill_blocks, s0));
-    HCHECK(hipEventRecord(b->fill_ev, s0));
-    HCHECK(hipStreamWaitEvent(s1, b->fill_ev, 0));
-    const SeriesArgs* args = b->dargs.as<SeriesArgs>();
-    for (int LL = 0; LL <= 16; ++LL) {
-        const int nw = (int)(b->cw_off[LL + 1] - b->cw_off[LL]);
-        if (nw <= 0) continue;
-        HCHECK(launch_contract_multi(args, b->dcw.as<ClassWork>() + b->cw_off[LL], nw,
-                                     b->cw_blocks[LL], LL, b->cls_tmax[LL], false, s0));
-        HCHECK(hipEventRecord(b->cev[LL], s0));
-    }
-    int waited = -1;
-    for (const auto& L : b->slaunch) {
-        if (L.LL != waited) {
-            HCHECK(hipStreamWaitEvent(s1, b->cev[L.LL], 0));
-            waited = L.LL;
-        }
-        HCHECK(launch_series_multi(args, b->dsw.as<SeriesWork>() + L.off, (int)L.n, L.blocks,
-                                   L.LL, L.sq, false, L.far, s1));
-    }
-    HCHECK(launch_reduce_multi(b->dred.as<RedWork>(), b->nred, b->red_blocks, s1));
-    HCHECK(hipEventRecord(b->ser_ev, s1));
-    HCHECK(hipStreamWaitEvent(s0, b->ser_ev, 0));
     return THISTLE_OK;
 }
 
@@ -1376,13 +1459,16 @@ int batch_timed(ThistleBatch* b, float& ms) {
 // rows per frame, i.e. records per series wave (fewer rows, longer chunks
 // and a shorter reduction; more rows, more waves in flight).
 int batch_prepare(ThistleBatch* b) {
-    const std::vector<int> cand = b->merged ? std::vector<int>{32, 128, 512, 2048}
-                                            : std::vector<int>{4, 8, 12};
+    // merged: (rows per frame, frame groups) = c / 16, c % 16
+    const std::vector<int> cand =
+        b->merged ? std::vector<int>{512 * 16 + 1, 128 * 16 + 1, 2048 * 16 + 1, 512 * 16 + 2,
+                                     512 * 16 + 4}
+                  : std::vector<int>{4, 8, 12};
     float best = 0.f;
     int pick = cand[0];
     for (int c : cand) {
         if (b->merged) {
-            int rc = batch_build_tables(b, c);
+            int rc = batch_configure(b, c / 16, c % 16);
             if (rc) return rc;
         } else {
             batch_set_waves(b, c);
@@ -1398,7 +1484,7 @@ int batch_prepare(ThistleBatch* b) {
         }
     }
     if (b->merged) {
-        int rc = batch_build_tables(b, pick);
+        int rc = batch_configure(b, pick / 16, pick % 16);
         if (rc) return rc;
     } else {
         batch_set_waves(b, pick);
@@ -1468,10 +1554,8 @@ int32_t thistle_batch_create(int32_t nframes, const ThistleBasis* bases, const i
     b->merged = o->precision == THISTLE_FP64;
     for (int f = 0; f < nframes; ++f) b->merged = b->merged && bases[f].nq <= 512;
     if (b->merged) {
-        for (auto& e : b->cev) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        HCHECK(hipEventCreateWithFlags(&b->fill_ev, hipEventDisableTiming));
         HCHECK(hipEventCreateWithFlags(&b->ser_ev, hipEventDisableTiming));
-        if ((rc = batch_build_tables(b.get(), 512))) return rc;
+        if ((rc = batch_configure(b.get(), 512, 1))) return rc;
     }
     *out = b.release();
     return THISTLE_OK;
