@@ -231,6 +231,23 @@ int64_t thistle_createtwordm_bit(const int64_t *alpha, const int64_t *beta,
 int32_t thistle_one_rdm_bit(const int64_t *alpha, const int64_t *beta,
                             const double *civs, int64_t n, const ThistleOpts *opts,
                             int32_t cap_norb, int32_t *norb_out, double *onerdm);
+/* createtwordm_bit_fci (src/one_rdm.f90:390-927): the FCI builder, from one
+ * coefficient column civs (the file's second column) and the driver's
+ * tables of unique alpha strings (src/scattering_main_routine.py:269-289):
+ * uniq = newdat(:,1) (nuniq strings, increasing; newdat(:,2) is never read),
+ * irep (determinants per string) and start / end (1-based first and last
+ * file position).  Rows and values as thistle_createtwordm_bit's.  The
+ * inputs must be ones the reference processes: one electron count in every
+ * string, nuniq > norbs (:479 reads row norbs + 1), every determinant's
+ * alpha and beta string among uniq, each string's determinants contiguous
+ * in the file, orbitals 1-31; THISTLE_EINVAL otherwise.  Both reference
+ * callers pass fci = .False. (src/scattering_main_routine.py:251,
+ * src/Main.f90:142,179); this entry serves callers of the routine itself. */
+int64_t thistle_createtwordm_bit_fci(const int64_t *alpha, const int64_t *beta,
+                                     const double *civs, int64_t n, const int64_t *uniq,
+                                     int64_t nuniq, const int64_t *irep, const int64_t *start,
+                                     const int64_t *end, double cutoff, const ThistleOpts *opts,
+                                     int32_t *mat, double *total, int64_t cap);
 /* total_scattering_calculation_2 (src/main_calculation.f90:501-849) on the
  * bit-string route (readrdm false, fci false): type 1 total j0 and 7 total
  * j2 from createtwordm_bit, 2 elastic j0 and 8 elastic j2 from one_rdm_bit.

@@ -78,6 +78,7 @@ def _declare(L):
     L.orc_mcci_to_bit.argtypes = [_p, _p, _p, _p, _l, _i, _p, _p]
     L.orc_createtwordm_bit.argtypes = [_p, _p, _p, _p, _l, _p, _i]
     L.orc_one_rdm_bit.argtypes = [_p, _p, _p, _l, _p, _i]
+    L.orc_createtwordm_bit_fci.argtypes = [_p, _p, _p, _l, _p, _l, _p, _p, _p, _p, _i]
 
 
 def _ptr(a):
@@ -424,3 +425,35 @@ def one_rdm_bit(alpha, beta, civs):
     g = np.zeros((N, N), order="F")
     lib().orc_one_rdm_bit(_ptr(a), _ptr(b), _ptr(c), len(a), _ptr(g), N)
     return g
+
+
+def fci_tables(alpha, beta):
+    """newdat(:,1), irep, start, end as the driver builds them
+    (src/scattering_main_routine.py:269-289): unique alpha strings in
+    increasing order, their determinant counts and 1-based first / last
+    file positions."""
+    alpha = np.asarray(alpha, dtype=np.int64)
+    uniq = np.unique(alpha)
+    irep = np.array([np.count_nonzero(alpha == u) for u in uniq], dtype=np.int64)
+    start = np.array([np.nonzero(alpha == u)[0][0] + 1 for u in uniq], dtype=np.int64)
+    end = np.array([np.nonzero(alpha == u)[0][-1] + 1 for u in uniq], dtype=np.int64)
+    return uniq, irep, start, end
+
+
+def twordm_bit_fci_dense(alpha, beta, civs):
+    """createtwordm_bit_fci's twordm (src/one_rdm.f90:390-875)."""
+    a, b, c = _L(alpha), _L(beta), _D(civs)
+    uniq, irep, start, end = (_L(x) for x in fci_tables(a, b))
+    args = (_ptr(a), _ptr(b), _ptr(c), len(a), _ptr(uniq), len(uniq), _ptr(irep),
+            _ptr(start), _ptr(end))
+    N = lib().orc_createtwordm_bit_fci(*args, None, 0)
+    if N < 0:
+        raise ValueError("createtwordm_bit_fci: a string is not among the unique alpha strings")
+    tw = np.zeros((N,) * 4, order="F")
+    lib().orc_createtwordm_bit_fci(*args, _ptr(tw), N)
+    return tw
+
+
+def createtwordm_bit_fci(alpha, beta, civs):
+    """createtwordm_bit_fci rows (p,s,q,r) and values (src/one_rdm.f90:390-927)."""
+    return twordm_select(twordm_bit_fci_dense(alpha, beta, civs), BIT_CUTOFF)

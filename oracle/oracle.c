@@ -1943,3 +1943,222 @@ int orc_one_rdm_bit(const int64_t *alpha, const int64_t *beta, const double *civ
 #undef G1
     return N;
 }
+
+/* createtwordm_bit_fci, src/one_rdm.f90:390-927, with the tables of single
+ * and double excitations of every unique alpha string (:474-657) and the
+ * main loop over the file's determinants (:668-875).  uniq holds
+ * newdat(:,1) (nuniq strings, increasing), irep/start/end the driver's
+ * per-string count and 1-based first / last file position
+ * (src/scattering_main_routine.py:269-289).  Returns norbs, or -1 on input
+ * the reference cannot process (see the header). */
+int orc_createtwordm_bit_fci(const int64_t *alpha, const int64_t *beta, const double *civs,
+                             int64_t n, const int64_t *uniq, int64_t nuniq,
+                             const int64_t *irep, const int64_t *start, const int64_t *end,
+                             double *tw, int cap_norb)
+{
+    if (n <= 0 || nuniq <= 0) return -1;
+    int64_t mx = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (alpha[i] > mx) mx = alpha[i];
+    int N = 0;
+    while (mx >= 1) {
+        ++N;
+        mx /= 2;
+    }
+    if (N == 0) return -1;
+    if (!tw || N > cap_norb) return N;
+    const size_t NN = (size_t)N;
+    memset(tw, 0, sizeof(double) * NN * NN * NN * NN);
+#define TWF(p, q, r, s) \
+    tw[(size_t)((p)-1) + NN * ((size_t)((q)-1) + NN * ((size_t)((r)-1) + NN * (size_t)((s)-1)))]
+    const uint64_t comp = (N >= 64) ? ~(uint64_t)0 : ((((uint64_t)1) << N) - 1);
+    const int nel = bit_popc((uint64_t)uniq[0]);
+    const int nsg = nel * (N - nel) + 1, ndb = nel * (nel - 1) / 2 * (N - nel) * (N - nel - 1) / 2 + 1;
+    /* single excitations (:498-559): added orbital j (unoccupied, increasing)
+     * outer, removed orbital k (occupied, increasing) inner */
+    int64_t *sx = calloc((size_t)nuniq * nsg, sizeof(int64_t));
+    int *sh = calloc((size_t)nuniq * nsg, sizeof(int)), *sp = calloc((size_t)nuniq * nsg, sizeof(int));
+    int *sph = calloc((size_t)nuniq * nsg, sizeof(int)), *sn = calloc((size_t)nuniq, sizeof(int));
+    int64_t *sidx = calloc((size_t)nuniq * nsg, sizeof(int64_t));
+    int64_t *dx = calloc((size_t)nuniq * ndb, sizeof(int64_t));
+    int *dh1 = calloc((size_t)nuniq * ndb, sizeof(int)), *dh2 = calloc((size_t)nuniq * ndb, sizeof(int));
+    int *dp1 = calloc((size_t)nuniq * ndb, sizeof(int)), *dp2 = calloc((size_t)nuniq * ndb, sizeof(int));
+    int *dph = calloc((size_t)nuniq * ndb, sizeof(int)), *dn = calloc((size_t)nuniq, sizeof(int));
+    int64_t *didx = calloc((size_t)nuniq * ndb, sizeof(int64_t));
+    for (int64_t i = 0; i < nuniq; ++i) {
+        const uint64_t s0 = (uint64_t)uniq[i];
+        int c2 = 0;
+        uint64_t b = s0 ^ comp;
+        while (b) {
+            const int j = bit_ctz(b);
+            uint64_t b2 = s0;
+            while (b2) {
+                const int k = bit_ctz(b2);
+                b2 &= b2 - 1;
+                const uint64_t t = (s0 | ((uint64_t)1 << j)) & ~((uint64_t)1 << k);
+                int64_t idx = -1;
+                for (int64_t l = 0; l < nuniq; ++l)
+                    if ((uint64_t)uniq[l] == t) {
+                        idx = l;
+                        break;
+                    }
+                if (idx < 0 || irep[idx] == 0) continue;
+                const int lo = (j + 1) < (k + 1) ? (j + 1) : (k + 1);
+                const int hi = (j + 1) < (k + 1) ? (k + 1) : (j + 1);
+                sx[i * nsg + c2] = (int64_t)t;
+                sh[i * nsg + c2] = k + 1;
+                sp[i * nsg + c2] = j + 1;
+                sph[i * nsg + c2] = bit_popc(s0 & bit_between(lo, hi));
+                sidx[i * nsg + c2] = idx;
+                ++c2;
+            }
+            b &= b - 1;
+        }
+        sn[i] = c2;
+        /* double excitations (:586-657): occ i<j, unocc k<l */
+        int occ[64], unocc[64], no = 0, nu = 0;
+        for (int k = 0; k < N; ++k) {
+            if ((s0 >> k) & 1) occ[no++] = k;
+            else unocc[nu++] = k;
+        }
+        int c3 = 0;
+        for (int a = 0; a < no - 1; ++a)
+            for (int bb = a + 1; bb < no; ++bb)
+                for (int k = 0; k < nu - 1; ++k)
+                    for (int l = k + 1; l < nu; ++l) {
+                        const uint64_t t = (((s0 & ~((uint64_t)1 << occ[a])) & ~((uint64_t)1 << occ[bb])) |
+                                            ((uint64_t)1 << unocc[k])) | ((uint64_t)1 << unocc[l]);
+                        int64_t idx = -1;
+                        for (int64_t m = 0; m < nuniq; ++m)
+                            if ((uint64_t)uniq[m] == t) {
+                                idx = m;
+                                break;
+                            }
+                        if (idx < 0 || irep[idx] == 0) continue;
+                        const int l1 = occ[a] + 1 < unocc[k] + 1 ? occ[a] + 1 : unocc[k] + 1;
+                        const int h1 = occ[a] + 1 < unocc[k] + 1 ? unocc[k] + 1 : occ[a] + 1;
+                        const int l2 = occ[bb] + 1 < unocc[l] + 1 ? occ[bb] + 1 : unocc[l] + 1;
+                        const int h2 = occ[bb] + 1 < unocc[l] + 1 ? unocc[l] + 1 : occ[bb] + 1;
+                        int ep = bit_popc(s0 & bit_between(l1, h1)) + bit_popc(s0 & bit_between(l2, h2));
+                        if (l2 > l1 && l2 < h1 && h2 > h1) ++ep;
+                        dx[i * ndb + c3] = (int64_t)t;
+                        dh1[i * ndb + c3] = occ[a] + 1;
+                        dh2[i * ndb + c3] = occ[bb] + 1;
+                        dp1[i * ndb + c3] = unocc[k] + 1;
+                        dp2[i * ndb + c3] = unocc[l] + 1;
+                        dph[i * ndb + c3] = ep;
+                        didx[i * ndb + c3] = idx;
+                        ++c3;
+                    }
+        dn[i] = c3;
+    }
+    /* first file position (0-based) of beta string bs in the block of
+     * unique alpha string u, or -1 (findloc over Nalphbet(1,2,ss:ee)) */
+#define FINDB(u, bs, out)                                              \
+    do {                                                               \
+        (out) = -1;                                                    \
+        for (int64_t z = start[(u)] - 1; z <= end[(u)] - 1; ++z)       \
+            if (beta[z] == (bs)) {                                     \
+                (out) = z;                                             \
+                break;                                                 \
+            }                                                          \
+    } while (0)
+    for (int64_t c1 = 0; c1 < n; ++c1) {
+        const double civ1 = civs[c1];
+        const double c = civ1 * civ1;
+        const int64_t *ab[2] = {alpha, beta};
+        for (int s1 = 0; s1 < 2; ++s1) {
+            uint64_t b1 = (uint64_t)ab[s1][c1];
+            while (b1) {
+                const int so = bit_ctz(b1) + 1;
+                b1 &= b1 - 1;
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    uint64_t b2 = (uint64_t)ab[s2][c1];
+                    while (b2) {
+                        const int qo = bit_ctz(b2) + 1;
+                        b2 &= b2 - 1;
+                        if (qo == so && s1 == s2) continue;
+                        if (s1 == s2) TWF(so, qo, so, qo) = TWF(so, qo, so, qo) - c;
+                        TWF(qo, so, so, qo) = TWF(qo, so, so, qo) + c;
+                    }
+                }
+            }
+        }
+        int64_t i1 = -1, i2 = -1;
+        for (int64_t m = 0; m < nuniq; ++m)
+            if (uniq[m] == alpha[c1]) {
+                i1 = m;
+                break;
+            }
+        for (int64_t m = 0; m < nuniq; ++m)
+            if (uniq[m] == beta[c1]) {
+                i2 = m;
+                break;
+            }
+        if (i1 < 0 || i2 < 0) {
+            free(sx); free(sh); free(sp); free(sph); free(sn); free(sidx);
+            free(dx); free(dh1); free(dh2); free(dp1); free(dp2); free(dph); free(dn); free(didx);
+            return -1;
+        }
+        for (int i = 0; i < sn[i1]; ++i) {
+            const int64_t u = sidx[i1 * nsg + i];
+            int64_t idx;
+            FINDB(u, beta[c1], idx);
+            if (idx >= 0) {
+                const double civ2 = civs[idx];
+                const int po = sh[i1 * nsg + i], qo = sp[i1 * nsg + i], ep = sph[i1 * nsg + i];
+                const double ct = (((ep & 1) ? -1.0 : 1.0) * 2.0 * civ1) * civ2;
+                uint64_t b = (uint64_t)alpha[c1] & ~((uint64_t)1 << (po - 1));
+                while (b) {
+                    const int ro = bit_ctz(b) + 1;
+                    b &= b - 1;
+                    TWF(po, ro, ro, qo) = TWF(po, ro, ro, qo) + ct;
+                    TWF(ro, po, qo, ro) = TWF(ro, po, qo, ro) + ct;
+                    TWF(ro, po, ro, qo) = TWF(ro, po, ro, qo) - ct;
+                    TWF(po, ro, qo, ro) = TWF(po, ro, qo, ro) - ct;
+                }
+                b = (uint64_t)beta[c1];
+                while (b) {
+                    const int ro = bit_ctz(b) + 1;
+                    b &= b - 1;
+                    TWF(po, ro, ro, qo) = TWF(po, ro, ro, qo) + ct;
+                    TWF(ro, po, qo, ro) = TWF(ro, po, qo, ro) + ct;
+                }
+            }
+            /* alpha-beta doubles: the single excitations of the beta string
+             * (taken from the alpha-string table) in the block of this alpha
+             * single (:783-812) */
+            for (int j = 0; j < sn[i2]; ++j) {
+                int64_t id2;
+                FINDB(u, sx[i2 * nsg + j], id2);
+                if (id2 < 0) continue;
+                const double civ2 = civs[id2];
+                const int ro = sh[i1 * nsg + i], so = sp[i1 * nsg + i];
+                const int po = sh[i2 * nsg + j], qo = sp[i2 * nsg + j];
+                const int ep = sph[i1 * nsg + i] + sph[i2 * nsg + j];
+                const double ct = ((1.0 * ((ep & 1) ? -1.0 : 1.0)) * civ1) * civ2;
+                TWF(po, ro, so, qo) = TWF(po, ro, so, qo) + ct;
+                TWF(ro, po, qo, so) = TWF(ro, po, qo, so) + ct;
+            }
+        }
+        for (int i = 0; i < dn[i1]; ++i) {
+            int64_t idx;
+            FINDB(didx[i1 * ndb + i], beta[c1], idx);
+            if (idx < 0) continue;
+            const double civ2 = civs[idx];
+            const int ep = dph[i1 * ndb + i];
+            const int so = dp1[i1 * ndb + i], qo = dp2[i1 * ndb + i];
+            const int po = dh2[i1 * ndb + i], ro = dh1[i1 * ndb + i];
+            const double ct = (((ep & 1) ? -1.0 : 1.0) * 2.0 * civ1) * civ2;
+            TWF(po, ro, so, qo) = TWF(po, ro, so, qo) + ct;
+            TWF(ro, po, so, qo) = TWF(ro, po, so, qo) - ct;
+            TWF(po, ro, qo, so) = TWF(po, ro, qo, so) - ct;
+            TWF(ro, po, qo, so) = TWF(ro, po, qo, so) + ct;
+        }
+    }
+#undef FINDB
+#undef TWF
+    free(sx); free(sh); free(sp); free(sph); free(sn); free(sidx);
+    free(dx); free(dh1); free(dh2); free(dp1); free(dp2); free(dph); free(dn); free(didx);
+    return N;
+}

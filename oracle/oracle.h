@@ -187,6 +187,19 @@ int orc_createtwordm_bit(const int64_t *alpha, const int64_t *beta, const double
  * civs is the third column of the bit file.  Same return convention. */
 int orc_one_rdm_bit(const int64_t *alpha, const int64_t *beta, const double *civs,
                     int64_t n, double *g, int cap_norb);
+/* createtwordm_bit_fci, src/one_rdm.f90:390-927: dense twordm (norbs^4,
+ * norbs the highest alpha orbital) from one coefficient column civs and the
+ * driver's unique-alpha tables (src/scattering_main_routine.py:269-289):
+ * uniq = newdat(:,1) (nuniq strings), irep (determinants per string), start
+ * / end (1-based first and last file position).  The lookups follow the
+ * reference: excited strings are searched in uniq, determinants in the
+ * string's [start, end] block by beta string.  Returns norbs (writes when
+ * tw != NULL and norbs <= cap_norb), or -1 when a determinant's alpha or
+ * beta string is not in uniq. */
+int orc_createtwordm_bit_fci(const int64_t *alpha, const int64_t *beta, const double *civs,
+                             int64_t n, const int64_t *uniq, int64_t nuniq,
+                             const int64_t *irep, const int64_t *start, const int64_t *end,
+                             double *tw, int cap_norb);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,9 @@
 !   route: mcci_to_bit('in.dat', 'cv.dat') then createtwordm_bit('cv.dat'),
 !          the type-1 route of total_scattering_calculation_2
 !   one  : one_rdm_bit('in.dat') -> maxnmo, onerdm(maxnmo,maxnmo)
+!   fci  : createtwordm_bit_fci('in.dat', ...) with newdat, irep, start and
+!          end read from fci.bin (stream: nuniq, newdat(nuniq,2), irep,
+!          start, end, all int64) -> nmat, mat(nmat,4), total(nmat)
 program ref_bit
     use onerdm
     implicit none
@@ -25,7 +28,9 @@ program ref_bit
     integer :: n4, dummy
     integer(ik), allocatable :: mat(:,:)
     real(dp), allocatable :: total(:), g(:,:), c1(:), c2(:)
-    integer*8, allocatable :: newdat(:,:), irep(:)
+    integer*8, allocatable :: newdat(:,:), irep(:), st(:), en(:)
+    integer*8 :: nu
+    character(len=20) :: f20
     integer :: a4, b4
 
     call get_command_argument(1, mode)
@@ -69,6 +74,18 @@ program ref_bit
         call one_rdm_bit(f30, g, maxnmo, n, newdat, irep)
         write(20) maxnmo
         write(20) g
+    case ('fci')
+        open(11, file='fci.bin', access='stream', form='unformatted', status='old')
+        read(11) nu
+        allocate(newdat(nu, 2), irep(nu), st(nu), en(nu))
+        read(11) newdat, irep, st, en
+        close(11)
+        f20 = 'in.dat'
+        call createtwordm_bit_fci(f20, n, newdat, irep, st, en, mat, total)
+        nmat = size(total)
+        write(20) nmat
+        write(20) mat(:, 1), mat(:, 2), mat(:, 3), mat(:, 4)
+        write(20) total
     end select
     close(20)
 end program ref_bit

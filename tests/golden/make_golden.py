@@ -8,9 +8,12 @@ then runs oracle/_ref/ref_driver, which calls the reference's set_P0,
 maxcoincidence, createtwordm, tot_integration, set_P0_j2,
 tot_integration_j2, onerdm_creat, elastic_integration, elastic_integration_j2
 and unique_total.  Inputs and outputs are saved as
-compressed .npz fixtures beside this script.
+compressed .npz fixtures beside this script.  `bit` runs the bit-string
+builders (mcci_to_bit, createtwordm_bit, one_rdm_bit) through
+oracle/_ref/ref_bit, `fci` createtwordm_bit_fci through oracle/_ref/ref_bit_fci,
+`vars` variables_total through oracle/_ref/ref_vars.
 
-    python tests/golden/make_golden.py [case ...]
+    python tests/golden/make_golden.py [case ... | bit | fci | vars]
 """
 from __future__ import annotations
 
@@ -239,7 +242,7 @@ def bit_dets(seed, norb, ne, n):
 
 
 def mrci_dets():
-    """src/MRCI_CI.dat (the reference's MRCI vector, 10 orbitals, 1368
+    """src/MRCI_CI.dat (the reference's MRCI vector, 10 orbitals, 1369
     rows) as bit strings: '2' on both spins, '/' alpha, '\\' beta; the
     printed coefficient on both columns."""
     al, be, c = [], [], []
@@ -264,7 +267,7 @@ BIT_CASES = {
     # 31 orbitals (the highest the reference's 32-bit strings hold), 3+3
     # electrons, 300 determinants: phase masks up to bit 30
     "wide": lambda: bit_dets(23, 31, 3, 300),
-    # the reference's MRCI vector
+    # the reference's MRCI vector (1369 rows)
     "mrci": mrci_dets,
 }
 
@@ -311,7 +314,75 @@ def save_bit(name):
           f"one-equal={np.array_equal(g, og)} size={os.path.getsize(path)}B", flush=True)
 
 
+REF_BIT_FCI = os.path.join(ROOT, "oracle", "_ref", "ref_bit_fci")
+
+
+def fci_space(seed, norb, ne, drop=0.0):
+    """Every (alpha, beta) pair of ne-electron strings in norb orbitals, alpha
+    major (each alpha string's determinants contiguous, as an FCI code lists
+    them), random normalised coefficients; drop removes that random fraction
+    of the determinants (the driver's |c| >= 1e-8 selection, made visible)."""
+    import itertools
+    rng = np.random.default_rng(seed)
+    strs = [sum(1 << k for k in c) for c in itertools.combinations(range(norb), ne)]
+    al = np.array([a for a in strs for _ in strs], dtype=np.int64)
+    be = np.array([b for _ in strs for b in strs], dtype=np.int64)
+    c = rng.standard_normal(al.size)
+    c /= np.linalg.norm(c)
+    if drop:
+        keep = rng.random(al.size) >= drop
+        al, be, c = al[keep], be[keep], c[keep]
+    return al, be, c
+
+
+FCI_CASES = {
+    "small": lambda: fci_space(41, 5, 2),
+    "mid": lambda: fci_space(42, 6, 3),
+    "drop": lambda: fci_space(43, 7, 3, drop=0.3),
+}
+
+
+def save_fci(name):
+    """createtwordm_bit_fci of the reference (oracle/_ref/ref_bit_fci: its
+    combo memo table zero-initialised, see oracle/Makefile) on one case,
+    with the driver's newdat / irep / start / end; checked against the
+    oracle."""
+    from thistle_amd import readers
+    al, be, c = FCI_CASES[name]()
+    uniq, irep, start, end = readers.fci_tables(al)
+    ub = np.unique(be)
+    ub = np.concatenate([ub, np.zeros(uniq.size - ub.size, dtype=np.int64)])
+    newdat = np.stack([uniq, ub], axis=1)
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "in.dat"), "w") as f:
+            for i in range(al.size):
+                f.write(f"{i} {float(c[i])!r} {al[i]} {be[i]}\n")
+        with open(os.path.join(d, "fci.bin"), "wb") as f:
+            f.write(np.int64(uniq.size).tobytes())
+            f.write(np.asfortranarray(newdat).tobytes(order="F"))
+            for x in (irep, start, end):
+                f.write(np.asarray(x, dtype=np.int64).tobytes())
+        subprocess.run([REF_BIT_FCI, "fci", str(al.size), "out.bin"], cwd=d, check=True,
+                       stdout=subprocess.DEVNULL)
+        with open(os.path.join(d, "out.bin"), "rb") as f:
+            raw = f.read()
+    nm = int(np.frombuffer(raw[:4], np.int32)[0])
+    mat = np.frombuffer(raw[4:4 + 16 * nm], np.int32).reshape(4, nm).T.copy()
+    tot = np.frombuffer(raw[4 + 16 * nm:], np.float64).copy()
+    om, ot = oracle.createtwordm_bit_fci(al, be, c)
+    path = os.path.join(HERE, f"fci_{name}.npz")
+    np.savez_compressed(path, alpha=al, beta=be, civs=c, mat=mat.astype(np.int8), total=tot)
+    print(f"fci_{name}: n={al.size} rows={tot.size} "
+          f"equal={np.array_equal(mat, om) and np.array_equal(tot, ot)} "
+          f"size={os.path.getsize(path)}B", flush=True)
+
+
 def main(names):
+    if "fci" in names:
+        for b in FCI_CASES:
+            save_fci(b)
+        names = [n for n in names if n != "fci"]
+
     if "bit" in names:
         for b in BIT_CASES:
             save_bit(b)

@@ -174,3 +174,43 @@ def test_boundary_rejects_bad_bit_strings():
     assert lib.thistle_total_scattering_calculation_2(3, None, 3, p(good_a), p(good_b), p(c),
                                                       p(c), 0, None, None, p(out)) == 7
     assert L.RDM_CUTOFF == float(np.float32(1e-30))
+
+
+def test_fci_boundary_rejects_what_the_reference_cannot_process(golden):
+    """createtwordm_bit_fci's input checks, all before any device call."""
+    lib = L.load()
+    g = golden("fci_small")
+    a, b, c = g["alpha"].copy(), g["beta"].copy(), g["civs"].copy()
+    p = L._ptr
+
+    def call(a, b, c, uniq, irep, start, end):
+        return lib.thistle_createtwordm_bit_fci(p(a), p(b), p(c), a.size, p(uniq), uniq.size,
+                                                p(irep), p(start), p(end), 0.0, None, None,
+                                                None, 0)
+
+    uniq, irep, start, end = readers.fci_tables(a)
+    # a determinant moved out of its alpha string's block
+    perm = np.arange(a.size)
+    perm[[3, 40]] = perm[[40, 3]]
+    u2, r2, s2, e2 = readers.fci_tables(a[perm])
+    assert call(a[perm].copy(), b[perm].copy(), c[perm].copy(), u2, r2, s2, e2) == -1
+    # a zero-padded string row (the driver's padding when there are more
+    # unique beta strings than alpha strings)
+    z = np.concatenate([[0], uniq]).astype(np.int64)
+    assert call(a, b, c, z, np.concatenate([[0], irep]), np.concatenate([[1], start]),
+                np.concatenate([[1], end])) == -1
+    # a beta string that is no alpha string
+    b2 = b.copy()
+    b2[0] = 0b10001 | 0b00100  # three electrons
+    assert call(a, b2, c, uniq, irep, start, end) == -1
+    # fewer unique strings than norbs + 1
+    keep = np.isin(a, uniq[:4]) & np.isin(b, uniq[:4])
+    u3, r3, s3, e3 = readers.fci_tables(a[keep])
+    assert call(a[keep].copy(), b[keep].copy(), c[keep].copy(), u3, r3, s3, e3) == -1
+
+
+def test_fci_tables_match_the_driver():
+    a = np.array([3, 3, 5, 5, 5, 6], dtype=np.int64)
+    uniq, irep, start, end = readers.fci_tables(a)
+    assert uniq.tolist() == [3, 5, 6] and irep.tolist() == [2, 3, 1]
+    assert start.tolist() == [1, 3, 6] and end.tolist() == [2, 5, 6]

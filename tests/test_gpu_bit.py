@@ -153,5 +153,64 @@ def test_route_readrdm_and_fci(golden, tmp_path):
             g["q"].size, g["group"], cz, cm, cc, "", p2, 0, None, None, None, None, None)
     S = sc.total_scattering_calculation_2(*args, False, None, None, None, None, True, False)[1]
     assert np.max(np.abs(S - g["S_total"]) / np.abs(g["S_total"])) < 1e-10
+    # the FCI file is read by the 2-RDM types only
     with pytest.raises(NotImplementedError):
-        sc.total_scattering_calculation_2(*args, True, None, None, None, None, False, False)
+        sc.total_scattering_calculation_2(2, *args[1:], True, None, None, None, None, False,
+                                          False)
+
+
+@pytest.mark.parametrize("name", ["small", "mid", "drop"])
+def test_fci_reference_goldens(golden, name):
+    """createtwordm_bit_fci against the reference's routine (flang build,
+    combo's memo table zero-initialised): same rows, identical doubles."""
+    g = golden(f"fci_{name}")
+    print(f"fci case {name}: n={g['alpha'].size}", flush=True)
+    assert same_rows(sc.createtwordm_bit_fci(g["alpha"], g["beta"], g["civs"]),
+                     (g["mat"].astype(np.int32), g["total"]))
+
+
+def fci_space(seed, norb, ne, drop=0.0):
+    import itertools
+    rng = np.random.default_rng(seed)
+    strs = [sum(1 << k for k in c) for c in itertools.combinations(range(norb), ne)]
+    al = np.array([a for a in strs for _ in strs], dtype=np.int64)
+    be = np.array([b for _ in strs for b in strs], dtype=np.int64)
+    c = rng.standard_normal(al.size)
+    c /= np.linalg.norm(c)
+    if drop:
+        keep = rng.random(al.size) >= drop
+        al, be, c = al[keep], be[keep], c[keep]
+    return al, be, c
+
+
+@pytest.mark.parametrize("seed,norb,ne,drop", [(44, 8, 4, 0.0), (45, 9, 3, 0.5)])
+def test_fci_vs_oracle(seed, norb, ne, drop):
+    a, b, c = fci_space(seed, norb, ne, drop)
+    print(f"fci oracle seed {seed}: n={a.size}", flush=True)
+    assert same_rows(sc.createtwordm_bit_fci(a, b, c), oracle.createtwordm_bit_fci(a, b, c))
+
+
+def test_route_fci(golden, tmp_path):
+    """total_scattering_calculation_2 with fci: types 1 and 7 from the FCI
+    file through createtwordm_bit_fci, against the oracle."""
+    g = golden("xyx_cas44")
+    a, b, c = fci_space(46, 7, 2)
+    path = str(tmp_path / "fci_calc.dat")
+    with open(path, "w") as f:
+        for i in range(a.size):
+            f.write(f"{i} {float(c[i])!r} {a[i]} {b[i]}\n")
+    uniq, irep, start, end = readers.fci_tables(a)
+    newdat = np.stack([uniq, np.unique(b)], axis=1)
+    cz, cm, cc = g["cutoffs"]
+    l = g["l"]
+    geo = (g["l"], g["m"], g["n"], g["group"], g["ga"], g["xx"], g["yy"], g["zz"])
+    kw = dict(cutoffz=cz, cutoffmd=cm, cutoffcentre=cc)
+    m0, t0 = oracle.createtwordm_bit_fci(a, b, c)
+    for typ, ref in ((1, oracle.total_j0(*geo, g["mmod"], m0, t0, g["q"], **kw)),
+                     (7, oracle.total_j2(*geo, g["mmod"], m0, t0, g["q"], **kw))):
+        S = sc.total_scattering_calculation_2(
+            typ, np.ones(3), np.zeros((3, 3)), 1, 1, int(l.max()), l.size,
+            int(g["group"].max()), g["ga"], l, g["m"], g["n"], g["xx"], g["yy"], g["zz"],
+            g["mmod"], g["q"], g["q"].size, g["group"], cz, cm, cc, path, "", a.size, newdat,
+            start, end, None, None, True, irep, None, None, None, False, False)[1]
+        assert _close(S, ref, 1e-11)

@@ -192,3 +192,13 @@ def test_bit_builders(golden, name):
     assert np.array_equal(m, g["route_mat"].astype(np.int32))
     assert np.array_equal(t, g["route_total"])
     assert np.array_equal(oracle.one_rdm_bit(a, b, c2), g["onerdm"])
+
+
+@pytest.mark.parametrize("name", ["small", "mid", "drop"])
+def test_fci_builder(golden, name):
+    """createtwordm_bit_fci against the reference's own routine (flang build
+    with its combo memo table zero-initialised): identical rows and doubles."""
+    g = golden(f"fci_{name}")
+    m, t = oracle.createtwordm_bit_fci(g["alpha"], g["beta"], g["civs"])
+    assert np.array_equal(m, g["mat"].astype(np.int32))
+    assert np.array_equal(t, g["total"])

@@ -2139,6 +2139,92 @@ int bit1_dense(const DevBuf& masks, int64_t n, int N, const double* civ, hipStre
     return THISTLE_OK;
 }
 
+// createtwordm_bit_fci on the device: every contribution of every file
+// determinant in the reference's order (fci_contribs), then the ordered
+// sums of rdm_accumulate's steps 4-5
+int fci_accumulate(const FciArgs& a, DevBuf& dense, hipStream_t st) {
+    const int64_t n4 = (int64_t)a.N * a.N * a.N * a.N;
+    DevBuf cnt, coff, keys, vals, keys2, vals2, temp, start;
+    DALLOC(cnt, sizeof(int64_t) * ((size_t)a.n + 1));
+    DALLOC(coff, sizeof(int64_t) * ((size_t)a.n + 1));
+    DCHECK(launch_fill(reinterpret_cast<double*>(cnt.p), a.n + 1, 0.0, st));
+    DCHECK(fci_ccount(a, cnt.as<int64_t>(), st));
+    size_t tb = 0;
+    DCHECK(rdm_scan_i64(nullptr, tb, cnt.as<int64_t>(), coff.as<int64_t>(), a.n + 1, st));
+    DALLOC(temp, tb);
+    DCHECK(rdm_scan_i64(temp.p, tb, cnt.as<int64_t>(), coff.as<int64_t>(), a.n + 1, st));
+    int64_t nc = 0;
+    DCHECK(hipMemcpyAsync(&nc, coff.as<int64_t>() + a.n, sizeof(int64_t), hipMemcpyDeviceToHost,
+                          st));
+    DCHECK(hipStreamSynchronize(st));
+    const size_t ncs = (size_t)std::max<int64_t>(nc, 1);
+    DALLOC(keys, sizeof(uint32_t) * ncs);
+    DALLOC(vals, sizeof(double) * ncs);
+    DALLOC(keys2, sizeof(uint32_t) * ncs);
+    DALLOC(vals2, sizeof(double) * ncs);
+    DCHECK(fci_cwrite(a, coff.as<int64_t>(), keys.as<uint32_t>(), vals.as<double>(), st));
+    int keybits = 1;
+    while (((int64_t)1 << keybits) < n4) ++keybits;
+    tb = 0;
+    DCHECK(rdm_sort(nullptr, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
+                    vals2.as<double>(), nc, keybits, st));
+    if (tb > temp.bytes) DALLOC(temp, tb);
+    if (nc > 0)
+        DCHECK(rdm_sort(temp.p, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
+                        vals2.as<double>(), nc, keybits, st));
+    DALLOC(start, sizeof(int64_t) * (size_t)(n4 + 1));
+    DALLOC(dense, sizeof(double) * (size_t)n4);
+    DCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, n4, start.as<int64_t>(),
+                    dense.as<double>(), st));
+    return THISTLE_OK;
+}
+
+// The inputs createtwordm_bit_fci can process: unique alpha strings
+// increasing, one electron count everywhere (the tables are sized from
+// the first string, :479, :565), at least norbs + 1 of them (:479 reads row
+// norbs + 1), every determinant's alpha and beta string among them (:719-721
+// look both up there), each string's determinants contiguous in the file
+// with start / end their first / last position, and orbitals 1-31 (the
+// 32-bit masks of :640-643).
+int fci_check(const int64_t* alpha, const int64_t* beta, const double* civs, int64_t n,
+              const int64_t* uniq, int64_t nuniq, const int64_t* irep, const int64_t* start,
+              const int64_t* end, int* N) {
+    if (!alpha || !beta || !civs || !uniq || !irep || !start || !end) return THISTLE_EINVAL;
+    if (n <= 0 || n > (int64_t)INT32_MAX || nuniq <= 0 || nuniq > (int64_t)INT32_MAX)
+        return THISTLE_EINVAL;
+    const int nel = __builtin_popcountll((uint64_t)uniq[0]);
+    int64_t mx = 0;
+    for (int64_t u = 0; u < nuniq; ++u) {
+        if (uniq[u] <= 0 || (u > 0 && uniq[u] <= uniq[u - 1])) return THISTLE_EINVAL;
+        if (__builtin_popcountll((uint64_t)uniq[u]) != nel) return THISTLE_EINVAL;
+        if (irep[u] < 0 || start[u] < 1 || end[u] < start[u] || end[u] > n) return THISTLE_EINVAL;
+    }
+    auto find = [&](int64_t s) -> int64_t {
+        const int64_t* p = std::lower_bound(uniq, uniq + nuniq, s);
+        return (p != uniq + nuniq && *p == s) ? (int64_t)(p - uniq) : -1;
+    };
+    std::vector<int64_t> first((size_t)nuniq, -1), last((size_t)nuniq, -1);
+    for (int64_t i = 0; i < n; ++i) {
+        mx = std::max(mx, alpha[i]);
+        const int64_t ua = find(alpha[i]);
+        if (ua < 0 || find(beta[i]) < 0) return THISTLE_EINVAL;
+        if (first[ua] < 0) first[ua] = i;
+        else if (last[ua] != i - 1) return THISTLE_EINVAL;  // not contiguous
+        last[ua] = i;
+    }
+    for (int64_t u = 0; u < nuniq; ++u)
+        if (first[u] < 0 || start[u] != first[u] + 1 || end[u] != last[u] + 1)
+            return THISTLE_EINVAL;
+    int w = 0;
+    while (mx) {
+        ++w;
+        mx >>= 1;
+    }
+    if (w == 0 || w > 31 || nuniq < w + 1) return THISTLE_EINVAL;
+    *N = w;
+    return THISTLE_OK;
+}
+
 #undef DALLOC
 #undef DCHECK
 
@@ -2331,6 +2417,46 @@ int32_t thistle_one_rdm_bit(const int64_t* alpha, const int64_t* beta, const dou
     DevBuf masks, civ;
     if ((rc = bits_to_device(alpha, beta, n, civs, civs, o->device, sg, masks, civ))) return rc;
     return bit1_dense(masks, n, N, civ.as<double>(), sg.s, onerdm);
+}
+
+int64_t thistle_createtwordm_bit_fci(const int64_t* alpha, const int64_t* beta,
+                                     const double* civs, int64_t n, const int64_t* uniq,
+                                     int64_t nuniq, const int64_t* irep, const int64_t* start,
+                                     const int64_t* end, double cutoff, const ThistleOpts* opts,
+                                     int32_t* mat, double* total, int64_t cap) {
+    if (cap > 0 && (!mat || !total)) return -THISTLE_EINVAL;
+    int N = 0;
+    int rc = fci_check(alpha, beta, civs, n, uniq, nuniq, irep, start, end, &N);
+    if (rc) return -rc;
+    const ThistleOpts* o = opts ? opts : &kDefaultOpts;
+    StreamGuard sg;
+    DevBuf masks, civ, du, direp, keys, keys2, idx, idx2, temp, tw;
+    if ((rc = bits_to_device(alpha, beta, n, civs, civs, o->device, sg, masks, civ))) return -rc;
+    if (dalloc(du, sizeof(uint64_t) * nuniq) || dalloc(direp, sizeof(int64_t) * nuniq) ||
+        dalloc(keys, sizeof(uint64_t) * n) || dalloc(keys2, sizeof(uint64_t) * n) ||
+        dalloc(idx, sizeof(int32_t) * n) || dalloc(idx2, sizeof(int32_t) * n))
+        return -THISTLE_ENOMEM;
+    if (hipMemcpyAsync(du.p, uniq, sizeof(int64_t) * nuniq, hipMemcpyHostToDevice, sg.s) !=
+            hipSuccess ||
+        hipMemcpyAsync(direp.p, irep, sizeof(int64_t) * nuniq, hipMemcpyHostToDevice, sg.s) !=
+            hipSuccess ||
+        fci_keys(masks.as<uint64_t>(), (int)n, keys.as<uint64_t>(), idx.as<int32_t>(), sg.s) !=
+            hipSuccess)
+        return -THISTLE_EDEVICE;
+    size_t tb = 0;
+    if (fci_key_sort(nullptr, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), idx.as<int32_t>(),
+                     idx2.as<int32_t>(), n, sg.s) != hipSuccess)
+        return -THISTLE_EDEVICE;
+    if (dalloc(temp, tb)) return -THISTLE_ENOMEM;
+    if (fci_key_sort(temp.p, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), idx.as<int32_t>(),
+                     idx2.as<int32_t>(), n, sg.s) != hipSuccess)
+        return -THISTLE_EDEVICE;
+    const FciArgs a{masks.as<uint64_t>(), civ.as<double>(), du.as<uint64_t>(),
+                    direp.as<int64_t>(), keys2.as<uint64_t>(), idx2.as<int32_t>(), (int)n,
+                    (int)nuniq, N};
+    if ((rc = fci_accumulate(a, tw, sg.s))) return -rc;
+    return twordm_select_device(tw, N, cutoff > 0 ? cutoff : THISTLE_RDM_CUTOFF,
+                                cap > 0 ? mat : nullptr, cap > 0 ? total : nullptr, cap, sg.s);
 }
 
 int32_t thistle_total_scattering_calculation_2(int32_t type, const ThistleBasis* basis,

@@ -674,6 +674,173 @@ __global__ void k_mcci_phase(const uint64_t* __restrict__ masks, int n,
     c2out[i] = ph * c2in[i];
 }
 
+// createtwordm_bit_fci (src/one_rdm.f90:390-875).  One item per file
+// determinant c1: its no-difference pass, then for every single excitation
+// A' of its alpha string A (added orbital j increasing, removed k increasing;
+// strings not among the unique alpha strings, or with no determinant,
+// skipped) the single contributions when (A', B) is a determinant, and for
+// every single excitation B' of its beta string (taken from the same
+// unique-alpha table) the alpha-beta double contributions when (A', B') is a
+// determinant; then every alpha-alpha double excitation (removed pair,
+// added pair, both increasing).  Determinants are found through the
+// (alpha, beta) keys sorted stably, i.e. the first in file order, which is
+// findloc's choice within a string's contiguous block.
+struct FciView {
+    const uint64_t* masks;   // (alpha, beta) per determinant
+    const double* civ;
+    const uint64_t* uniq;    // unique alpha strings, increasing
+    const int64_t* irep;     // determinants per unique string
+    const uint64_t* keys;    // alpha * 2^32 + beta, sorted stably
+    const int32_t* kidx;     // file position of each sorted key
+    int n, nuniq, N;
+};
+
+__device__ __forceinline__ bool fci_has_string(const FciView& V, uint64_t s) {
+    int lo = 0, hi = V.nuniq;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (V.uniq[mid] < s) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < V.nuniq && V.uniq[lo] == s && V.irep[lo] != 0;
+}
+
+__device__ __forceinline__ int fci_find_det(const FciView& V, uint64_t a, uint64_t b) {
+    const uint64_t key = (a << 32) | b;
+    int lo = 0, hi = V.n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (V.keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < V.n && V.keys[lo] == key) ? V.kidx[lo] : -1;
+}
+
+template <class F>
+__device__ void fci_contribs(const FciView& V, int c1, F&& emit) {
+    const int N = V.N;
+    auto TW = [N](int p, int q, int r, int s) -> uint32_t {
+        return (uint32_t)((p - 1) + N * ((q - 1) + N * ((r - 1) + N * (s - 1))));
+    };
+    const DMask<2> m = load_mask<2>(V.masks, c1);
+    const uint64_t A = m.w[0], B = m.w[1];
+    const uint64_t comp = N >= 64 ? ~0ull : ((1ull << N) - 1ull);
+    const double civ1 = V.civ[c1];
+    bit2_contribs(m, m, 0, true, civ1, civ1, N, emit);
+    // alpha singles, each followed by its alpha-beta doubles
+    uint64_t bj = A ^ comp;
+    while (bj) {
+        const int j = __builtin_ctzll(bj);
+        bj &= bj - 1;
+        uint64_t bk = A;
+        while (bk) {
+            const int k = __builtin_ctzll(bk);
+            bk &= bk - 1;
+            const uint64_t A1 = (A | (1ull << j)) & ~(1ull << k);
+            if (!fci_has_string(V, A1)) continue;
+            const int ep = __popcll(A & bit_between(j + 1, k + 1));
+            const int d = fci_find_det(V, A1, B);
+            if (d >= 0) {
+                const double ct = ((ep & 1) ? -1.0 : 1.0) * 2.0 * civ1 * V.civ[d];
+                const int po = k + 1, qo = j + 1;
+                uint64_t b = A & ~(1ull << (po - 1));
+                while (b) {
+                    const int ro = __builtin_ctzll(b) + 1;
+                    b &= b - 1;
+                    emit(TW(po, ro, ro, qo), ct);
+                    emit(TW(ro, po, qo, ro), ct);
+                    emit(TW(ro, po, ro, qo), -ct);
+                    emit(TW(po, ro, qo, ro), -ct);
+                }
+                b = B;
+                while (b) {
+                    const int ro = __builtin_ctzll(b) + 1;
+                    b &= b - 1;
+                    emit(TW(po, ro, ro, qo), ct);
+                    emit(TW(ro, po, qo, ro), ct);
+                }
+            }
+            uint64_t cj = B ^ comp;
+            while (cj) {
+                const int jb = __builtin_ctzll(cj);
+                cj &= cj - 1;
+                uint64_t ck = B;
+                while (ck) {
+                    const int kb = __builtin_ctzll(ck);
+                    ck &= ck - 1;
+                    const uint64_t B1 = (B | (1ull << jb)) & ~(1ull << kb);
+                    if (!fci_has_string(V, B1)) continue;
+                    const int d2 = fci_find_det(V, A1, B1);
+                    if (d2 < 0) continue;
+                    const int e2 = ep + __popcll(B & bit_between(jb + 1, kb + 1));
+                    const double ct = 1.0 * ((e2 & 1) ? -1.0 : 1.0) * civ1 * V.civ[d2];
+                    const int ro = k + 1, so = j + 1, po = kb + 1, qo = jb + 1;
+                    emit(TW(po, ro, so, qo), ct);
+                    emit(TW(ro, po, qo, so), ct);
+                }
+            }
+        }
+    }
+    // alpha-alpha doubles
+    int occ[64], unocc[64], no = 0, nu = 0;
+    for (int k = 0; k < N; ++k) {
+        if ((A >> k) & 1) occ[no++] = k;
+        else unocc[nu++] = k;
+    }
+    for (int a = 0; a + 1 < no; ++a)
+        for (int b2 = a + 1; b2 < no; ++b2)
+            for (int k = 0; k + 1 < nu; ++k)
+                for (int l = k + 1; l < nu; ++l) {
+                    const uint64_t A2 = (((A & ~(1ull << occ[a])) & ~(1ull << occ[b2])) |
+                                         (1ull << unocc[k])) | (1ull << unocc[l]);
+                    if (!fci_has_string(V, A2)) continue;
+                    const int d = fci_find_det(V, A2, B);
+                    if (d < 0) continue;
+                    const int l1 = min(occ[a], unocc[k]) + 1, h1 = max(occ[a], unocc[k]) + 1;
+                    const int l2 = min(occ[b2], unocc[l]) + 1, h2 = max(occ[b2], unocc[l]) + 1;
+                    int ep = __popcll(A & bit_between(l1, h1)) + __popcll(A & bit_between(l2, h2));
+                    if (l2 > l1 && l2 < h1 && h2 > h1) ++ep;
+                    const double ct = ((ep & 1) ? -1.0 : 1.0) * 2.0 * civ1 * V.civ[d];
+                    const int so = unocc[k] + 1, qo = unocc[l] + 1, po = occ[b2] + 1,
+                              ro = occ[a] + 1;
+                    emit(TW(po, ro, so, qo), ct);
+                    emit(TW(ro, po, so, qo), -ct);
+                    emit(TW(po, ro, qo, so), -ct);
+                    emit(TW(ro, po, qo, so), ct);
+                }
+}
+
+__global__ void k_fci_keys(const uint64_t* __restrict__ masks, int n, uint64_t* __restrict__ keys,
+                           int32_t* __restrict__ idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = (masks[2 * (size_t)i] << 32) | masks[2 * (size_t)i + 1];
+    idx[i] = i;
+}
+
+__global__ void k_fci_ccount(FciView V, int64_t* __restrict__ cnt) {
+    const int c1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c1 >= V.n) return;
+    int64_t k = 0;
+    fci_contribs(V, c1, [&](uint32_t, double) { ++k; });
+    cnt[c1] = k;
+}
+
+__global__ void k_fci_cwrite(FciView V, const int64_t* __restrict__ coff,
+                             uint32_t* __restrict__ keys, double* __restrict__ vals) {
+    const int c1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c1 >= V.n) return;
+    int64_t pos = coff[c1];
+    const int64_t end = coff[c1 + 1];  // k_fci_ccount's count for this determinant
+    fci_contribs(V, c1, [&](uint32_t k, double v) {
+        if (pos < end) {
+            keys[pos] = k;
+            vals[pos] = v;
+        }
+        ++pos;
+    });
+}
+
 inline unsigned nb64(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -857,5 +1024,28 @@ hipError_t mcci_phase(const uint64_t* masks, int n, const double* c1in, const do
                       double* c1out, double* c2out, hipStream_t s) {
     hipLaunchKernelGGL(k_mcci_phase, dim3(nb64(n, 256)), dim3(256), 0, s, masks, n, c1in, c2in,
                        c1out, c2out);
+    return hipGetLastError();
+}
+
+hipError_t fci_keys(const uint64_t* masks, int n, uint64_t* keys, int32_t* idx, hipStream_t s) {
+    hipLaunchKernelGGL(k_fci_keys, dim3(nb64(n, 256)), dim3(256), 0, s, masks, n, keys, idx);
+    return hipGetLastError();
+}
+
+hipError_t fci_key_sort(void* temp, size_t& bytes, const uint64_t* kin, uint64_t* kout,
+                        const int32_t* vin, int32_t* vout, int64_t n, hipStream_t s) {
+    return hipcub::DeviceRadixSort::SortPairs(temp, bytes, kin, kout, vin, vout, n, 0, 64, s);
+}
+
+hipError_t fci_ccount(const FciArgs& a, int64_t* cnt, hipStream_t s) {
+    const FciView V{a.masks, a.civ, a.uniq, a.irep, a.keys, a.kidx, a.n, a.nuniq, a.N};
+    hipLaunchKernelGGL(k_fci_ccount, dim3(nb64(a.n, 64)), dim3(64), 0, s, V, cnt);
+    return hipGetLastError();
+}
+
+hipError_t fci_cwrite(const FciArgs& a, const int64_t* coff, uint32_t* keys, double* vals,
+                      hipStream_t s) {
+    const FciView V{a.masks, a.civ, a.uniq, a.irep, a.keys, a.kidx, a.n, a.nuniq, a.N};
+    hipLaunchKernelGGL(k_fci_cwrite, dim3(nb64(a.n, 64)), dim3(64), 0, s, V, coff, keys, vals);
     return hipGetLastError();
 }

@@ -139,6 +139,23 @@ hipError_t bit1_cwrite(const uint64_t* masks, const int32_t* c1s, const int32_t*
 hipError_t mcci_phase(const uint64_t* masks, int n, const double* c1in, const double* c2in,
                       double* c1out, double* c2out, hipStream_t s);
 
+// createtwordm_bit_fci (rdm.hip): device tables of one call
+struct FciArgs {
+    const uint64_t* masks;   // (alpha, beta) per determinant
+    const double* civ;
+    const uint64_t* uniq;    // unique alpha strings, increasing
+    const int64_t* irep;
+    const uint64_t* keys;    // alpha * 2^32 + beta, sorted stably
+    const int32_t* kidx;
+    int n, nuniq, N;
+};
+hipError_t fci_keys(const uint64_t* masks, int n, uint64_t* keys, int32_t* idx, hipStream_t s);
+hipError_t fci_key_sort(void* temp, size_t& bytes, const uint64_t* kin, uint64_t* kout,
+                        const int32_t* vin, int32_t* vout, int64_t n, hipStream_t s);
+hipError_t fci_ccount(const FciArgs& a, int64_t* cnt, hipStream_t s);
+hipError_t fci_cwrite(const FciArgs& a, const int64_t* coff, uint32_t* keys, double* vals,
+                      hipStream_t s);
+
 // Host helpers (rdm_host.cpp)
 int thistle_twordm_dense_host(const int32_t* confs, int nconf, int lconfs,
                               const double* civs, int nstates, int state1,

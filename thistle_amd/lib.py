@@ -33,6 +33,7 @@ EXPORTS = [
     "thistle_elastic_from_onerdm", "thistle_twordm_dense", "thistle_twordm_select",
     "thistle_createtwordm", "thistle_nuclei_electron",
     "thistle_mcci_to_bit", "thistle_createtwordm_bit", "thistle_one_rdm_bit",
+    "thistle_createtwordm_bit_fci",
     "thistle_total_scattering_calculation_2",
     "thistle_onerdm", "thistle_createonerdm", "thistle_quartet_list",
     "thistle_quartet_list_screened",
@@ -103,6 +104,8 @@ def _declare(L):
     L.thistle_mcci_to_bit.restype = i32
     L.thistle_createtwordm_bit.argtypes = [p, p, p, p, i64, d, p, p, p, i64]
     L.thistle_createtwordm_bit.restype = i64
+    L.thistle_createtwordm_bit_fci.argtypes = [p, p, p, i64, p, i64, p, p, p, d, p, p, p, i64]
+    L.thistle_createtwordm_bit_fci.restype = i64
     L.thistle_one_rdm_bit.argtypes = [p, p, p, i64, p, i32, p, p]
     L.thistle_one_rdm_bit.restype = i32
     L.thistle_total_scattering_calculation_2.argtypes = [i32, p, i64, p, p, p, p, i32, p, p, p]

@@ -519,6 +519,37 @@ def read_bit_file(filename, numberlines=None):
             np.array(c1, dtype=np.float64), np.array(c2, dtype=np.float64))
 
 
+def read_fci_file(filename, numberlines=None):
+    """``i civ alpha beta`` lines, the FCI file createtwordm_bit_fci reads
+    (src/one_rdm.f90:443-446; written at src/scattering_main_routine.py:302-308)."""
+    al, be, c = [], [], []
+    with open(filename, "r") as fh:
+        for line in fh:
+            if numberlines is not None and len(al) >= numberlines:
+                break
+            tok = line.split()
+            if not tok:
+                continue
+            c.append(float(tok[1].replace("D", "E")))
+            al.append(int(tok[2]))
+            be.append(int(tok[3]))
+    if numberlines is not None and len(al) < numberlines:
+        raise ValueError(f"{filename}: {len(al)} lines, {numberlines} expected")
+    return (np.array(al, dtype=np.int64), np.array(be, dtype=np.int64),
+            np.array(c, dtype=np.float64))
+
+
+def fci_tables(alpha):
+    """newdat(:,1), irep, start and end as the driver builds them
+    (src/scattering_main_routine.py:269-289): the unique alpha strings in
+    increasing order, their determinant counts and their 1-based first and
+    last positions."""
+    alpha = np.asarray(alpha, dtype=np.int64)
+    uniq, first, counts = np.unique(alpha, return_index=True, return_counts=True)
+    last = np.array([np.nonzero(alpha == u)[0][-1] for u in uniq], dtype=np.int64)
+    return uniq, counts.astype(np.int64), first.astype(np.int64) + 1, last + 1
+
+
 def write_bit_file(filename, alpha, beta, c1, c2):
     """The same layout, coefficients written to round-trip exactly."""
     with open(filename, "w") as fh:

@@ -296,6 +296,32 @@ def one_rdm_bit(alpha, beta, civs):
     return g
 
 
+def createtwordm_bit_fci(alpha, beta, civs, uniq=None, irep=None, start=None, end=None,
+                         cutoff=L.RDM_CUTOFF):
+    """createtwordm_bit_fci (src/one_rdm.f90:390-927) on the GPU: rows
+    (p,s,q,r) and values, bit-identical to the reference's.  uniq (the
+    driver's newdat(:,1)), irep, start and end default to the driver's
+    construction from the alpha strings."""
+    from . import readers
+    lib = L.load()
+    a, b, c = _bits(alpha, beta, civs)
+    if uniq is None:
+        uniq, irep, start, end = readers.fci_tables(a)
+    u, r, s0, e0 = (np.require(np.asarray(x), dtype=np.int64, requirements=["C"])
+                    for x in (uniq, irep, start, end))
+    args = (L._ptr(a), L._ptr(b), L._ptr(c), a.size, L._ptr(u), u.size, L._ptr(r), L._ptr(s0),
+            L._ptr(e0), cutoff, None)
+    cnt = lib.thistle_createtwordm_bit_fci(*args, None, None, 0)
+    if cnt < 0:
+        L.check(-cnt, "createtwordm_bit_fci")
+    mat = np.zeros((max(cnt, 1), 4), dtype=np.int32, order="F")
+    tot = np.zeros(max(cnt, 1))
+    c2n = lib.thistle_createtwordm_bit_fci(*args, L._ptr(mat), L._ptr(tot), cnt)
+    if c2n < 0:
+        L.check(-c2n, "createtwordm_bit_fci")
+    return np.ascontiguousarray(mat[:cnt]), tot[:cnt].copy()
+
+
 def total_scattering_calculation_2(type, Zn, geom, state1, state2, maxl, ngto, ng, ga, l, m,
                                    n, xx, yy, zz, mmod, q, nq, group, cutoffz, cutoffmd,
                                    cutoffcentre, fileJ, fileJ_2, numberlines, newdat, start,
@@ -306,10 +332,11 @@ def total_scattering_calculation_2(type, Zn, geom, state1, state2, maxl, ngto, n
 
     fileJ is the ``i c1 c2 alpha beta`` file (numberlines lines); with readrdm
     types 1 / 7 read the 2-RDM file fileJ_2 and types 2 / 8 the 1-RDM file
-    fileJ instead.  newdat, start, end, start_2, end_2, irep1, irep2 and the
-    orderings feed only createtwordm_bit_fci, which both of the reference's
-    callers switch off (fci = False at src/scattering_main_routine.py:251 and
-    src/Main.f90:142,179); fci=True raises.  Returns (q_abs, result)."""
+    fileJ instead.  With fci (which both of the reference's callers switch
+    off: src/scattering_main_routine.py:251, src/Main.f90:142,179) types 1 / 7
+    build the 2-RDM with createtwordm_bit_fci from the ``i civ alpha beta``
+    file and newdat(:,1), irep1, start and end; start_2, end_2, irep2 and the
+    orderings are read by none of the routines.  Returns (q_abs, result)."""
     from . import readers
     lib = L.load()
     q = np.asarray(q, dtype=np.float64)
@@ -332,9 +359,16 @@ def total_scattering_calculation_2(type, Zn, geom, state1, state2, maxl, ngto, n
         f = elastic_from_onerdm if int(type) == 2 else elastic_j2_from_onerdm
         return q_abs, f(*basis, g, **kw)
     if fci:
-        raise NotImplementedError(
-            "createtwordm_bit_fci: the reference's callers never set fci "
-            "(src/scattering_main_routine.py:251, src/Main.f90:142,179)")
+        if int(type) not in (1, 7):
+            raise NotImplementedError(
+                f"type {type} with fci: the reference reads the FCI file only in the "
+                "2-RDM types 1 and 7 (src/main_calculation.f90:587-588, 731-732)")
+        a, b, c = readers.read_fci_file(fileJ, int(numberlines))
+        nd = np.asarray(newdat)
+        uniq = nd[:, 0] if nd.ndim == 2 else nd
+        mat, tot = createtwordm_bit_fci(a, b, c, uniq, irep1, start, end)
+        f = total_from_twordm if int(type) == 1 else total_j2_from_twordm
+        return q_abs, f(*basis, mat, tot, **kw)
     a, b, c1, c2 = readers.read_bit_file(fileJ, int(numberlines))
     a, b, c1, c2 = _bits(a, b, c1, c2)
     bb = _basis(ga, l, m, n, xx, yy, zz, mmod, q, group, cutoffz, cutoffmd, cutoffcentre,
