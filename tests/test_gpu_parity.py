@@ -231,3 +231,21 @@ def test_drop_in_types_5_6(golden, name):
         err = np.max(np.abs(res - ref)) / np.max(np.abs(ref))
         print(f"{name} type {typ}: normwise err {err:.3e}", flush=True)
         assert err < RTOL_FP64
+
+
+@pytest.mark.parametrize("name", ["sp3", "xyx_cas44"])
+def test_device_class_lists_match_host_bucketing(golden, name):
+    """The LL-class lists a plan builds on the device (class keys + stable
+    radix sort) against the host bucketing of thistle_plan_set_quartets over
+    every quartet: the same lists give bitwise the same S(q)."""
+    g = golden(name)
+    cz, cm, cc = g["cutoffs"]
+    plan = sc.Plan(1, *_args(g), g["mmod"], g["q"], g["group"], mat=g["mat"].astype(np.int32),
+                   total=g["total"], cutoffz=cz, cutoffmd=cm, cutoffcentre=cc)
+    try:
+        dev = plan.execute()
+        plan.set_quartets(None)
+        host = plan.execute()
+        assert np.array_equal(dev, host)
+    finally:
+        plan.close()
