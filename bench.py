@@ -281,7 +281,8 @@ def run_batch(args):
         "config": {"workload": f"batch: {len(frames)} frames of 2-12 atoms, 2s1p1d per atom, "
                                f"CAS(2,2)-(6,6), 64-512 q in [0, 10]",
                    "units": info["units"], "graph": info["graph"],
-                   "series_waves": info["waves"], "create_s": create_s,
+                   "series_waves": info["waves"], "rows_per_frame": info["rows"],
+                   "frame_groups": info["groups"], "create_s": create_s,
                    "tune_capture_s": prep_s},
     }
     print(json.dumps(line), flush=True)
@@ -300,6 +301,9 @@ def main():
                          "meets the per-q 1e-6 bound")
     ap.add_argument("--cpu-stride", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device to device three times (torch), the "
+                         "reference traffic of the FETCH_SIZE / WRITE_SIZE calibration")
     ap.add_argument("--per-q-out", default=None,
                     help="write q, S_fp64, S_mixed and the per-q relative error here")
     args = ap.parse_args()
@@ -308,6 +312,15 @@ def main():
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("# copied 3 x 1 GiB", file=sys.stderr)
+        return
     if args.workload == "batch":
         run_batch(args)
         return

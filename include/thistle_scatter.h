@@ -390,6 +390,10 @@ int32_t thistle_batch_execute(ThistleBatch *batch, double *S_out);
  * replayed (1) or the stages are enqueued eagerly (0). */
 int32_t thistle_batch_info(ThistleBatch *batch, double *ms_last,
                            int64_t *units, int32_t *waves, int32_t *graph);
+/* The batch's tuned configuration: partial rows (series chunks) per frame
+ * and frame groups of the cross-frame quartet stage (0, 0 when the batch
+ * runs per-frame launches).  No counterpart in the reference. */
+int32_t thistle_batch_tuning(ThistleBatch *b, int32_t *rows, int32_t *groups);
 void thistle_batch_destroy(ThistleBatch *batch);
 
 /* Number of HIP devices visible to the library (negative on error), and a

@@ -1594,6 +1594,13 @@ int32_t thistle_batch_info(ThistleBatch* b, double* ms_last, int64_t* units, int
     return THISTLE_OK;
 }
 
+int32_t thistle_batch_tuning(ThistleBatch* b, int32_t* rows, int32_t* groups) {
+    if (!b) return THISTLE_EINVAL;
+    if (rows) *rows = b->merged ? b->rows : 0;
+    if (groups) *groups = b->merged ? b->ngroups : 0;
+    return THISTLE_OK;
+}
+
 void thistle_batch_destroy(ThistleBatch* b) { delete b; }
 
 int32_t thistle_total_scattering_adjoint(const ThistleBasis* basis, int64_t nmat,

@@ -40,7 +40,7 @@ EXPORTS = [
     "thistle_unique_total", "thistle_unique_total_noalloc", "thistle_pair_tables", "thistle_plan_create",
     "thistle_plan_execute", "thistle_plan_result", "thistle_plan_timings",
     "thistle_plan_destroy", "thistle_plan_set_quartets", "thistle_plan_kernel_timings", "thistle_plan_mixed_fallback", "thistle_batch_create", "thistle_batch_execute",
-    "thistle_batch_info", "thistle_batch_destroy", "thistle_device_count",
+    "thistle_batch_info", "thistle_batch_tuning", "thistle_batch_destroy", "thistle_device_count",
     "thistle_device_synchronize",
     "thistle_version",
 ]
@@ -148,6 +148,8 @@ def _declare(L):
     L.thistle_batch_execute.restype = i32
     L.thistle_batch_info.argtypes = [p, p, p, p, p]
     L.thistle_batch_info.restype = i32
+    L.thistle_batch_tuning.argtypes = [p, p, p]
+    L.thistle_batch_tuning.restype = i32
     L.thistle_batch_destroy.argtypes = [p]
     L.thistle_batch_destroy.restype = None
     L.thistle_device_count.argtypes = []

@@ -590,8 +590,11 @@ class Batch:
         graph = C.c_int32()
         L.check(self._lib.thistle_batch_info(self._h, C.byref(ms), C.byref(units), C.byref(waves),
                                              C.byref(graph)), "batch_info")
+        rows, groups = C.c_int32(), C.c_int32()
+        L.check(self._lib.thistle_batch_tuning(self._h, C.byref(rows), C.byref(groups)),
+                "batch_tuning")
         return {"ms": ms.value, "units": units.value, "waves": waves.value,
-                "graph": bool(graph.value)}
+                "graph": bool(graph.value), "rows": rows.value, "groups": groups.value}
 
     def close(self):
         if self._h:
