@@ -67,3 +67,25 @@ def test_batch_frames_vs_oracle(frames):
         print(f"frame {i}: max rel err vs oracle {rel:.3e}", flush=True)
         assert rel < 1e-10
     b.close()
+
+
+def test_batch_per_frame_paths(frames):
+    """Batches that keep per-frame quartet launches: a frame with more than
+    512 q points (FP64), and mixed precision; each frame bitwise equal to its
+    single-frame run with the batch's waves."""
+    from thistle_amd import lib as L
+    inp, q, mat, tot = frames[0]
+    long_q = np.linspace(1e-10, 8.0, 600)
+    sets = [([_args(inp, long_q, mat, tot)] + [_args(*fr) for fr in frames[1:3]], L.FP64),
+            ([_args(*fr) for fr in frames[:3]], L.MIXED)]
+    for fr, prec in sets:
+        b = sc.Batch(fr, precision=prec)
+        res = b.execute()
+        info = b.info()
+        print(f"per-frame batch: precision {prec} waves={info['waves']} rows={info['rows']}",
+              flush=True)
+        assert info["rows"] == 0 and info["groups"] == 0
+        for k, a in enumerate(fr):
+            single = sc.total_from_twordm(*a, precision=prec, waves=info["waves"])
+            assert np.array_equal(single, res[k]), (prec, k)
+        b.close()

@@ -214,3 +214,37 @@ def test_route_fci(golden, tmp_path):
             g["mmod"], g["q"], g["q"].size, g["group"], cz, cm, cc, path, "", a.size, newdat,
             start, end, None, None, True, irep, None, None, None, False, False)[1]
         assert _close(S, ref, 1e-11)
+
+
+def test_driver_file_route(tmp_path):
+    """driver.main_file, the driver's external-file branch: a file of at most
+    300 lines (``i civ alpha beta``) through total_scattering_calculation and
+    a longer MCCI file (``i c1 c2 alpha beta``) through
+    total_scattering_calculation_2, against the punch-file route on the same
+    wavefunction (mcci_to_bit's 10-digit rounding of the coefficients is the
+    only difference)."""
+    from thistle_amd import driver, synth
+    for nact, nel, short in ((4, 4, True), (6, 6, False)):
+        w = synth.make_small(seed=71, natom=2, spec={"s": 1, "p": 1}, nact=nact, nelec=nel,
+                             ncore=1, nq=24, qmax=6.0)
+        mld, pun = synth.write_inputs(w, str(tmp_path))
+        ref, q, _ = driver.main(mld, pun, w.ncore)
+        inp = driver.prepare_inputs(mld, pun, w.ncore)
+        a = np.zeros(inp.confs.shape[0], dtype=np.int64)
+        b = np.zeros_like(a)
+        for k in range(inp.norb):
+            a |= (inp.confs[:, 2 * k] != 0).astype(np.int64) << k
+            b |= (inp.confs[:, 2 * k + 1] != 0).astype(np.int64) << k
+        c = inp.civs[:, 0]
+        path = str(tmp_path / f"bits_{nact}.dat")
+        with open(path, "w") as f:
+            for i in range(a.size):
+                if short:
+                    f.write(f"{i + 1} {float(c[i])!r} {a[i]} {b[i]}\n")
+                else:
+                    f.write(f"{i + 1} {float(c[i])!r} {float(c[i])!r} {a[i]} {b[i]}\n")
+        assert (a.size <= 300) == short
+        res, q2, _ = driver.main_file(mld, path, q=q, nmo_max=inp.norb)
+        rel = np.max(np.abs(res - ref) / np.abs(ref))
+        print(f"file route CAS({nel},{nact}) {a.size} dets: max rel {rel:.2e}", flush=True)
+        assert rel < (1e-12 if short else 1e-8)
