@@ -1815,6 +1815,30 @@ struct StreamGuard {
     }
 };
 
+// Steps 4-5 of the ordered accumulation: the nc contributions (element
+// keys, values) in the reference's global order, stably radix-sorted by
+// element, then one ordered sum per element from 0 into dense (nel sums).
+int sort_and_sum(DevBuf& keys, DevBuf& vals, DevBuf& keys2, DevBuf& vals2, DevBuf& temp,
+                 int64_t nc, int64_t nel, DevBuf& dense, hipStream_t st) {
+    int keybits = 1;
+    while (((int64_t)1 << keybits) < nel) ++keybits;
+    size_t tb = 0;
+    DCHECK(rdm_sort(nullptr, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
+                    vals2.as<double>(), nc, keybits, st));
+    if (tb > temp.bytes) DALLOC(temp, tb);
+    if (nc > 0)
+        DCHECK(rdm_sort(temp.p, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
+                        vals2.as<double>(), nc, keybits, st));
+    DevBuf start;
+    DALLOC(start, sizeof(int64_t) * (size_t)(nel + 1));
+    DALLOC(dense, sizeof(double) * (size_t)nel);
+    DCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, nel, start.as<int64_t>(),
+                    dense.as<double>(), st));
+    // start is freed on return: the sums are queued on st ahead of the free
+    DCHECK(hipStreamSynchronize(st));
+    return THISTLE_OK;
+}
+
 // The determinant builders of rdm.hip share one ordered accumulation: an
 // item list in the reference's loop order ((c1, c2) pairs with ndiff <=
 // maxdiff, after ndiag leading no-difference items for the bit builders),
@@ -1831,7 +1855,7 @@ int rdm_accumulate(RdmKind kind, const uint64_t* masks, int nconf, int nw, int N
     const int64_t nel = (kind == RdmKind::Twordm || kind == RdmKind::Bit2)
                             ? (int64_t)N * N * N * N
                             : (int64_t)N * N;
-    DevBuf cnt, off, c1s, c2s, nds, ccnt, coff, keys, vals, keys2, vals2, temp, start;
+    DevBuf cnt, off, c1s, c2s, nds, ccnt, coff, keys, vals, keys2, vals2, temp;
     // 1. connected pairs per c1, c2 increasing
     DALLOC(cnt, sizeof(int64_t) * ((size_t)nconf + 1));
     DALLOC(off, sizeof(int64_t) * ((size_t)nconf + 1));
@@ -1905,21 +1929,7 @@ int rdm_accumulate(RdmKind kind, const uint64_t* masks, int nconf, int nw, int N
             DCHECK(bit1_cwrite(masks, i1, i2, id, ni, (int)ndiag, c1v, co, N, k, v, st));
             break;
     }
-    // 4. stable sort by element
-    int keybits = 1;
-    while (((int64_t)1 << keybits) < nel) ++keybits;
-    tb = 0;
-    DCHECK(rdm_sort(nullptr, tb, k, keys2.as<uint32_t>(), v, vals2.as<double>(), nc, keybits, st));
-    if (tb > temp.bytes) DALLOC(temp, tb);
-    if (nc > 0)
-        DCHECK(rdm_sort(temp.p, tb, k, keys2.as<uint32_t>(), v, vals2.as<double>(), nc, keybits,
-                        st));
-    // 5. ordered sums per element
-    DALLOC(start, sizeof(int64_t) * (size_t)(nel + 1));
-    DALLOC(dense, sizeof(double) * (size_t)nel);
-    DCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, nel, start.as<int64_t>(),
-                    dense.as<double>(), st));
-    return THISTLE_OK;
+    return sort_and_sum(keys, vals, keys2, vals2, temp, nc, nel, dense, st);
 }
 
 // 6. the selection loop (p, q, r, s) of a dense N^4 twordm: count of rows,
@@ -2151,7 +2161,7 @@ int bit1_dense(const DevBuf& masks, int64_t n, int N, const double* civ, hipStre
 // sums of rdm_accumulate's steps 4-5
 int fci_accumulate(const FciArgs& a, DevBuf& dense, hipStream_t st) {
     const int64_t n4 = (int64_t)a.N * a.N * a.N * a.N;
-    DevBuf cnt, coff, keys, vals, keys2, vals2, temp, start;
+    DevBuf cnt, coff, keys, vals, keys2, vals2, temp;
     DALLOC(cnt, sizeof(int64_t) * ((size_t)a.n + 1));
     DALLOC(coff, sizeof(int64_t) * ((size_t)a.n + 1));
     DCHECK(launch_fill(reinterpret_cast<double*>(cnt.p), a.n + 1, 0.0, st));
@@ -2170,20 +2180,7 @@ int fci_accumulate(const FciArgs& a, DevBuf& dense, hipStream_t st) {
     DALLOC(keys2, sizeof(uint32_t) * ncs);
     DALLOC(vals2, sizeof(double) * ncs);
     DCHECK(fci_cwrite(a, coff.as<int64_t>(), keys.as<uint32_t>(), vals.as<double>(), st));
-    int keybits = 1;
-    while (((int64_t)1 << keybits) < n4) ++keybits;
-    tb = 0;
-    DCHECK(rdm_sort(nullptr, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
-                    vals2.as<double>(), nc, keybits, st));
-    if (tb > temp.bytes) DALLOC(temp, tb);
-    if (nc > 0)
-        DCHECK(rdm_sort(temp.p, tb, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<double>(),
-                        vals2.as<double>(), nc, keybits, st));
-    DALLOC(start, sizeof(int64_t) * (size_t)(n4 + 1));
-    DALLOC(dense, sizeof(double) * (size_t)n4);
-    DCHECK(rdm_esum(keys2.as<uint32_t>(), vals2.as<double>(), nc, n4, start.as<int64_t>(),
-                    dense.as<double>(), st));
-    return THISTLE_OK;
+    return sort_and_sum(keys, vals, keys2, vals2, temp, nc, n4, dense, st);
 }
 
 // The inputs createtwordm_bit_fci can process: unique alpha strings
