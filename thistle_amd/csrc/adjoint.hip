@@ -178,7 +178,11 @@ __device__ __forceinline__ void b_all(double (&B)[M + 1], double q, double h, do
         v = vm;
         gg[b - 1] = v;
     }
-    const double nrm = j0 * rcp64(v);
+    // normalised by j0 or j1 = x g_1, whichever is larger in magnitude (j0
+    // alone loses accuracy next to its zeros)
+    const double j1 = x > 0.0 ? (j0 - cx) * inv : 0.0;
+    const bool by0 = fabs(j0) >= fabs(j1);
+    const double nrm = (by0 ? j0 : j1) * rcp64(by0 ? v : x * vp1);
     const double q2 = q * q;
     double qp = 1.0;
 #pragma unroll

@@ -409,17 +409,17 @@ __device__ __noinline__ double f_dphrec(const double* __restrict__ hs, double q,
     return F;
 }
 
-// Miller backward recurrence for g_b = j_b(x)/x^b, normalised by
-// g_0 = sin(x)/x:  F = sum_b h(b) q^(2b) g_b.  FP64 (small arguments, where
-// the terms cancel); fixed trip count so that a wave does not diverge.
-template <int LL>
+// Miller backward recurrence for g_b = j_b(x)/x^b started M orders above LL,
+// normalised by g_0 = j0 or by x g_1 = j1, whichever of j0, j1 is larger in
+// magnitude: F = sum_b h(b) q^(2b) g_b.  FP64 (small arguments, where the
+// terms cancel); fixed trip count so that a wave does not diverge.
+template <int LL, int M>
 __device__ __forceinline__ double f_miller(const double (&hs)[LL + 1], double q, double x,
-                                           double j0) {
+                                           double j0, double j1) {
     const double x2 = x * x, q2 = q * q;
-    constexpr int N = SeriesRegime<LL>::nstart;
     double vp1 = 0.0, v = 1.0;
 #pragma unroll
-    for (int b = N; b > LL; --b) {
+    for (int b = LL + M; b > LL; --b) {
         double vm = fma((double)(2 * b + 1), v, -x2 * vp1);
         vp1 = v;
         v = vm;
@@ -432,7 +432,8 @@ __device__ __forceinline__ double f_miller(const double (&hs)[LL + 1], double q,
         v = vm;
         acc = fma(acc, q2, hs[b - 1] * v);
     }
-    return acc * (j0 * rcp64(v));
+    const bool by0 = fabs(j0) >= fabs(j1);
+    return acc * ((by0 ? j0 : j1) * rcp64(by0 ? v : x * vp1));
 }
 
 // Upward recurrence on u_b = j_b(x) (q/h)^b, so that the series needs no
@@ -538,7 +539,7 @@ __device__ __forceinline__ double f_series(const double (&hs)[LL + 1], const dou
     // q = 0 (x = 0): the j0 limit 1 instead of 0/0
     const double j0 = x > 0.0 ? sx * inv : 1.0;
     if (LL == 0 || x >= SeriesRegime<LL>::xup) return f_up64<LL>(hs, cb, q, invh, j0, cx);
-    return f_miller<LL>(hs, q, x, j0);
+    return f_miller<LL, SeriesRegime<LL>::nstart - LL>(hs, q, x, j0, (j0 - cx) * inv);
 }
 
 // F(q) of one quartet.  J2: the series divided by q^2 (bessels0rr then
@@ -555,7 +556,8 @@ __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const doubl
             Fa = 0.0;
             return f_pzero_j2<LL>(hs, q);
         }
-        const double F = f_series<LL, MIXED>(hs, cb, hf, hw, cbf, q, qinv, h, invh, sx, cx, Fa);
+        const double F =
+            f_series<LL, MIXED>(hs, cb, hf, hw, cbf, q, qinv, h, invh, sx, cx, Fa);
         if (q > 0.0) {
             Fa = Fa / (q * q);
             return F / (q * q);
@@ -613,6 +615,11 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
             phs[2 * lane + 1] = cd;
         }
         __syncthreads();
+        // plain sums over the batch's records, added to S with compensation
+        // once per batch
+        double Tb[SQ];
+#pragma unroll
+        for (int s = 0; s < SQ; ++s) Tb[s] = 0.0;
         for (int j = 0; j < nb; ++j) {
             // the record is wave-uniform: kept in scalar registers
             const double* rc = recs + j * R;
@@ -625,7 +632,8 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
             const double mult = (double)((meta >> 40) & 15u);
             const int br = (int)((meta >> 44) & 1u);
             // both pair factors of the lane's q points, issued before the
-            // series so that their latency hides behind it
+            // series so that their latency hides behind it (loading them one
+            // record ahead measured slower: more registers live)
             const double* eb = A.e12 + (int64_t)pb * A.nq;
             const double* ek = A.e12 + (int64_t)pk * A.nq;
             double ebv[SQ], ekv[SQ];
@@ -682,7 +690,7 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
                 double v = mult * F;
                 v = v * ebv[s];
                 v = v * ekv[s];
-                kahan(S[s], Sc[s], v);
+                Tb[s] += v;
                 // error scale of the FP32 terms: |weight| sum_b |h_b u_b|
                 if (MIXED) {
                     const double e = fabs((mult * ebv[s]) * ekv[s]) * Fa;
@@ -690,6 +698,8 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
                 }
             }
         }
+#pragma unroll
+        for (int s = 0; s < SQ; ++s) kahan(S[s], Sc[s], Tb[s]);
     }
 #pragma unroll
     for (int s = 0; s < SQ; ++s) {
