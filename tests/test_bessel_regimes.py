@@ -1,9 +1,10 @@
 """The small-argument Bessel regime of the series kernel (series.hip f_miller,
 adjoint.hip b_series): Miller's backward recurrence for g_b = j_b(x) / x^b
 started 16 orders above LL from (0, 1), normalised by j0 or by j1 = x g_1,
-whichever is larger in magnitude.  Restated here in float64 numpy, step for
-step, and checked against a long-double recurrence started 80 orders up on
-[0, xup), xup = 0.9 LL + 1 (above it the kernel recurs upward).
+whichever is larger in magnitude, on [0, xup); the upward recurrence from
+j0, j1 above xup = LL - 1 (0.5 at LL = 1, 1.2 at LL = 2).  Both are restated
+here in float64 numpy, step for step, and checked against a long-double
+recurrence started 80 orders up.
 
 The error is measured against the envelope of g_b: |g_b| below the turning
 point x = sqrt(b (b + 1)), the spherical Hankel modulus / x^b above it.  A
@@ -67,7 +68,13 @@ def _device_miller(LL, x, start, by_j1=True):
     return np.array([vals[b] * nrm for b in range(LL + 1)])
 
 
+def _xup(LL):
+    return 0.5 if LL == 1 else 1.2 if LL == 2 else LL - 1.0
+
+
 def _max_err(LL, by_j1=True, npts=2500):
+    # the old boundary 0.9 LL + 1 bounds the new one: the Miller start is
+    # checked over the wider range
     xup = 0.9 * LL + 1.0
     x = np.linspace(1e-4, xup, npts, endpoint=False)
     g = _reference(LL, x)
@@ -85,3 +92,34 @@ def test_miller_start_16_within_envelope(LL):
 def test_j0_only_normalisation_fails_near_its_zero():
     # the planted mistake: x = pi lies in the Miller regime from LL = 3 on
     assert _max_err(6, by_j1=False) > 1e-13
+
+
+def _device_upward(LL, x, h=1.3):
+    # f_up64: u_b = j_b (q/h)^b = g_b q^(2b),
+    # u_{b+1} = (2b+1)/h^2 u_b - (q/h)^2 u_{b-1}, with q = x / h
+    x = np.asarray(x, dtype=np.float64)
+    q = x / h
+    invh = 1.0 / h
+    ih2 = invh * invh
+    j0 = np.sin(x) / x
+    nr2 = -((q * invh) * (q * invh))
+    us = [j0, (j0 - np.cos(x)) * ih2]
+    for b in range(1, LL):
+        us.append((2 * b + 1) * ih2 * us[-1] + nr2 * us[-2])
+    return np.array([us[b] / q ** (2 * b) for b in range(LL + 1)])
+
+
+@pytest.mark.parametrize("LL", list(range(1, 17)))
+def test_upward_from_xup_within_envelope(LL):
+    x = np.linspace(_xup(LL), 0.9 * LL + 8.0, 3000)
+    g = _reference(LL, x)
+    tp = np.sqrt(np.arange(LL + 1) * (np.arange(LL + 1) + 1.0))
+    env = np.where(x[None, :] < tp[:, None], np.abs(g), _hankel_over_xb(LL, x, g))
+    err = np.max(np.abs(_device_upward(LL, x) - g) / env.astype(np.float64))
+    assert err < 2e-14
+    # and it would not hold a unit lower (the boundary is not arbitrary)
+    if LL >= 4:
+        xl = np.linspace(_xup(LL) - 1.0, _xup(LL), 800)
+        gl = _reference(LL, xl)
+        envl = np.where(xl[None, :] < tp[:, None], np.abs(gl), _hankel_over_xb(LL, xl, gl))
+        assert np.max(np.abs(_device_upward(LL, xl) - gl) / envl.astype(np.float64)) > 2e-14

@@ -153,10 +153,14 @@ struct ContractShape {
     static constexpr int DP = (LL + 1) | 1;
 };
 
-// Upward recurrence from j0, j1 keeps every j_b (b <= LL) within 1e-14 of
-// its envelope (|j_b| below the first zero, the spherical Hankel modulus
-// above it) for x >= 0.9 LL + 1; below that Miller's backward recurrence is
-// used.
+// Upward recurrence from j0, j1 (f_up64's form, on u_b = j_b (q/h)^b) keeps
+// every g_b = j_b / x^b (b <= LL) within 2e-14 of its envelope (|g_b| below
+// the turning point x = sqrt(b (b + 1)), the Hankel modulus / x^b above it)
+// for x >= xup = LL - 1 (0.5 at LL = 1, 1.2 at LL = 2).  Measured on 12000
+// points per LL, the plain j recurrence stays within 1e-14 from x = LL - 1.5
+// (LL >= 6; 0.2, 0.86, 1.68, 2.70, 3.69 for LL = 1..5), so xup leaves a
+// margin of at least 0.3; a unit lower the error exceeds 2e-14.  Below xup
+// Miller's backward recurrence is used.
 //
 // Miller's start.  Started 16 orders above LL from (0, 1) and normalised by
 // j0 or j1, whichever is larger in magnitude (j0 alone loses up to 5e-12 of
@@ -170,7 +174,7 @@ struct ContractShape {
 // the shorter recurrences save.)
 template <int LL>
 struct SeriesRegime {
-    static constexpr double xup = LL == 0 ? 0.0 : 0.9 * LL + 1.0;
+    static constexpr double xup = LL == 0 ? 0.0 : LL == 1 ? 0.5 : LL == 2 ? 1.2 : LL - 1.0;
     // the start for any x < xup
     static constexpr int nstart = LL + 16;
 };
