@@ -428,7 +428,8 @@ __global__ void k_gs(const double* __restrict__ V, int norb,
 }
 
 // T(P, x) = C(p,a)C(s,b) + C(p,b)C(s,a) for pair-space element x = (a,b);
-// column-major with leading dimension npmo.
+// column-major with leading dimension npmo; 0 for the padding elements
+// (a = b = -1) at the end of the pair space.
 __global__ void k_tmat(const double* __restrict__ mmod, int nmo,
                        const int2* __restrict__ mopairs, int npmo,
                        const int32_t* __restrict__ xa,
@@ -441,6 +442,10 @@ __global__ void k_tmat(const double* __restrict__ mmod, int nmo,
     int64_t x = idx / npmo;
     int2 ps = mopairs[P];
     int a = xa[x], b = xb[x];
+    if (a < 0) {
+        T[idx] = 0.0;
+        return;
+    }
     double ca_p = mmod[ps.x + (int64_t)nmo * a], cb_s = mmod[ps.y + (int64_t)nmo * b];
     double cb_p = mmod[ps.x + (int64_t)nmo * b], ca_s = mmod[ps.y + (int64_t)nmo * a];
     T[idx] = ca_p * cb_s + cb_p * ca_s;
