@@ -416,7 +416,9 @@ def main():
         "host_boundary": {"ms": hb_ms, "value": units / (hb_ms / 1000.0),
                           "note": "one thistle_total_from_twordm call with host buffers: "
                                   "plan creation (allocation, host-to-device copies), "
-                                  "execution, result copy back"},
+                                  "execution, result copy back; best of 2 calls (the "
+                                  "library reuses the device blocks a released plan "
+                                  "leaves, so a repeated call allocates nothing)"},
         "roofline": roof,
     }
     if err is not None:
