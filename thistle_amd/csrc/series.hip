@@ -81,6 +81,19 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
         const double rx = PK.Px - PB.Px, ry = PK.Py - PB.Py, rz = PK.Pz - PB.Pz;
         const double hx = br ? 0.0 : rx, hy = br ? 0.0 : ry, hz = br ? 0.0 : rz;
 
+        // The quadruple loop is software-pipelined: the lane's first table
+        // entry is requested here, before the table build, and its Z before
+        // the build's closing barrier; each step of the loop then requests
+        // the next step's Z and the entry after that before it contracts its
+        // own quadruple, so that a quartet waits for one chain of dependent
+        // loads (entry, then Z) once, not once per step (XY6 step 8.15 ->
+        // 7.57 ms; two quadruples per step: 7.83)
+        const uint32_t nq4 = valid ? (uint32_t)(Gi.count * Gj.count * Gk.count * Gr.count) : 0u;
+        const int S = A.nsig;
+        const uint64_t* qt =
+            A.quadtab + A.comboff[((Gi.pad * S + Gj.pad) * S + Gk.pad) * S + Gr.pad];
+        uint64_t wc = lg < nq4 ? qt[lg] : 0ull;
+
         __syncthreads();  // the previous quartets are done with the tables
         // px(m) = sum_T c(T) a(T, 2m - T) per axis and exponent tuple, with
         // a(l,p) = kAcoef(l,p) h^p
@@ -124,19 +137,17 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
                 out[m] = s;
             }
         }
+        const ZView ZV = zview(A, pb, pk, PB.off, PK.off, Gi, Gj, Gk, Gr);
+        double zc = lg < nq4 ? zget(A, ZV, wc) : 0.0;
+        uint64_t wn = lg + G < nq4 ? qt[lg + G] : 0ull;
         __syncthreads();
 
-        const uint32_t nq4 = valid ? (uint32_t)(Gi.count * Gj.count * Gk.count * Gr.count) : 0u;
-        const int S = A.nsig;
-        const uint64_t* qt =
-            A.quadtab + A.comboff[((Gi.pad * S + Gj.pad) * S + Gk.pad) * S + Gr.pad];
         double h[NA][D];
 #pragma unroll
         for (int k = 0; k < NA; ++k)
 #pragma unroll
             for (int b = 0; b < D; ++b) h[k][b] = 0.0;
         int any = 0;
-        const ZView ZV = zview(A, pb, pk, PB.off, PK.off, Gi, Gj, Gk, Gr);
         // one GTO quadruple: h(beta) += z * (px * py * pz)(beta)
         auto quad = [&](uint64_t w, double z) {
             const int tx = (int)((w >> 16) & 1023u), ty = (int)((w >> 26) & 1023u),
@@ -184,28 +195,23 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
                 }
             }
         };
-        // ZB quadruples per lane per step: their Z gathers are issued together
-        // (the lane's quadruples are still accumulated in increasing u)
-        constexpr int ZB = 2;
-        for (uint32_t u0 = lg; u0 < nq4; u0 += ZB * G) {
-            uint64_t wv[ZB];
-            double zv[ZB];
-#pragma unroll
-            for (int k = 0; k < ZB; ++k) {
-                const uint32_t u = u0 + k * G;
-                wv[k] = u < nq4 ? qt[u] : 0ull;
-            }
-#pragma unroll
-            for (int k = 0; k < ZB; ++k) {
-                const uint32_t u = u0 + k * G;
-                zv[k] = u < nq4 ? zget(A, ZV, wv[k]) : 0.0;
-            }
-#pragma unroll
-            for (int k = 0; k < ZB; ++k) {
-                if (u0 + k * G >= nq4 || fabs(zv[k]) < A.cutoffz) continue;
+        // the lane's quadruples u = lg, lg + G, ... in increasing u
+        for (uint32_t u = lg; u < nq4; u += G) {
+            // every load issued so far has arrived (the step's entry and Z):
+            // said explicitly, so that the loads issued below are not waited
+            // for before the contraction (without it the compiler's wait
+            // before the first use of zc also covered them: 8.02 against
+            // 7.57 ms).  vmcnt(0); expcnt and lgkmcnt at their maxima.
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+            const double zn = u + G < nq4 ? zget(A, ZV, wn) : 0.0;
+            const uint64_t w2 = u + 2 * G < nq4 ? qt[u + 2 * G] : 0ull;
+            if (!(fabs(zc) < A.cutoffz)) {
                 any = 1;
-                quad(wv[k], zv[k]);
+                quad(wc, zc);
             }
+            wc = wn;
+            zc = zn;
+            wn = w2;
         }
         // the values of h(beta) (j0) or of the j2 series / h = 0 polynomial
         // coefficients (DO of them), given the reduced sums in hr / gr
