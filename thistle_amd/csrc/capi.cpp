@@ -212,7 +212,7 @@ struct ThistlePlan {
     rocblas_handle blas = nullptr;
     std::vector<double> h_q;
     // device buffers
-    DevBuf G, P, E, e12, q, lxyz, rows, qlist, flags, scan, qcomp, count;
+    DevBuf G, P, E, e12, q, lxyz, rows, qlist, flags, scan, qcomp, count, qhdr;
     DevBuf mmod, mat, total, rmat, rtot, hkeys, hmin, hcnt, hslot, uscan;
     DevBuf V, Gs, T, M, Z, Cp, W3, jtab, aend, mopairs, xa, xb, onerdm;
     DevBuf part, partc, S, cls;
@@ -599,6 +599,9 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     std::vector<int64_t> comboff;
     std::vector<uint64_t> qtab;
     build_quad_tables(sigs, used, comboff, qtab);
+    // quartet headers hold 32-bit table starts (a larger table, 32 GB, would
+    // not be held anyway)
+    if (qtab.size() > 0xffffffffull) return THISTLE_ENOMEM;
     pl->nsig = S;
     int L1 = maxl + 1, T1 = 2 * maxl + 1;
     if ((rc = dalloc(pl->G, sizeof(GroupInfo) * ng))) return rc;
@@ -609,6 +612,7 @@ int setup_common(ThistlePlan* pl, const ThistleBasis* b, const ThistleOpts* o) {
     if ((rc = dalloc(pl->lxyz, sizeof(char4) * b->ngto))) return rc;
     if ((rc = dalloc(pl->qlist, sizeof(uint64_t) * Q))) return rc;
     if ((rc = dalloc(pl->flags, Q))) return rc;
+    if ((rc = dalloc(pl->qhdr, sizeof(QHdr) * Q))) return rc;
     int64_t nb = (Q + 2047) / 2048 + 1;
     if ((rc = dalloc(pl->scan, sizeof(int64_t) * (Q + nb + 1)))) return rc;
     if ((rc = dalloc(pl->qcomp, sizeof(uint64_t) * Q))) return rc;
@@ -690,6 +694,9 @@ int run_tables_and_enum(ThistlePlan* pl, bool timing) {
     if (timing) HCHECK(hipEventRecord(pl->ev[1], pl->stream));
     HCHECK(launch_enumerate(pl->rows.as<EnumRow>(), pl->nrows, pl->P.as<PairInfo>(), pl->ng,
                             pl->cutoffcentre, pl->qlist.as<uint64_t>(), pl->stream));
+    HCHECK(launch_qhdr(pl->cls.as<int32_t>(), pl->cls_off[17], pl->qlist.as<uint64_t>(),
+                       pl->G.as<GroupInfo>(), pl->comboff.as<int64_t>(), pl->nsig,
+                       pl->qhdr.as<QHdr>(), pl->stream));
     if (timing) HCHECK(hipEventRecord(pl->ev[2], pl->stream));
     return THISTLE_OK;
 }
@@ -806,7 +813,7 @@ int run_series(ThistlePlan* pl, bool timing) {
         // beyond): XY6's quartet stage 8.0 ms against 8.15 with 16384
         int blocks = (int)std::min<int64_t>(1 << 20, n);
         a.hs = pl->hs.as<double>() + pl->cls_hoff[LL];
-        HCHECK(launch_contract(a, LL, pl->cls_tmax[LL], pl->cls.as<int32_t>() + pl->cls_off[LL],
+        HCHECK(launch_contract(a, LL, pl->cls_tmax[LL], pl->qhdr.as<QHdr>() + pl->cls_off[LL],
                                n, blocks, pl->stream));
         if (ov) HCHECK(hipEventRecord(pl->cev[LL], pl->stream));
     }
@@ -1377,7 +1384,7 @@ int group_tables(ThistleBatch* b, ThistleBatch::Group* g) {
             ThistlePlan* p = b->plans[f].get();
             const int64_t n = p->cls_off[LL + 1] - p->cls_off[LL];
             if (n <= 0) continue;
-            cw.push_back({blocks, n, p->cls.as<int32_t>() + p->cls_off[LL],
+            cw.push_back({blocks, n, p->qhdr.as<QHdr>() + p->cls_off[LL],
                           p->hs.as<double>() + p->cls_hoff[LL], f});
             blocks += (n + ngq - 1) / ngq;
             tmax = std::max(tmax, p->cls_tmax[LL]);

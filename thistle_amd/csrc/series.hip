@@ -44,11 +44,11 @@
 //           factorised like the j0 case: the rrdj2b weight 2 ceil((l+p)/2)
 //           is 2 m per axis, and -2 on z)
 //   h = 0:  c0(b) = sqrt(5/pi)/2 (b h(b) - 3 g(b)),  b <= LL  (P0_j2, :3034-3170)
-// Quartet groups g0, g0 + gstep, ... (NG quartets each) of a class list idx
-// (nidx entries) whose records go to hsb.
+// Quartet groups g0, g0 + gstep, ... (NG quartets each) of a class list
+// (nidx headers) whose records go to hsb.
 template <int LL, bool J2>
 __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __restrict__ hsb,
-                                                const int32_t* __restrict__ idx, int64_t nidx,
+                                                const QHdr* __restrict__ hdr, int64_t nidx,
                                                 int tmax, int64_t g0, int64_t gstep) {
     constexpr int D = LL + 1;
     constexpr int NA = J2 ? 2 : 1;  // accumulated vectors: h, and g for j2
@@ -69,8 +69,9 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
     for (int64_t base = g0 * NG; base < nidx; base += gstep * NG) {
         const int64_t kq = base + grp;
         const bool valid = kq < nidx;
-        const int64_t e = idx[valid ? kq : base];
-        const uint64_t ent = A.qlist[e];
+        const QHdr H = hdr[valid ? kq : base];
+        const int64_t e = H.e;
+        const uint64_t ent = QH_ENT(H.w);
         const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
         const int br = QE_BR(ent);
         const int pb = pair_index(gi, gj, A.ng), pk = pair_index(gk, gr, A.ng);
@@ -88,24 +89,35 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
         // own quadruple, so that a quartet waits for one chain of dependent
         // loads (entry, then Z) once, not once per step (XY6 step 8.15 ->
         // 7.57 ms; two quadruples per step: 7.83)
-        const uint32_t nq4 = valid ? (uint32_t)(Gi.count * Gj.count * Gk.count * Gr.count) : 0u;
-        const int S = A.nsig;
-        const uint64_t* qt =
-            A.quadtab + A.comboff[((Gi.pad * S + Gj.pad) * S + Gk.pad) * S + Gr.pad];
+        const uint32_t nq4 = valid ? QH_NQ4(H.w) : 0u;
+        const uint64_t* qt = A.quadtab + H.qoff;
         uint64_t wc = lg < nq4 ? qt[lg] : 0ull;
 
         __syncthreads();  // the previous quartets are done with the tables
         // px(m) = sum_T c(T) a(T, 2m - T) per axis and exponent tuple, with
         // a(l,p) = kAcoef(l,p) h^p
-        const double* E0 = A.E;
+        // w = ax T + t, t = ((la (Lj+1) + lb) (Lk+1) + lc) (Lr+1) + ld, decoded
+        // with 16-bit multiply-shift quotients: floor(t m / 2^16), m =
+        // ceil(2^16 / d), is t / d exactly for t < 2^16 / d (every shell has
+        // L <= 4: t < 625, d <= 5); offsets in 24-bit products (XY6 step 7.51 -> 7.38 ms
+        // against the compiler's division sequences and 64-bit offsets)
+        const int rowax = L1 * L1 * T1;
+        const double* Ebb = A.E + (int64_t)pb * (3 * rowax);
+        const double* Ekb = A.E + (int64_t)pk * (3 * rowax);
+        const unsigned mr = (65536u + Lr) / (unsigned)(Lr + 1);
+        const unsigned mk = (65536u + Lk) / (unsigned)(Lk + 1);
+        const unsigned mj = (65536u + Lj) / (unsigned)(Lj + 1);
         for (int w = lg; w < 3 * T; w += G) {
-            int ax = w / T, t = w % T;
-            int ld = t % (Lr + 1), t2 = t / (Lr + 1);
-            int lc = t2 % (Lk + 1);
-            t2 /= (Lk + 1);
-            int lb = t2 % (Lj + 1), la = t2 / (Lj + 1);
-            const double* Eb = E0 + (((int64_t)pb * 3 + ax) * L1 * L1 + la * L1 + lb) * T1;
-            const double* Ek = E0 + (((int64_t)pk * 3 + ax) * L1 * L1 + lc * L1 + ld) * T1;
+            const int ax = (w >= T) + (w >= 2 * T);
+            const int t = w - ax * T;
+            const int t1 = (int)(__umul24((unsigned)t, mr) >> 16);
+            const int ld = t - t1 * (Lr + 1);
+            const int t2 = (int)(__umul24((unsigned)t1, mk) >> 16);
+            const int lc = t1 - t2 * (Lk + 1);
+            const int la = (int)(__umul24((unsigned)t2, mj) >> 16);
+            const int lb = t2 - la * (Lj + 1);
+            const double* Eb = Ebb + (__mul24(ax, rowax) + __mul24(__mul24(la, L1) + lb, T1));
+            const double* Ek = Ekb + (__mul24(ax, rowax) + __mul24(__mul24(lc, L1) + ld, T1));
             const int lab = la + lb, lcd = lc + ld;
             double eb[D], ek[D];
 #pragma unroll
@@ -303,10 +315,38 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
     }
 }
 
+// The header of each quartet of a class list: its entry, GTO-quadruple
+// count (at most 15^4 < 2^16: a group holds at most 15 components) and
+// quadruple-table start.
+__global__ void __launch_bounds__(256)
+    k_qhdr(const int32_t* __restrict__ cls, int64_t n, const uint64_t* __restrict__ qlist,
+           const GroupInfo* __restrict__ G, const int64_t* __restrict__ comboff, int S,
+           QHdr* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int32_t e = cls[k];
+    const uint64_t ent = qlist[e];
+    const GroupInfo Gi = G[QE_GI(ent)], Gj = G[QE_GJ(ent)], Gk = G[QE_GK(ent)], Gr = G[QE_GR(ent)];
+    const uint64_t nq4 = (uint64_t)(Gi.count * Gj.count * Gk.count * Gr.count);
+    QHdr h;
+    h.w = ent | (nq4 << 45);
+    h.e = e;
+    h.qoff = (uint32_t)comboff[((Gi.pad * S + Gj.pad) * S + Gk.pad) * S + Gr.pad];
+    out[k] = h;
+}
+
+hipError_t launch_qhdr(const int32_t* cls, int64_t n, const uint64_t* qlist, const GroupInfo* G,
+                       const int64_t* comboff, int nsig, QHdr* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_qhdr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cls, n, qlist,
+                       G, comboff, nsig, out);
+    return hipGetLastError();
+}
+
 template <int LL, bool J2>
 __global__ void __launch_bounds__(ContractShape<LL>::CT)
-    k_contract(SeriesArgs A, const int32_t* __restrict__ idx, int64_t nidx, int tmax) {
-    contract_groups<LL, J2>(A, A.hs, idx, nidx, tmax, blockIdx.x, gridDim.x);
+    k_contract(SeriesArgs A, const QHdr* __restrict__ hdr, int64_t nidx, int tmax) {
+    contract_groups<LL, J2>(A, A.hs, hdr, nidx, tmax, blockIdx.x, gridDim.x);
 }
 
 // The entry of a cross-frame work table that holds block b: the last one
@@ -329,7 +369,7 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
     k_contract_multi(const SeriesArgs* __restrict__ args, const ClassWork* __restrict__ work,
                      int nwork, int tmax) {
     const ClassWork& W = find_work(work, nwork, blockIdx.x);
-    contract_groups<LL, J2>(args[W.frame], W.hs, W.idx, W.n, tmax,
+    contract_groups<LL, J2>(args[W.frame], W.hs, W.hdr, W.n, tmax,
                             (int64_t)blockIdx.x - W.blk0, (int64_t)1 << 40);
 }
 
@@ -1099,7 +1139,7 @@ size_t contract_lds_bytes(int LL, int Tmax) {
     }
 }
 
-hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t* idx,
+hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const QHdr* hdr,
                            int64_t nidx, int max_blocks, hipStream_t s) {
     if (nidx <= 0) return hipSuccess;
     size_t lds = contract_lds_bytes(LL, Tmax);
@@ -1110,10 +1150,10 @@ hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t*
         int blocks = (int)std::min<int64_t>(max_blocks, (nidx + NG - 1) / NG);                 \
         if (a.j2)                                                                              \
             hipLaunchKernelGGL((k_contract<k, true>), dim3(blocks), dim3(ContractShape<k>::CT),  \
-                               lds, s, a, idx, nidx, Tmax);                                    \
+                               lds, s, a, hdr, nidx, Tmax);                                    \
         else                                                                                   \
             hipLaunchKernelGGL((k_contract<k, false>), dim3(blocks), dim3(ContractShape<k>::CT), \
-                               lds, s, a, idx, nidx, Tmax);                                    \
+                               lds, s, a, hdr, nidx, Tmax);                                    \
         break;                                                                                 \
     }
         ALL_LL(C_)

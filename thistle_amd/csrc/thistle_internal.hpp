@@ -209,10 +209,20 @@ struct SeriesArgs {
 
 // Cross-frame work tables (batched frames): one entry per frame that has
 // quartets in the class, in increasing first block blk0.
+// A quartet of a class list, as the contraction reads it: one 16-byte load
+// instead of the chain class list -> quartet entry -> groups -> table start.
+struct alignas(16) QHdr {
+    uint64_t w;     // quartet entry (bits 0-44) | GTO quadruples (bits 45-60)
+    int32_t e;      // position in the quartet list
+    uint32_t qoff;  // start of its GTO-quadruple table in quadtab
+};
+#define QH_NQ4(w) ((uint32_t)((w) >> 45))
+#define QH_ENT(w) ((w) & ((1ull << 45) - 1ull))
+
 struct ClassWork {           // contraction: one block per quartet group
     int64_t blk0;
     int64_t n;               // quartets of the class in this frame
-    const int32_t* idx;      // the frame's class list
+    const QHdr* hdr;         // the frame's class list, as headers
     double* hs;              // the frame's class records
     int64_t frame;           // index into the SeriesArgs array
 };
@@ -299,8 +309,11 @@ hipError_t class_sort(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* 
 // k_contract<LL>: h(beta) of one LL class (series.hip)
 size_t contract_lds_bytes(int LL, int Tmax);
 hipError_t set_contract_lds_limit(size_t bytes);
-hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const int32_t* idx,
+hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const QHdr* hdr,
                            int64_t nidx, int blocks, hipStream_t s);
+// the headers of a class list cls (n quartets, every class in turn)
+hipError_t launch_qhdr(const int32_t* cls, int64_t n, const uint64_t* qlist, const GroupInfo* G,
+                       const int64_t* comboff, int nsig, QHdr* out, hipStream_t s);
 // k_series_far<LL>: the q h >= XDPH points of the class (same launch shape
 // and wave partition as launch_series, added to its partial rows)
 hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int waves,
