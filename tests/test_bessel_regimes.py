@@ -1,10 +1,10 @@
 """The small-argument Bessel regime of the series kernel (series.hip f_miller,
 adjoint.hip b_series): Miller's backward recurrence for g_b = j_b(x) / x^b
-started 16 orders above LL from (0, 1), normalised by j0 or by j1 = x g_1,
-whichever is larger in magnitude, on [0, xup); the upward recurrence from
-j0, j1 above xup = LL - 1 (0.5 at LL = 1, 1.2 at LL = 2).  Both are restated
-here in float64 numpy, step for step, and checked against a long-double
-recurrence started 80 orders up.
+started M(LL) orders above LL from (0, 1) (series_math.hpp miller_orders),
+normalised by j0 or by j1 = x g_1, whichever is larger in magnitude, on
+[0, xup); the upward recurrence from j0, j1 above xup = LL - 1 (0.5 at
+LL = 1, 1.2 at LL = 2).  Both are restated here in float64 numpy, step for
+step, and checked against a long-double recurrence started 80 orders up.
 
 The error is measured against the envelope of g_b: |g_b| below the turning
 point x = sqrt(b (b + 1)), the spherical Hankel modulus / x^b above it.  A
@@ -13,7 +13,10 @@ planted mistake (normalising by j0 alone) fails the bound near x = pi.
 import numpy as np
 import pytest
 
-START = 16  # orders above LL (SeriesRegime::nstart - LL)
+# orders above LL at which Miller's recurrence starts (SeriesRegime::nstart -
+# LL): the least start meeting 4e-15 on [0, xup) plus two, at most 16
+MILLER = {1: 7, 2: 9, 3: 10, 4: 11, 5: 12, 6: 13, 7: 13, 8: 14, 9: 15, 10: 15,
+          11: 16, 12: 16, 13: 16, 14: 16, 15: 16, 16: 16}
 
 
 def _reference(LL, x):
@@ -72,26 +75,27 @@ def _xup(LL):
     return 0.5 if LL == 1 else 1.2 if LL == 2 else LL - 1.0
 
 
-def _max_err(LL, by_j1=True, npts=2500):
-    # the old boundary 0.9 LL + 1 bounds the new one: the Miller start is
-    # checked over the wider range
-    xup = 0.9 * LL + 1.0
+def _max_err(LL, start, by_j1=True, xup=None, npts=2500):
+    xup = _xup(LL) if xup is None else xup
     x = np.linspace(1e-4, xup, npts, endpoint=False)
     g = _reference(LL, x)
     tp = np.sqrt(np.arange(LL + 1) * (np.arange(LL + 1) + 1.0))
     env = np.where(x[None, :] < tp[:, None], np.abs(g), _hankel_over_xb(LL, x, g))
-    got = _device_miller(LL, x, START, by_j1)
+    got = _device_miller(LL, x, start, by_j1)
     return float(np.max(np.abs(got - g) / env.astype(np.float64)))
 
 
 @pytest.mark.parametrize("LL", list(range(1, 17)))
-def test_miller_start_16_within_envelope(LL):
-    assert _max_err(LL) < 4e-15
+def test_miller_start_within_envelope(LL):
+    assert _max_err(LL, MILLER[LL]) < 4e-15
+    # and three orders fewer would not hold (the start is not arbitrary)
+    if LL >= 2:
+        assert _max_err(LL, MILLER[LL] - 3) > 4e-15
 
 
 def test_j0_only_normalisation_fails_near_its_zero():
-    # the planted mistake: x = pi lies in the Miller regime from LL = 3 on
-    assert _max_err(6, by_j1=False) > 1e-13
+    # the planted mistake: x = pi lies in the Miller regime from LL = 5 on
+    assert _max_err(6, MILLER[6], by_j1=False) > 1e-13
 
 
 def _device_upward(LL, x, h=1.3):

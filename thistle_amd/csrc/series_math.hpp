@@ -162,21 +162,26 @@ struct ContractShape {
 // margin of at least 0.3; a unit lower the error exceeds 2e-14.  Below xup
 // Miller's backward recurrence is used.
 //
-// Miller's start.  Started 16 orders above LL from (0, 1) and normalised by
-// j0 or j1, whichever is larger in magnitude (j0 alone loses up to 5e-12 of
-// the envelope next to its zeros, x = pi, 2 pi, ...), the backward
-// recurrence gives every g_b = j_b / x^b, b <= LL, within 2e-15 of its
-// envelope (|g_b| below the turning point x = sqrt(b (b + 1)), the Hankel
-// modulus above it) for x < xup, LL = 1..16, checked against a long-double
-// recurrence started 80 orders up on 8000 points of [0, xup].  (8 orders
-// suffice below x = 0.28 LL + 1.2 and 12 below 0.5 LL + 2.8, but choosing
-// the start per wave measured slower: the extra code copies cost more than
+// Miller's start.  Started M(LL) orders above LL from (0, 1) and
+// normalised by j0 or j1, whichever is larger in magnitude (j0 alone loses
+// up to 5e-12 of the envelope next to its zeros, x = pi, 2 pi, ...), the
+// backward recurrence gives every g_b = j_b / x^b, b <= LL, within 4e-15 of
+// its envelope (|g_b| below the turning point x = sqrt(b (b + 1)), the
+// Hankel modulus above it) for x < xup, checked against a long-double
+// recurrence started 80 orders up on 4000 points of [0, xup).  The least M
+// that does so is 5, 7, 8, 9, 10, 11, 11, 12, 13, 13, 14, 14, 14, 15, 15, 16
+// for LL = 1..16; M(LL) adds two orders to it, at most 16.  (Chosen per wave
+// from the slice's largest x instead, the extra code copies cost more than
 // the shorter recurrences save.)
+constexpr int miller_orders(int LL) {
+    return LL <= 1 ? 7 : LL == 2 ? 9 : LL == 3 ? 10 : LL == 4 ? 11 : LL == 5 ? 12
+         : LL <= 7 ? 13 : LL == 8 ? 14 : LL <= 10 ? 15 : 16;
+}
 template <int LL>
 struct SeriesRegime {
     static constexpr double xup = LL == 0 ? 0.0 : LL == 1 ? 0.5 : LL == 2 ? 1.2 : LL - 1.0;
     // the start for any x < xup
-    static constexpr int nstart = LL + 16;
+    static constexpr int nstart = LL + miller_orders(LL);
 };
 
 // Z of one quartet.  Total: the lower-triangle elements (max(xb,xk),
