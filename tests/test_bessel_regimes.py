@@ -127,3 +127,78 @@ def test_upward_from_xup_within_envelope(LL):
         gl = _reference(LL, xl)
         envl = np.where(xl[None, :] < tp[:, None], np.abs(gl), _hankel_over_xb(LL, xl, gl))
         assert np.max(np.abs(_device_upward(LL, xl) - gl) / envl.astype(np.float64)) > 2e-14
+
+
+# ---- the upward regime's sum: Clenshaw's recurrence (series.hip f_up64)
+
+def _long_g(LL, x):
+    # g_b = j_b / x^b in long double, Miller started far above max(x)
+    x = np.asarray(x, dtype=np.longdouble)
+    x2 = x * x
+    vp1 = np.zeros_like(x)
+    v = np.ones_like(x)
+    out = np.zeros((LL + 1,) + x.shape, dtype=np.longdouble)
+    for b in range(LL + int(x.max()) + 90, 0, -1):
+        vm = (2 * b + 1) * v - x2 * vp1
+        vp1, v = v, vm
+        if b - 1 <= LL:
+            out[b - 1] = vm
+        s = np.abs(v) + np.abs(vp1)
+        big = s > 1e300
+        if big.any():
+            v[big] /= s[big]
+            vp1[big] /= s[big]
+            out[:, big] /= s[big]
+    j0 = np.sin(x) / x
+    j1 = (j0 - np.cos(x)) / x
+    pick = np.abs(j0) >= np.abs(j1)
+    return out * np.where(pick, j0 / out[0], j1 / (x * out[1]))
+
+
+def _device_clenshaw(LL, x, h, hs, flip=1.0):
+    # f_up64 step for step; flip = -1 plants a sign error in (q/h)^2
+    q = x / h
+    invh = 1.0 / h
+    ih2 = invh * invh
+    j0 = np.sin(x) / x
+    cx = np.cos(x)
+    cb = [(2 * b + 1) * ih2 for b in range(LL + 1)]
+    r = q * invh
+    nr2 = -flip * (r * r)
+    y1, y2 = np.full_like(x, hs[LL]), np.zeros_like(x)
+    if LL >= 2:
+        y2 = y1
+        y1 = cb[LL - 1] * y2 + hs[LL - 1]
+        for b in range(LL - 2, 0, -1):
+            y0 = cb[b] * y1 + (nr2 * y2 + hs[b])
+            y2, y1 = y1, y0
+    u1 = (j0 - cx) * cb[0]
+    return y1 * u1 + j0 * (nr2 * y2 + hs[0])
+
+
+def _clenshaw_err(LL, flip=1.0):
+    rng = np.random.default_rng(100 + LL)
+    worst = 0.0
+    for h in (0.3, 5.0):
+        x = np.linspace(_xup(LL), 74.0, 1500)
+        g = _long_g(LL, x)
+        env_g = _hankel_over_xb(LL, x, g)
+        q = x.astype(np.longdouble) / np.longdouble(h)
+        u = np.array([g[b] * q ** (2 * b) for b in range(LL + 1)])
+        eu = np.array([env_g[b] * q ** (2 * b) for b in range(LL + 1)])
+        hs = rng.standard_normal(LL + 1) * np.exp(rng.uniform(-3.0, 3.0, LL + 1))
+        ref = np.sum(hs[:, None].astype(np.longdouble) * u, axis=0)
+        env = np.sum(np.abs(hs[:, None]).astype(np.longdouble) * eu, axis=0)
+        got = _device_clenshaw(LL, x, h, hs, flip)
+        worst = max(worst, float(np.max(np.abs(got - ref) / env)))
+    return worst
+
+
+@pytest.mark.parametrize("LL", list(range(1, 17)))
+def test_clenshaw_sum_within_envelope(LL):
+    # sum_b h(b) u_b within 4e-15 of sum_b |h(b)| times u_b's envelope
+    assert _clenshaw_err(LL) < 4e-15
+
+
+def test_clenshaw_sign_error_fails():
+    assert _clenshaw_err(6, flip=-1.0) > 1e-3
