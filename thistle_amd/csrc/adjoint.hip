@@ -432,7 +432,8 @@ __global__ void __launch_bounds__(ContractShape<LL>::CT)
                 double Cb = 0.0, Ch = 0.0;
 #pragma unroll
                 for (int b = 0; b < D; ++b) {
-                    const double hb = A.hs[kq * HS_ROW(D) + b];
+                    // the record holds mult h(beta); 1 / mult (1, 2, 4 or 8) is exact
+                    const double hb = A.hs[kq * HS_ROW(D) + b] * (1.0 / (double)QE_MULT(ent));
                     Cb = fma(hb, Gb[b], Cb);
                     Ch = fma(hb, Gb[b + 1], Ch);
                 }

@@ -12,8 +12,9 @@
 //   k_series<LL,M>  one wave per (quartet slice, 64-point q slice).  Evaluates
 //                   F(q) = sum_beta h(beta) (q/h)^beta j_beta(q h)
 //                   (bessels0rr / dphrec, src/Integrals.f90:6662-6739,
-//                   7147-7184) or the h = 0 polynomial, weights it with
-//                   mult * e12(q,i,j) * e12(q,k,r) and accumulates S(q) with
+//                   7147-7184) or the h = 0 polynomial (h(beta) carrying the
+//                   quartet's multiplicity mult), weights it with
+//                   e12(q,i,j) * e12(q,k,r) and accumulates S(q) with
 //                   compensated sums.  M = true runs the large-argument
 //                   Bessel recursion in FP32 (mixed precision).
 //
@@ -268,7 +269,7 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
 #pragma unroll
                 for (int b = 0; b < DO; ++b)
                     if (lg == b) mine = coef(b, h[0], h[NA - 1]);
-                if (lg < DO) rec[lg] = mine;
+                if (lg < DO) rec[lg] = mine * (double)QE_MULT(ent);
                 if (lg == 0) {
                     tail();
                     A.flags[e] = (bal & gmask) ? 1 : 0;
@@ -302,7 +303,7 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
 #pragma unroll
                 for (int b = 0; b < DO; ++b)
                     if (tid == b) mine = coef(b, hr, grr);
-                rec[tid] = mine;
+                rec[tid] = mine * (double)QE_MULT(ent);
             }
             if (valid && tid == 0) {
                 tail();
@@ -686,7 +687,6 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
             const double h = uniform(rc[LL + 1]), invh = uniform(rc[LL + 2]);
             const uint64_t meta = (uint64_t)__double_as_longlong(uniform(rc[LL + 3]));
             const int pb = (int)(meta & 0xFFFFFu), pk = (int)((meta >> 20) & 0xFFFFFu);
-            const double mult = (double)((meta >> 40) & 15u);
             const int br = (int)((meta >> 44) & 1u);
             // both pair factors of the lane's q points, issued before the
             // series so that their latency hides behind it (loading them one
@@ -744,13 +744,12 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
                 double Fa;
                 const double F = f_eval<LL, MIXED, J2>(hs, cb, hf, hw, cbf, qv[s], qiv[s], h,
                                                        invh, br, sxs[s], cxs[s], Fa);
-                double v = mult * F;
-                v = v * ebv[s];
+                double v = F * ebv[s];
                 v = v * ekv[s];
                 Tb[s] += v;
                 // error scale of the FP32 terms: |weight| sum_b |h_b u_b|
                 if (MIXED) {
-                    const double e = fabs((mult * ebv[s]) * ekv[s]) * Fa;
+                    const double e = fabs(ebv[s] * ekv[s]) * Fa;
                     Sa2[s] = fma(e, e, Sa2[s]);
                 }
             }
@@ -826,7 +825,6 @@ __device__ __forceinline__ void far_chunk(const SeriesArgs& A, const double* __r
         const uint64_t meta = (uint64_t)__double_as_longlong(rc[LL + 3]);
         if (((meta >> 44) & 1u) || qmx * h < XDPH) continue;
         const int pb = (int)(meta & 0xFFFFFu), pk = (int)((meta >> 20) & 0xFFFFFu);
-        const double mult = (double)((meta >> 40) & 15u);
 #pragma unroll
         for (int s = 0; s < SQ; ++s) {
             const double x = qv[s] * h;
@@ -835,8 +833,7 @@ __device__ __forceinline__ void far_chunk(const SeriesArgs& A, const double* __r
             sincos_cw(x, sx, cx);
             double F = f_dphrec<LL>(rc, qv[s], h, x, sx, cx);
             if constexpr (J2) F = F / (qv[s] * qv[s]);
-            double v = mult * F;
-            v = v * A.e12[(int64_t)pb * A.nq + iqs[s]];
+            double v = F * A.e12[(int64_t)pb * A.nq + iqs[s]];
             v = v * A.e12[(int64_t)pk * A.nq + iqs[s]];
             kahan(S[s], Sc[s], v);
         }
@@ -903,7 +900,6 @@ __global__ void __launch_bounds__(WAVE) k_series_pts(SeriesArgs A, int64_t nidx)
             const double h = rc[LL + 1], invh = rc[LL + 2];
             const uint64_t meta = (uint64_t)__double_as_longlong(rc[LL + 3]);
             const int pb = (int)(meta & 0xFFFFFu), pk = (int)((meta >> 20) & 0xFFFFFu);
-            const double mult = (double)((meta >> 40) & 15u);
             const int br = (int)((meta >> 44) & 1u);
             const double ih2 = invh * invh;
             double cb[LL + 1];
@@ -921,8 +917,7 @@ __global__ void __launch_bounds__(WAVE) k_series_pts(SeriesArgs A, int64_t nidx)
                 if (!br) sincos_cw(x, sx, cx);
                 F = f_eval<LL, false, J2>(hs, cb, zf, zf, zf, q, qinv, h, invh, br, sx, cx, Fa);
             }
-            double v = mult * F;
-            v = v * A.e12[(int64_t)pb * A.nq + iq];
+            double v = F * A.e12[(int64_t)pb * A.nq + iq];
             v = v * A.e12[(int64_t)pk * A.nq + iq];
             kahan(S, Sc, v);
         }

@@ -37,8 +37,10 @@ struct EnumRow {
 
 // Per-quartet records written by the contraction and read by the series
 // kernel, in class-list order: the DO series coefficients (DO = LL + 1, j2
-// LL + 3), then h = |P_ket - P_bra|, 1/h (0 on the h = 0 branch) and the
-// packed pair indices and flags of HS_META.  The classes' records follow one
+// LL + 3) times the quartet's multiplicity (1, 2, 4 or 8: an exact scaling,
+// so the series needs no multiplication by it), then h = |P_ket - P_bra|,
+// 1/h (0 on the h = 0 branch) and the packed pair indices and flags of
+// HS_META.  The classes' records follow one
 // another (largest LL last); each class starts at the plan's offset for it.
 #define HS_ROW(DO) ((DO) + 3)
 // record word DO + 2: bra pair (bits 0-19), ket pair (20-39), multiplicity
