@@ -241,10 +241,10 @@ __global__ void __launch_bounds__(WAVE) k_series_adj(AdjointArgs J, const int32_
 // Per group and axis: PX (forward), PXH (d/dh), PXB (d/dX_AB), PXK (d/dX_CD);
 // each [3][T][D].  In LDS when they fit, else in a global scratch slice.
 template <int LL, bool GLOB>
-__global__ void __launch_bounds__(ContractShape<LL>::CT)
+__global__ void __launch_bounds__(AdjointShape<LL>::CT)
     k_contract_adj(AdjointArgs J, const int32_t* __restrict__ idx, int64_t nidx, int tmax) {
     constexpr int D = LL + 1;
-    constexpr int CT = ContractShape<LL>::CT, G = ContractShape<LL>::G, NG = ContractShape<LL>::NG;
+    constexpr int CT = AdjointShape<LL>::CT, G = AdjointShape<LL>::G, NG = AdjointShape<LL>::NG;
     constexpr int NW = CT / WAVE;
     constexpr int NV = 9;  // dh (3), dX_AB (3), dX_CD (3)
     const SeriesArgs& A = J.s;
@@ -537,7 +537,7 @@ size_t contract_adj_table_doubles(int LL, int Tmax) {
 int contract_adj_groups(int LL) {
     switch (LL) {
 #define C_(k) \
-    case k: return ContractShape<k>::NG;
+    case k: return AdjointShape<k>::NG;
         ALL_LL(C_)
 #undef C_
         default: return 1;
@@ -556,10 +556,10 @@ hipError_t launch_contract_adj(const AdjointArgs& j, int LL, int Tmax, const int
     case k:                                                                               \
         if (glob)                                                                         \
             hipLaunchKernelGGL((k_contract_adj<k, true>), dim3(blocks),                   \
-                               dim3(ContractShape<k>::CT), 0, s, j, idx, nidx, Tmax);     \
+                               dim3(AdjointShape<k>::CT), 0, s, j, idx, nidx, Tmax);     \
         else                                                                              \
             hipLaunchKernelGGL((k_contract_adj<k, false>), dim3(blocks),                  \
-                               dim3(ContractShape<k>::CT), lds, s, j, idx, nidx, Tmax);   \
+                               dim3(AdjointShape<k>::CT), lds, s, j, idx, nidx, Tmax);   \
         break;
         ALL_LL(C_)
 #undef C_

@@ -265,11 +265,16 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
             const uint64_t gmask =
                 G == WAVE ? ~0ull : (((1ull << G) - 1ull) << ((lane / G) * G));
             if (valid) {
-                double mine = 0.0;
+                // lane lg writes the coefficients lg, lg + G, ... (j2 records
+                // of the 4-lane classes have more than G)
 #pragma unroll
-                for (int b = 0; b < DO; ++b)
-                    if (lg == b) mine = coef(b, h[0], h[NA - 1]);
-                if (lg < DO) rec[lg] = mine * (double)QE_MULT(ent);
+                for (int b0 = 0; b0 < DO; b0 += G) {
+                    double mine = 0.0;
+#pragma unroll
+                    for (int b = b0; b < DO && b < b0 + G; ++b)
+                        if (lg == b - b0) mine = coef(b, h[0], h[NA - 1]);
+                    if (b0 + lg < DO) rec[b0 + lg] = mine * (double)QE_MULT(ent);
+                }
                 if (lg == 0) {
                     tail();
                     A.flags[e] = (bal & gmask) ? 1 : 0;

@@ -137,20 +137,32 @@ __constant__ ACoef kAcoef = make_acoef();
 }  // namespace
 
 // G lanes per quartet: small classes (a few GTO quadruples and exponent
-// tuples per quartet) share one wave between 64/G quartets (two quartets per
-// wave up to LL = 6: XY6's quartet stage 7.75 against 7.97 ms with one; LL =
-// 7 is slower with two); LL = 9 quartets take two waves and from LL = 10 on,
-// with thousands of quadruples, four (XY6: 7.50 ms; LL 9 on one wave 7.82,
-// on four 7.55; LL 8 on four 7.92; LL 10+ on two 8.11, on eight 8.10).
+// tuples per quartet) share one wave between 64/G quartets; LL = 9 quartets
+// take two waves and from LL = 10 on, with thousands of quadruples, four
+// (XY6: LL 9 on one wave 7.82 ms against 7.50, on four 7.55; LL 8 on four
+// 7.92; LL 10+ on two 8.11, on eight 8.10).  With the pipelined quadruple
+// loop the small classes gained from narrower groups: 4 lanes up to LL = 2,
+// 8 for LL 3-4, 16 for LL 5, 32 for LL 6, a wave for LL 7-8 (XY6 step 7.12
+// against 7.35 ms with 4, 8, 16 and 32 lanes up to LL 1, 2, 3 and 6; LL 6
+// on 16 lanes or LL 7 on 32: slower).
 template <int LL>
 struct ContractShape {
     static constexpr int CT = LL < 9 ? 64 : LL == 9 ? 128 : 256;
-    static constexpr int G = LL <= 1 ? 4 : LL == 2 ? 8 : LL == 3 ? 16 : LL <= 6 ? 32 : CT;
+    static constexpr int G = LL <= 2 ? 4 : LL <= 4 ? 8 : LL == 5 ? 16 : LL == 6 ? 32 : CT;
     static constexpr int NG = CT / G;
     // row stride of the px/py/pz tables: odd, so that rows gathered by the
     // lanes of a wave spread over all LDS banks (an even stride of 8 doubles
     // maps every row onto two bank positions)
     static constexpr int DP = (LL + 1) | 1;
+};
+
+// The contraction adjoint's lanes per quartet (its tables are four times
+// the forward's, so its groups stay wider).
+template <int LL>
+struct AdjointShape {
+    static constexpr int CT = LL < 9 ? 64 : LL == 9 ? 128 : 256;
+    static constexpr int G = LL <= 1 ? 4 : LL == 2 ? 8 : LL == 3 ? 16 : LL <= 6 ? 32 : CT;
+    static constexpr int NG = CT / G;
 };
 
 // Upward recurrence from j0, j1 (f_up64's form, on u_b = j_b (q/h)^b) keeps
