@@ -635,6 +635,18 @@ __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const doubl
     }
 }
 
+// Row pb of e12 ([pair][nq] doubles) as a raw buffer resource (gfx9 word 3:
+// 32-bit dword data, no swizzle); loads past its nq values return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t e12_row(const SeriesArgs& A, int pb) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(A.e12 + (int64_t)pb * A.nq), (short)0,
+                                             A.nq * 8, 0x00020000);
+}
+__device__ __forceinline__ double buf_ld64(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+    return __longlong_as_double((long long)(((unsigned long long)v.y << 32) | v.x));
+}
+
 // Chunk `chunk` of nchunk (the records of one wave) of a class whose records
 // are hsb (nidx of them), on q slice group ys of the points [qs0, qs1) of
 // the chunk of q; the sums go to partial row `chunk`.
@@ -648,11 +660,13 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
     __shared__ double phs[2 * NB];  // sin, cos of the records' phase steps
     const int lane = threadIdx.x;
     int iqs[SQ];
+    uint32_t qbyte[SQ];
     double qv[SQ], qiv[SQ], S[SQ], Sc[SQ], Sa2[SQ];
 #pragma unroll
     for (int s = 0; s < SQ; ++s) {
         const int iql = qs0 + (ys * SQ + s) * WAVE + lane;
         iqs[s] = A.q0 + (iql < qs1 ? iql : qs0);
+        qbyte[s] = (uint32_t)iqs[s] * 8u;
         qv[s] = A.q[iqs[s]];
         qiv[s] = qv[s] > 0.0 ? rcp64(qv[s]) : 0.0;
         S[s] = Sc[s] = Sa2[s] = 0.0;
@@ -696,13 +710,14 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
             // both pair factors of the lane's q points, issued before the
             // series so that their latency hides behind it (loading them one
             // record ahead measured slower: more registers live)
-            const double* eb = A.e12 + (int64_t)pb * A.nq;
-            const double* ek = A.e12 + (int64_t)pk * A.nq;
+            // (buffer loads: the pair's row as a scalar resource plus the
+            // lane's byte offset, no per-record vector address arithmetic)
+            const __amdgpu_buffer_rsrc_t eb = e12_row(A, pb), ek = e12_row(A, pk);
             double ebv[SQ], ekv[SQ];
 #pragma unroll
             for (int s = 0; s < SQ; ++s) {
-                ebv[s] = eb[iqs[s]];
-                ekv[s] = ek[iqs[s]];
+                ebv[s] = buf_ld64(eb, qbyte[s]);
+                ekv[s] = buf_ld64(ek, qbyte[s]);
             }
             const double ih2 = invh * invh;
             double cb[LL + 1];
