@@ -635,6 +635,13 @@ __device__ __forceinline__ double f_eval(const double (&hs)[LL + 1], const doubl
     }
 }
 
+// lane l's value of v, wave-uniform
+__device__ __forceinline__ double rdlane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 // Row pb of e12 ([pair][nq] doubles) as a raw buffer resource (gfx9 word 3:
 // 32-bit dword data, no swizzle); loads past its nq values return 0.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t e12_row(const SeriesArgs& A, int pb) {
@@ -655,9 +662,7 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
                                              int64_t nidx, int first, int64_t chunk,
                                              int64_t nchunk, int ys, int qs0, int qs1) {
     constexpr int R = HS_ROW(LL + 1);
-    constexpr int NB = 64;  // quartet records staged in LDS per batch
-    __shared__ double recs[NB * R];
-    __shared__ double phs[2 * NB];  // sin, cos of the records' phase steps
+    constexpr int NB = WAVE;  // records per batch (one phase step per lane)
     const int lane = threadIdx.x;
     int iqs[SQ];
     uint32_t qbyte[SQ];
@@ -671,40 +676,47 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
         qiv[s] = qv[s] > 0.0 ? rcp64(qv[s]) : 0.0;
         S[s] = Sc[s] = Sa2[s] = 0.0;
     }
-    // a contiguous chunk of the class's records per wave, staged NB at a time
-    // by coalesced loads; each record then is an LDS broadcast
+    // a contiguous chunk of the class's records per wave.  The records are
+    // wave-uniform: read by scalar loads (through the constant address space;
+    // the contraction wrote them in an earlier kernel), the next record's
+    // requested before the current one is evaluated, so that no vector
+    // register, LDS broadcast or readfirstlane carries them (XY6 series alone
+    // 3.04 -> 2.91 ms against records staged through LDS)
+    typedef const __attribute__((address_space(4))) double* crec_p;
     const int64_t k0 = chunk * nidx / nchunk, k1 = (chunk + 1) * nidx / nchunk;
     for (int64_t kb = k0; kb < k1; kb += NB) {
         const int nb = (int)(k1 - kb < NB ? k1 - kb : NB);
-        const double* src = hsb + kb * R;
-        __syncthreads();  // the previous batch is consumed
-        for (int t = lane; t < nb * R; t += WAVE) recs[t] = src[t];
-        __syncthreads();
         // the phase step sin, cos(64 dq h) between a lane's q slices on a
-        // uniform grid, one lane per staged record (0 on the h = 0 branch,
+        // uniform grid, lane j for record kb + j (0 on the h = 0 branch,
         // which uses no sine); once per record and wave rather than per lane
+        double psd = 0.0, pcd = 0.0;
         if (SQ > 1 && A.qlin && lane < nb) {
-            const double* r = recs + lane * R;
+            const double* r = hsb + (kb + lane) * R;
             const uint64_t m = (uint64_t)__double_as_longlong(r[LL + 3]);
-            double sd = 0.0, cd = 0.0;
-            if (!((m >> 44) & 1u)) sincos_cw((double)WAVE * A.dq * r[LL + 1], sd, cd);
-            phs[2 * lane] = sd;
-            phs[2 * lane + 1] = cd;
+            if (!((m >> 44) & 1u)) sincos_cw((double)WAVE * A.dq * r[LL + 1], psd, pcd);
         }
-        __syncthreads();
+        double nrec[R];  // the next record
+        {
+            const crec_p rn = (crec_p)(hsb + kb * R);
+#pragma unroll
+            for (int b = 0; b < R; ++b) nrec[b] = rn[b];
+        }
         // plain sums over the batch's records, added to S with compensation
         // once per batch
         double Tb[SQ];
 #pragma unroll
         for (int s = 0; s < SQ; ++s) Tb[s] = 0.0;
         for (int j = 0; j < nb; ++j) {
-            // the record is wave-uniform: kept in scalar registers
-            const double* rc = recs + j * R;
             double hs[LL + 1];
 #pragma unroll
-            for (int b = 0; b <= LL; ++b) hs[b] = uniform(rc[b]);
-            const double h = uniform(rc[LL + 1]), invh = uniform(rc[LL + 2]);
-            const uint64_t meta = (uint64_t)__double_as_longlong(uniform(rc[LL + 3]));
+            for (int b = 0; b <= LL; ++b) hs[b] = nrec[b];
+            const double h = nrec[LL + 1], invh = nrec[LL + 2];
+            const uint64_t meta = (uint64_t)__double_as_longlong(nrec[LL + 3]);
+            if (j + 1 < nb) {
+                const crec_p rn = (crec_p)(hsb + (kb + j + 1) * R);
+#pragma unroll
+                for (int b = 0; b < R; ++b) nrec[b] = rn[b];
+            }
             const int pb = (int)(meta & 0xFFFFFu), pk = (int)((meta >> 20) & 0xFFFFFu);
             const int br = (int)((meta >> 44) & 1u);
             // both pair factors of the lane's q points, issued before the
@@ -745,7 +757,7 @@ __device__ __forceinline__ void series_chunk(const SeriesArgs& A, const double* 
                     sxs[0] = sb;
                     cxs[0] = cbb;
                     if (off) sincos_cw(qv[0] * h, sxs[0], cxs[0]);
-                    const double sd = uniform(phs[2 * j]), cd = uniform(phs[2 * j + 1]);
+                    const double sd = rdlane(psd, j), cd = rdlane(pcd, j);
 #pragma unroll
                     for (int s = 1; s < SQ; ++s) {
                         const double ns = fma(sb, cd, cbb * sd), nc = fma(cbb, cd, -(sb * sd));
