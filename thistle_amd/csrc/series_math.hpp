@@ -22,15 +22,6 @@ __device__ __forceinline__ double wsum(double v) {
     return v;
 }
 
-// a wave-uniform value read into vector registers (an LDS broadcast), moved
-// to scalar registers
-__device__ __forceinline__ double uniform(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readfirstlane((int)b);
-    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 __device__ __forceinline__ void kahan(double& s, double& c, double v) {
 #pragma clang fp contract(off)
     double y = v - c;
