@@ -67,17 +67,46 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
     // grid-stride over the class list: neighbouring quartets go to
     // neighbouring workgroups, hence to different XCDs (measured faster on
     // XY6 than contiguous runs per workgroup or per XCD)
-    for (int64_t base = g0 * NG; base < nidx; base += gstep * NG) {
+    // The quartet's header, pairs and groups are loaded before the loop for
+    // the first group and at the end of an iteration for the next (XY6 step
+    // 6.86 -> 6.74 ms against loading them at the top of the loop body, with
+    // fewer VGPRs: 100 against 106 at LL 4).  Loading the next group's at
+    // the start of an iteration, beside the current one's table build, with
+    // 4096-16384 workgroups looping over the groups of LL <= 6: 6.72-6.77 ms
+    // against 6.68 on one box.
+    const int64_t stride = gstep * NG;
+    QHdr Hn;
+    PairInfo PBn, PKn;
+    GroupInfo Gin, Gjn, Gkn, Grn;
+    auto fetch_h = [&](int64_t b) {
+        const int64_t k = b + grp;
+        Hn = hdr[k < nidx ? k : b];
+    };
+    auto fetch_pg = [&]() {
+        const uint64_t en = QH_ENT(Hn.w);
+        PBn = A.P[pair_index(QE_GI(en), QE_GJ(en), A.ng)];
+        PKn = A.P[pair_index(QE_GK(en), QE_GR(en), A.ng)];
+        Gin = A.G[QE_GI(en)];
+        Gjn = A.G[QE_GJ(en)];
+        Gkn = A.G[QE_GK(en)];
+        Grn = A.G[QE_GR(en)];
+    };
+    if (g0 * NG < nidx) {
+        fetch_h(g0 * NG);
+        fetch_pg();
+    }
+    for (int64_t base = g0 * NG; base < nidx; base += stride) {
         const int64_t kq = base + grp;
         const bool valid = kq < nidx;
-        const QHdr H = hdr[valid ? kq : base];
+        const bool more = base + stride < nidx;
+        const QHdr H = Hn;
+        const PairInfo PB = PBn, PK = PKn;
+        const GroupInfo Gi = Gin, Gj = Gjn, Gk = Gkn, Gr = Grn;
         const int64_t e = H.e;
         const uint64_t ent = QH_ENT(H.w);
         const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
         const int br = QE_BR(ent);
         const int pb = pair_index(gi, gj, A.ng), pk = pair_index(gk, gr, A.ng);
-        const PairInfo PB = A.P[pb], PK = A.P[pk];
-        const GroupInfo Gi = A.G[gi], Gj = A.G[gj], Gk = A.G[gk], Gr = A.G[gr];
         const int Li = Gi.L, Lj = Gj.L, Lk = Gk.L, Lr = Gr.L;
         const int T = valid ? (Li + 1) * (Lj + 1) * (Lk + 1) * (Lr + 1) : 0;
         const double rx = PK.Px - PB.Px, ry = PK.Py - PB.Py, rz = PK.Pz - PB.Pz;
@@ -317,6 +346,10 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
                 for (int w = 0; w < NW; ++w) f |= anyw[w];
                 A.flags[e] = f ? 1 : 0;
             }
+        }
+        if (more) {
+            fetch_h(base + stride);
+            fetch_pg();
         }
     }
 }
