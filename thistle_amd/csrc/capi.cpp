@@ -695,8 +695,8 @@ int run_tables_and_enum(ThistlePlan* pl, bool timing) {
     HCHECK(launch_enumerate(pl->rows.as<EnumRow>(), pl->nrows, pl->P.as<PairInfo>(), pl->ng,
                             pl->cutoffcentre, pl->qlist.as<uint64_t>(), pl->stream));
     HCHECK(launch_qhdr(pl->cls.as<int32_t>(), pl->cls_off[17], pl->qlist.as<uint64_t>(),
-                       pl->G.as<GroupInfo>(), pl->comboff.as<int64_t>(), pl->nsig,
-                       pl->qhdr.as<QHdr>(), pl->stream));
+                       pl->G.as<GroupInfo>(), pl->P.as<PairInfo>(), pl->ng,
+                       pl->comboff.as<int64_t>(), pl->nsig, pl->qhdr.as<QHdr>(), pl->stream));
     if (timing) HCHECK(hipEventRecord(pl->ev[2], pl->stream));
     return THISTLE_OK;
 }

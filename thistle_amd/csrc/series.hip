@@ -67,25 +67,25 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
     // grid-stride over the class list: neighbouring quartets go to
     // neighbouring workgroups, hence to different XCDs (measured faster on
     // XY6 than contiguous runs per workgroup or per XCD)
-    // The quartet's header, pairs and groups are loaded before the loop for
-    // the first group and at the end of an iteration for the next (XY6 step
-    // 6.86 -> 6.74 ms against loading them at the top of the loop body, with
-    // fewer VGPRs: 100 against 106 at LL 4).  Loading the next group's at
-    // the start of an iteration, beside the current one's table build, with
-    // 4096-16384 workgroups looping over the groups of LL <= 6: 6.72-6.77 ms
-    // against 6.68 on one box.
+    // The quartet's header (and, for the elastic kinds, its groups) are
+    // loaded before the loop for the first group and at the end of an
+    // iteration for the next (XY6 step 6.86 -> 6.74 ms against loading them
+    // at the top of the loop body, with fewer VGPRs: 100 against 106 at LL
+    // 4).  Loading the next group's at the start of an iteration, beside the
+    // current one's table build, with 4096-16384 workgroups looping over the
+    // groups of LL <= 6: 6.72-6.77 ms against 6.68 on one box.  The header
+    // carries the pair centres' difference, the pair block offsets and the
+    // shells' L, so the total kinds read no pair or group record.
     const int64_t stride = gstep * NG;
     QHdr Hn;
-    PairInfo PBn, PKn;
-    GroupInfo Gin, Gjn, Gkn, Grn;
+    GroupInfo Gin{}, Gjn{}, Gkn{}, Grn{};
     auto fetch_h = [&](int64_t b) {
         const int64_t k = b + grp;
         Hn = hdr[k < nidx ? k : b];
     };
     auto fetch_pg = [&]() {
+        if (!A.elastic) return;
         const uint64_t en = QH_ENT(Hn.w);
-        PBn = A.P[pair_index(QE_GI(en), QE_GJ(en), A.ng)];
-        PKn = A.P[pair_index(QE_GK(en), QE_GR(en), A.ng)];
         Gin = A.G[QE_GI(en)];
         Gjn = A.G[QE_GJ(en)];
         Gkn = A.G[QE_GK(en)];
@@ -100,16 +100,15 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
         const bool valid = kq < nidx;
         const bool more = base + stride < nidx;
         const QHdr H = Hn;
-        const PairInfo PB = PBn, PK = PKn;
         const GroupInfo Gi = Gin, Gj = Gjn, Gk = Gkn, Gr = Grn;
         const int64_t e = H.e;
         const uint64_t ent = QH_ENT(H.w);
         const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
         const int br = QE_BR(ent);
         const int pb = pair_index(gi, gj, A.ng), pk = pair_index(gk, gr, A.ng);
-        const int Li = Gi.L, Lj = Gj.L, Lk = Gk.L, Lr = Gr.L;
+        const int Li = QH_LA(H.ob), Lj = QH_LB(H.ob), Lk = QH_LA(H.ok), Lr = QH_LB(H.ok);
         const int T = valid ? (Li + 1) * (Lj + 1) * (Lk + 1) * (Lr + 1) : 0;
-        const double rx = PK.Px - PB.Px, ry = PK.Py - PB.Py, rz = PK.Pz - PB.Pz;
+        const double rx = H.rx, ry = H.ry, rz = H.rz;
         const double hx = br ? 0.0 : rx, hy = br ? 0.0 : ry, hz = br ? 0.0 : rz;
 
         // The quadruple loop is software-pipelined: the lane's first table
@@ -179,7 +178,7 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
                 out[m] = s;
             }
         }
-        const ZView ZV = zview(A, pb, pk, PB.off, PK.off, Gi, Gj, Gk, Gr);
+        const ZView ZV = zview(A, pb, pk, QH_OFF(H.ob), QH_OFF(H.ok), Gi, Gj, Gk, Gr);
         double zc = lg < nq4 ? zget(A, ZV, wc) : 0.0;
         uint64_t wn = lg + G < nq4 ? qt[lg + G] : 0ull;
         __syncthreads();
@@ -359,26 +358,36 @@ __device__ __forceinline__ void contract_groups(const SeriesArgs& A, double* __r
 // quadruple-table start.
 __global__ void __launch_bounds__(256)
     k_qhdr(const int32_t* __restrict__ cls, int64_t n, const uint64_t* __restrict__ qlist,
-           const GroupInfo* __restrict__ G, const int64_t* __restrict__ comboff, int S,
-           QHdr* __restrict__ out) {
+           const GroupInfo* __restrict__ G, const PairInfo* __restrict__ P, int ng,
+           const int64_t* __restrict__ comboff, int S, QHdr* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     const int32_t e = cls[k];
     const uint64_t ent = qlist[e];
-    const GroupInfo Gi = G[QE_GI(ent)], Gj = G[QE_GJ(ent)], Gk = G[QE_GK(ent)], Gr = G[QE_GR(ent)];
+    const int gi = QE_GI(ent), gj = QE_GJ(ent), gk = QE_GK(ent), gr = QE_GR(ent);
+    const GroupInfo Gi = G[gi], Gj = G[gj], Gk = G[gk], Gr = G[gr];
+    const PairInfo PB = P[pair_index(gi, gj, ng)], PK = P[pair_index(gk, gr, ng)];
     const uint64_t nq4 = (uint64_t)(Gi.count * Gj.count * Gk.count * Gr.count);
     QHdr h;
     h.w = ent | (nq4 << 45);
     h.e = e;
     h.qoff = (uint32_t)comboff[((Gi.pad * S + Gj.pad) * S + Gk.pad) * S + Gr.pad];
+    h.rx = PK.Px - PB.Px;
+    h.ry = PK.Py - PB.Py;
+    h.rz = PK.Pz - PB.Pz;
+    // (pair offsets below 2^24: checked where Z is allocated; the elastic
+    // kinds read only the L fields)
+    h.ob = ((uint32_t)PB.off & 0xFFFFFFu) | ((uint32_t)Gi.L << 24) | ((uint32_t)Gj.L << 28);
+    h.ok = ((uint32_t)PK.off & 0xFFFFFFu) | ((uint32_t)Gk.L << 24) | ((uint32_t)Gr.L << 28);
     out[k] = h;
 }
 
 hipError_t launch_qhdr(const int32_t* cls, int64_t n, const uint64_t* qlist, const GroupInfo* G,
-                       const int64_t* comboff, int nsig, QHdr* out, hipStream_t s) {
+                       const PairInfo* P, int ng, const int64_t* comboff, int nsig, QHdr* out,
+                       hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_qhdr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cls, n, qlist,
-                       G, comboff, nsig, out);
+                       G, P, ng, comboff, nsig, out);
     return hipGetLastError();
 }
 

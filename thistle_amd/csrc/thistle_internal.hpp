@@ -211,15 +211,22 @@ struct SeriesArgs {
 
 // Cross-frame work tables (batched frames): one entry per frame that has
 // quartets in the class, in increasing first block blk0.
-// A quartet of a class list, as the contraction reads it: one 16-byte load
-// instead of the chain class list -> quartet entry -> groups -> table start.
+// A quartet of a class list, as the contraction reads it: three 16-byte
+// loads instead of the chain class list -> quartet entry -> pairs and groups
+// -> table start.
 struct alignas(16) QHdr {
-    uint64_t w;     // quartet entry (bits 0-44) | GTO quadruples (bits 45-60)
-    int32_t e;      // position in the quartet list
-    uint32_t qoff;  // start of its GTO-quadruple table in quadtab
+    uint64_t w;       // quartet entry (bits 0-44) | GTO quadruples (bits 45-60)
+    int32_t e;        // position in the quartet list
+    uint32_t qoff;    // start of its GTO-quadruple table in quadtab
+    double rx, ry, rz;  // ket pair centre - bra pair centre
+    uint32_t ob;      // bra pair block offset (bits 0-23) | L_i << 24 | L_j << 28
+    uint32_t ok;      // ket pair block offset (bits 0-23) | L_k << 24 | L_r << 28
 };
 #define QH_NQ4(w) ((uint32_t)((w) >> 45))
 #define QH_ENT(w) ((w) & ((1ull << 45) - 1ull))
+#define QH_OFF(o) ((int)((o) & 0xFFFFFFu))
+#define QH_LA(o) ((int)(((o) >> 24) & 15u))
+#define QH_LB(o) ((int)((o) >> 28))
 
 struct ClassWork {           // contraction: one block per quartet group
     int64_t blk0;
@@ -315,7 +322,8 @@ hipError_t launch_contract(const SeriesArgs& a, int LL, int Tmax, const QHdr* hd
                            int64_t nidx, int blocks, hipStream_t s);
 // the headers of a class list cls (n quartets, every class in turn)
 hipError_t launch_qhdr(const int32_t* cls, int64_t n, const uint64_t* qlist, const GroupInfo* G,
-                       const int64_t* comboff, int nsig, QHdr* out, hipStream_t s);
+                       const PairInfo* P, int ng, const int64_t* comboff, int nsig, QHdr* out,
+                       hipStream_t s);
 // k_series_far<LL>: the q h >= XDPH points of the class (same launch shape
 // and wave partition as launch_series, added to its partial rows)
 hipError_t launch_series_far(const SeriesArgs& a, int LL, int64_t nidx, int waves,
